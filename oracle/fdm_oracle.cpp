@@ -1,0 +1,988 @@
+// fdm_oracle.cpp -- CPU FP64 oracle for the fullmag FDM LLG time step.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP library is
+// compared against; it is never linked into, loaded by, or called from the
+// product library (fullmag_amd/).  Only tests/, __graft_entry__.smoke() and
+// the cpu_baseline leg of bench.py load it.
+//
+// It restates, in plain C++17, the CPU reference engine of fullmag:
+//   crates/fullmag-engine/src/lib.rs        (fields, energies, integrators)
+//   crates/fullmag-engine/src/vector.rs     (vector helpers, normalisation)
+//   crates/fullmag-fdm-demag/src/newell.rs  (Newell-Williams-Dunlop tensor)
+//   crates/fullmag-plan/src/lib.rs          (seeded random m0 generator)
+// The reference engine is Rust and uses rustfft 6.4.1 (not vendored, no Rust
+// toolchain here), so the FFT below is this file's own mixed-radix Stockham
+// FFT with Bluestein for large prime factors; its output is pinned against
+// the DFT definition (brute force), numpy.fft fixtures and Parseval in tests.
+//
+// Pinning status of each part (how a plausible mistake would be caught):
+//   newell_f/newell_g/ldia/asymptotic/placement  pinned by the reference's own
+//       KATs (newell.rs:676-907): trace=1, N_xx=1/3 for a cube, N_xy(1,1,0),
+//       N_xy(2,2,0), N_xy(5,5,0), odd/even symmetry, dipole asymptotics.
+//   fft / fft3                                   pinned by brute-force DFT and
+//       numpy.fft golden vectors, inverse(forward)=N*identity.
+//   exchange field                               pinned by the KATs at
+//       lib.rs:3001-3028 ([2,-2,0] and [1,-1,0]) and summation by parts.
+//   demag field                                  pinned by the closed form for
+//       a single cube (H = -M/3), extended thin film (H_z -> -M_z), E_d >= 0.
+//   LLG rhs / Heun / RK4 / RK23 / DP45 / ABM3    pinned by the analytic
+//       single-spin Zeeman precession solution (closed form), the formal
+//       order of each method on that solution, |m| = 1 preservation.
+//   random unit vectors                          bit-exact integer recurrence,
+//       checked against hand-computed xorshift64 states.
+//   per-cell Ms                                  extension with no reference
+//       counterpart: reduces to the uniform case when all Ms_i are equal.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cd = std::complex<double>;
+using V3 = std::array<double, 3>;
+
+constexpr double PI = 3.14159265358979323846;
+// lib.rs:25
+constexpr double MU0 = 4.0 * PI * 1e-7;
+
+// ---------------------------------------------------------------------------
+// vector.rs:11-76
+// ---------------------------------------------------------------------------
+inline V3 add(const V3 &a, const V3 &b) { return {a[0] + b[0], a[1] + b[1], a[2] + b[2]}; }
+inline V3 sub(const V3 &a, const V3 &b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline V3 scale(const V3 &a, double f) { return {a[0] * f, a[1] * f, a[2] * f}; }
+inline double dot(const V3 &a, const V3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline V3 cross(const V3 &a, const V3 &b) {
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+inline double norm(const V3 &a) { return std::sqrt(dot(a, a)); }
+// vector.rs:11-19: zero vectors (inactive cells) stay zero.
+inline V3 normalized(const V3 &v) {
+    double n = norm(v);
+    if (n <= 0.0) return {0.0, 0.0, 0.0};
+    return scale(v, 1.0 / n);
+}
+double max_norm(const std::vector<V3> &v) {
+    double m = 0.0;
+    for (const auto &x : v) m = std::max(m, norm(x));
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// FFT: forward is unnormalised exp(-2 pi i jk/N), inverse unnormalised
+// exp(+2 pi i jk/N), matching rustfft's conventions used at lib.rs:343-386.
+// Stockham autosort, mixed radix 2/3/4/5 plus generic small primes; prime
+// factors above 61 go through Bluestein on a power-of-two length.
+// ---------------------------------------------------------------------------
+struct FftPlan {
+    int n = 0;
+    std::vector<int> radices;
+    std::vector<cd> tw;          // tw[t] = exp(-2 pi i t / n), long-double accurate
+    bool bluestein = false;
+    int m = 0;                   // Bluestein convolution length
+    std::vector<cd> chirp;       // exp(-pi i j^2 / n)
+    std::vector<cd> bfft;        // forward FFT of the conjugate chirp (length m)
+    std::shared_ptr<FftPlan> sub;
+};
+
+std::shared_ptr<FftPlan> make_plan(int n);
+void fft_exec(const FftPlan &p, cd *data, bool inverse, cd *scratch);
+
+std::vector<int> factorize(int n, bool &has_large) {
+    std::vector<int> r;
+    has_large = false;
+    while (n % 4 == 0) { r.push_back(4); n /= 4; }
+    while (n % 2 == 0) { r.push_back(2); n /= 2; }
+    for (int p = 3; p * p <= n; p += 2) {
+        while (n % p == 0) { r.push_back(p); n /= p; if (p > 61) has_large = true; }
+    }
+    if (n > 1) { r.push_back(n); if (n > 61) has_large = true; }
+    return r;
+}
+
+std::vector<cd> roots(int n) {
+    std::vector<cd> w(n);
+    for (int t = 0; t < n; ++t) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+        w[t] = cd((double)cosl(a), (double)sinl(a));
+    }
+    return w;
+}
+
+std::shared_ptr<FftPlan> make_plan(int n) {
+    auto p = std::make_shared<FftPlan>();
+    p->n = n;
+    bool large = false;
+    p->radices = factorize(n, large);
+    p->tw = roots(n);
+    if (large) {
+        p->bluestein = true;
+        int m = 1;
+        while (m < 2 * n - 1) m <<= 1;
+        p->m = m;
+        p->chirp.resize(n);
+        for (int j = 0; j < n; ++j) {
+            // exact reduction of j^2 modulo 2n keeps the chirp angle small
+            long long q = ((long long)j * j) % (2LL * n);
+            long double a = -3.141592653589793238462643383279502884L * (long double)q / (long double)n;
+            p->chirp[j] = cd((double)cosl(a), (double)sinl(a));
+        }
+        p->sub = make_plan(m);
+        std::vector<cd> b(m, cd(0, 0));
+        b[0] = std::conj(p->chirp[0]);
+        for (int j = 1; j < n; ++j) b[j] = b[m - j] = std::conj(p->chirp[j]);
+        std::vector<cd> scratch(m);
+        p->bfft = b;
+        // forward transform of b
+        fft_exec(*p->sub, p->bfft.data(), false, scratch.data());
+    }
+    return p;
+}
+
+std::shared_ptr<FftPlan> get_plan(int n) {
+    static std::mutex mu;
+    static std::map<int, std::shared_ptr<FftPlan>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(n);
+    if (it != cache.end()) return it->second;
+    auto p = make_plan(n);
+    cache[n] = p;
+    return p;
+}
+
+// Stockham DIT pass structure: for radix R and span Ns (product of radices
+// already applied), butterfly j reads x[j + r*N/R], twiddles by w^(r*k*N/(Ns R))
+// with k = j mod Ns, applies the R-point DFT and writes
+// y[(j/Ns)*Ns*R + k + r*Ns].  Output ends in natural order.
+void fft_exec(const FftPlan &p, cd *data, bool inverse, cd *scratch) {
+    const int n = p.n;
+    if (n <= 1) return;
+    if (p.bluestein) {
+        const int m = p.m;
+        std::vector<cd> a(m, cd(0, 0)), s(m);
+        for (int j = 0; j < n; ++j) {
+            cd c = inverse ? std::conj(p.chirp[j]) : p.chirp[j];
+            a[j] = data[j] * c;
+        }
+        fft_exec(*p.sub, a.data(), false, s.data());
+        // conv with conj chirp (forward) or chirp (inverse): FFT(b) for the
+        // inverse direction is the conjugate-reversed spectrum of the forward one
+        for (int k = 0; k < m; ++k) {
+            cd b = inverse ? std::conj(p.bfft[(m - k) % m]) : p.bfft[k];
+            a[k] *= b;
+        }
+        fft_exec(*p.sub, a.data(), true, s.data());
+        for (int k = 0; k < n; ++k) {
+            cd c = inverse ? std::conj(p.chirp[k]) : p.chirp[k];
+            data[k] = a[k] / (double)m * c;
+        }
+        return;
+    }
+    cd *x = data, *y = scratch;
+    int ns = 1;
+    for (int R : p.radices) {
+        const int nr = n / R;
+        const int stride_tw = n / (ns * R);
+        for (int j = 0; j < nr; ++j) {
+            const int k = j % ns;
+            cd v[64];
+            cd vv[64];
+            for (int r = 0; r < R; ++r) {
+                cd a = x[j + r * nr];
+                int t = (int)(((long long)r * k * stride_tw) % n);
+                cd w = inverse ? std::conj(p.tw[t]) : p.tw[t];
+                v[r] = (r == 0 || k == 0) ? a : a * w;
+            }
+            // R-point DFT with exact table roots
+            if (R == 2) {
+                vv[0] = v[0] + v[1];
+                vv[1] = v[0] - v[1];
+            } else if (R == 4) {
+                cd a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = v[1] - v[3];
+                // -i (forward) or +i (inverse) times a3
+                cd ja3 = inverse ? cd(-a3.imag(), a3.real()) : cd(a3.imag(), -a3.real());
+                vv[0] = a0 + a2;
+                vv[1] = a1 + ja3;
+                vv[2] = a0 - a2;
+                vv[3] = a1 - ja3;
+            } else
+            for (int q = 0; q < R; ++q) {
+                cd acc = v[0];
+                for (int r = 1; r < R; ++r) {
+                    int t = (int)(((long long)r * q % R) * (n / R));
+                    cd w = inverse ? std::conj(p.tw[t]) : p.tw[t];
+                    acc += v[r] * w;
+                }
+                vv[q] = acc;
+            }
+            const int base = (j / ns) * ns * R + k;
+            for (int q = 0; q < R; ++q) y[base + q * ns] = vv[q];
+        }
+        std::swap(x, y);
+        ns *= R;
+    }
+    if (x != data) std::memcpy(data, x, sizeof(cd) * n);
+}
+
+
+void fft1(cd *data, int n, bool inverse) {
+    auto p = get_plan(n);
+    std::vector<cd> s(n);
+    fft_exec(*p, data, inverse, s.data());
+}
+
+// lib.rs:2841-2843
+inline size_t padded_index(size_t nx, size_t ny, size_t x, size_t y, size_t z) {
+    return x + nx * (y + ny * z);
+}
+
+// lib.rs:2768-2812: x lines, then y lines, then z lines.
+void fft3_core(cd *data, int nx, int ny, int nz, bool inverse) {
+    auto px = get_plan(nx), py = get_plan(ny), pz = get_plan(nz);
+    std::vector<cd> s(std::max({nx, ny, nz}));
+    std::vector<cd> line(std::max(ny, nz));
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) fft_exec(*px, data + padded_index(nx, ny, 0, y, z), inverse, s.data());
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            for (int y = 0; y < ny; ++y) line[y] = data[padded_index(nx, ny, x, y, z)];
+            fft_exec(*py, line.data(), inverse, s.data());
+            for (int y = 0; y < ny; ++y) data[padded_index(nx, ny, x, y, z)] = line[y];
+        }
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            for (int z = 0; z < nz; ++z) line[z] = data[padded_index(nx, ny, x, y, z)];
+            fft_exec(*pz, line.data(), inverse, s.data());
+            for (int z = 0; z < nz; ++z) data[padded_index(nx, ny, x, y, z)] = line[z];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Newell-Williams-Dunlop tensor, newell.rs:21-519
+// ---------------------------------------------------------------------------
+
+// newell.rs:21-61
+double newell_f(double x, double y, double z) {
+    double x2 = x * x, y2 = y * y, z2 = z * z;
+    double r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    double r = std::sqrt(r2);
+    double result = (2.0 * x2 - y2 - z2) * r / 6.0;
+    double rxz2 = x2 + z2;
+    if (rxz2 > 1e-300) {
+        double arg = 2.0 * y * (y + r) / rxz2;
+        if (arg > -1.0) result += y * (z2 - x2) / 4.0 * std::log1p(arg);
+    }
+    double rxy2 = x2 + y2;
+    if (rxy2 > 1e-300) {
+        double arg = 2.0 * z * (z + r) / rxy2;
+        if (arg > -1.0) result += z * (y2 - x2) / 4.0 * std::log1p(arg);
+    }
+    if (std::fabs(x) > 1e-300) result -= x * y * z * std::atan(y * z / (x * r));
+    return result;
+}
+
+// newell.rs:63-123
+double newell_g(double x, double y, double z) {
+    double x2 = x * x, y2 = y * y, z2 = z * z;
+    double r2 = x2 + y2 + z2;
+    if (r2 < 1e-300) return 0.0;
+    double r = std::sqrt(r2);
+    double result = -x * y * r / 3.0;
+    double rxy2 = x2 + y2;
+    if (rxy2 > 1e-300) {
+        double arg = 2.0 * z * (z + r) / rxy2;
+        if (arg > -1.0) result += x * y * z * std::log1p(arg) / 2.0;
+    }
+    double ryz2 = y2 + z2;
+    if (ryz2 > 1e-300) {
+        double arg = 2.0 * x * (x + r) / ryz2;
+        if (arg > -1.0) result += y * (3.0 * z2 - y2) * std::log1p(arg) / 12.0;
+    }
+    double rxz2 = x2 + z2;
+    if (rxz2 > 1e-300) {
+        double arg = 2.0 * y * (y + r) / rxz2;
+        if (arg > -1.0) result += x * (3.0 * z2 - x2) * std::log1p(arg) / 12.0;
+    }
+    if (std::fabs(z) > 1e-300) result -= z2 * z / 6.0 * std::atan(x * y / (z * r));
+    if (std::fabs(y) > 1e-300) result -= y2 * z / 2.0 * std::atan(x * z / (y * r));
+    if (std::fabs(x) > 1e-300) result -= x2 * z / 2.0 * std::atan(y * z / (x * r));
+    return result;
+}
+
+// newell.rs:133-146 (Kahan-Neumaier)
+double kahan_sum(const double *terms, int n) {
+    double sum = 0.0, comp = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double val = terms[i];
+        double t = sum + val;
+        if (std::fabs(sum) >= std::fabs(val)) comp += (sum - t) + val;
+        else comp += (val - t) + sum;
+        sum = t;
+    }
+    return sum + comp;
+}
+
+// newell.rs:150-203: 27-point stencil with weights 8/-4/2/-1.
+double ldia(size_t i, size_t j, size_t k, const std::vector<double> &f, size_t sx, size_t sy,
+            double hx, double hy, double hz) {
+    i += 1; j += 1; k += 1;
+    auto id = [&](size_t a, size_t b, size_t c) { return c * sy * sx + b * sx + a; };
+    const double t[27] = {
+        8.0 * f[id(i, j, k)],
+        -4.0 * f[id(i + 1, j, k)], -4.0 * f[id(i - 1, j, k)],
+        -4.0 * f[id(i, j + 1, k)], -4.0 * f[id(i, j - 1, k)],
+        -4.0 * f[id(i, j, k + 1)], -4.0 * f[id(i, j, k - 1)],
+        2.0 * f[id(i - 1, j - 1, k)], 2.0 * f[id(i - 1, j + 1, k)],
+        2.0 * f[id(i + 1, j - 1, k)], 2.0 * f[id(i + 1, j + 1, k)],
+        2.0 * f[id(i - 1, j, k - 1)], 2.0 * f[id(i - 1, j, k + 1)],
+        2.0 * f[id(i + 1, j, k - 1)], 2.0 * f[id(i + 1, j, k + 1)],
+        2.0 * f[id(i, j - 1, k - 1)], 2.0 * f[id(i, j - 1, k + 1)],
+        2.0 * f[id(i, j + 1, k - 1)], 2.0 * f[id(i, j + 1, k + 1)],
+        -f[id(i - 1, j - 1, k - 1)], -f[id(i - 1, j - 1, k + 1)],
+        -f[id(i - 1, j + 1, k - 1)], -f[id(i + 1, j - 1, k - 1)],
+        -f[id(i - 1, j + 1, k + 1)], -f[id(i + 1, j - 1, k + 1)],
+        -f[id(i + 1, j + 1, k - 1)], -f[id(i + 1, j + 1, k + 1)],
+    };
+    return kahan_sum(t, 27) / (4.0 * PI * hx * hy * hz);
+}
+
+// newell.rs:212-226
+double asymptotic_nxx(double x, double y, double z, double vol) {
+    double r2 = x * x + y * y + z * z;
+    double r = std::sqrt(r2);
+    double r3 = r2 * r;
+    return (1.0 / r3 - 3.0 * x * x / (r3 * r2)) / (4.0 * PI) * vol;
+}
+double asymptotic_nxy(double x, double y, double z, double vol) {
+    double r2 = x * x + y * y + z * z;
+    double r = std::sqrt(r2);
+    double r5 = r2 * r2 * r;
+    return -3.0 * x * y / (4.0 * PI * r5) * vol;
+}
+
+constexpr size_t ASYMPTOTIC_DISTANCE = 40;  // newell.rs:229
+
+// newell.rs:254-519: six real-space components on the (2nx,2ny,2nz) grid.
+void compute_newell_kernels(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz,
+                            std::vector<double> out[6]) {
+    const size_t px = 2 * nx, py = 2 * ny, pz = 2 * nz;
+    const size_t plen = px * py * pz;
+    const size_t fsx = std::min(nx, ASYMPTOTIC_DISTANCE) + 2;
+    const size_t fsy = std::min(ny, ASYMPTOTIC_DISTANCE) + 2;
+    const size_t fsz = std::min(nz, ASYMPTOTIC_DISTANCE) + 2;
+    const size_t flen = fsx * fsy * fsz;
+    std::vector<double> fxx(flen), fyy(flen), fzz(flen), gxy(flen), gxz(flen), gyz(flen);
+#pragma omp parallel for schedule(dynamic)
+    for (long kk = 0; kk < (long)fsz; ++kk) {
+        for (size_t j = 0; j < fsy; ++j)
+            for (size_t i = 0; i < fsx; ++i) {
+                double x = (double)((long)i - 1) * dx;
+                double y = (double)((long)j - 1) * dy;
+                double z = (double)(kk - 1) * dz;
+                size_t id = (size_t)kk * fsx * fsy + j * fsx + i;
+                fxx[id] = newell_f(x, y, z);
+                fyy[id] = newell_f(y, x, z);
+                fzz[id] = newell_f(z, y, x);
+                gxy[id] = newell_g(x, y, z);
+                gxz[id] = newell_g(x, z, y);
+                gyz[id] = newell_g(y, z, x);
+            }
+    }
+    for (int c = 0; c < 6; ++c) out[c].assign(plen, 0.0);
+#pragma omp parallel for schedule(dynamic)
+    for (long kl = 0; kl < (long)nz; ++kl) {
+        size_t k = (size_t)kl;
+        for (size_t j = 0; j < ny; ++j)
+            for (size_t i = 0; i < nx; ++i) {
+                double nxx, nyy, nzz, nxy, nxz, nyz;
+                size_t dist2 = i * i + j * j + k * k;
+                bool asym = i >= ASYMPTOTIC_DISTANCE || j >= ASYMPTOTIC_DISTANCE || k >= ASYMPTOTIC_DISTANCE ||
+                            dist2 >= ASYMPTOTIC_DISTANCE * ASYMPTOTIC_DISTANCE;
+                if (asym) {
+                    double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz;
+                    double vol = dx * dy * dz;
+                    nxx = asymptotic_nxx(x, y, z, vol);
+                    nyy = asymptotic_nxx(y, x, z, vol);
+                    nzz = asymptotic_nxx(z, y, x, vol);
+                    nxy = asymptotic_nxy(x, y, z, vol);
+                    nxz = asymptotic_nxy(x, z, y, vol);
+                    nyz = asymptotic_nxy(y, z, x, vol);
+                } else {
+                    nxx = ldia(i, j, k, fxx, fsx, fsy, dx, dy, dz);
+                    nyy = ldia(i, j, k, fyy, fsx, fsy, dy, dx, dz);
+                    nzz = ldia(i, j, k, fzz, fsx, fsy, dz, dy, dx);
+                    nxy = ldia(i, j, k, gxy, fsx, fsy, dx, dy, dz);
+                    nxz = ldia(i, j, k, gxz, fsx, fsy, dx, dz, dy);
+                    nyz = ldia(i, j, k, gyz, fsx, fsy, dy, dz, dx);
+                }
+                size_t xs_i[2] = {i, px - i}, ys_i[2] = {j, py - j}, zs_i[2] = {k, pz - k};
+                double xs_s[2] = {1.0, -1.0}, ys_s[2] = {1.0, -1.0}, zs_s[2] = {1.0, -1.0};
+                int nxs = i == 0 ? 1 : 2, nys = j == 0 ? 1 : 2, nzs = k == 0 ? 1 : 2;
+                for (int a = 0; a < nxs; ++a)
+                    for (int b = 0; b < nys; ++b)
+                        for (int c = 0; c < nzs; ++c) {
+                            size_t p = zs_i[c] * py * px + ys_i[b] * px + xs_i[a];
+                            double sx = xs_s[a], sy = ys_s[b], sz = zs_s[c];
+                            out[0][p] = nxx;
+                            out[1][p] = nyy;
+                            out[2][p] = nzz;
+                            out[3][p] = nxy * sx * sy;
+                            out[4][p] = nxz * sx * sz;
+                            out[5][p] = nyz * sy * sz;
+                        }
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Engine: ExchangeLlgProblem + state (lib.rs:869-2700)
+// ---------------------------------------------------------------------------
+enum Integrator { HEUN = 1, DP45 = 2, ABM3 = 3, RK4 = 4, RK23 = 5 };
+
+struct Report {
+    double time_seconds, dt_used, suggested_next_dt;
+    double e_ex, e_demag, e_ext, e_total;
+    double max_h_eff, max_h_demag, max_rhs;
+};
+
+struct Problem {
+    size_t nx, ny, nz, n;
+    double dx, dy, dz;
+    double ms, a, alpha, gamma;
+    std::vector<double> ms_cell;  // extension: per-cell Ms (empty = uniform)
+    int integrator;
+    bool precession, exchange, demag, has_ext;
+    V3 hext;
+    std::vector<uint8_t> mask;  // empty = all active
+    double max_error, dt_min, dt_max, headroom;
+    // state
+    std::vector<V3> m;
+    double time = 0.0;
+    std::vector<V3> k_fsal;
+    bool has_fsal = false;
+    // ABM3 history (lib.rs:620-680)
+    std::vector<V3> f_n, f_n1, f_n2;
+    int abm_count = 0;  // number of stored entries (0..3)
+    int startup_steps = 0;
+    double abm_last_dt = 0.0;
+    // FFT workspace (lib.rs:337-405)
+    size_t px, py, pz;
+    std::vector<cd> kern[6];
+    std::vector<cd> bm[3], bh[3];
+
+    bool active(size_t i) const { return mask.empty() || mask[i] != 0; }
+    double ms_at(size_t i) const { return ms_cell.empty() ? ms : ms_cell[i]; }
+};
+
+void build_workspace(Problem &p) {
+    p.px = 2 * p.nx; p.py = 2 * p.ny; p.pz = 2 * p.nz;
+    size_t plen = p.px * p.py * p.pz;
+    for (int c = 0; c < 3; ++c) { p.bm[c].assign(plen, cd(0, 0)); p.bh[c].assign(plen, cd(0, 0)); }
+    if (!p.demag) return;
+    std::vector<double> nk[6];
+    compute_newell_kernels(p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, nk);
+    for (int c = 0; c < 6; ++c) {
+        p.kern[c].resize(plen);
+        for (size_t i = 0; i < plen; ++i) p.kern[c][i] = cd(nk[c][i], 0.0);
+        fft3_core(p.kern[c].data(), (int)p.px, (int)p.py, (int)p.pz, false);
+    }
+}
+
+// lib.rs:2247-2299 (per-cell Ms generalises the prefactor 2A/(mu0 Ms))
+std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m) {
+    std::vector<V3> h(p.n);
+    const double dx2 = p.dx * p.dx, dy2 = p.dy * p.dy, dz2 = p.dz * p.dz;
+#pragma omp parallel for schedule(static)
+    for (long fl = 0; fl < (long)p.n; ++fl) {
+        size_t f = (size_t)fl;
+        if (!p.active(f)) { h[f] = {0, 0, 0}; continue; }
+        const double prefactor = 2.0 * p.a / (MU0 * p.ms_at(f));
+        size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
+        const V3 c = m[f];
+        auto nb = [&](size_t a, size_t b, size_t cc) -> V3 {
+            size_t id = a + p.nx * (b + p.ny * cc);
+            return p.active(id) ? m[id] : c;
+        };
+        V3 xm = nb(x > 0 ? x - 1 : 0, y, z), xp = nb(std::min(x + 1, p.nx - 1), y, z);
+        V3 ym = nb(x, y > 0 ? y - 1 : 0, z), yp = nb(x, std::min(y + 1, p.ny - 1), z);
+        V3 zm = nb(x, y, z > 0 ? z - 1 : 0), zp = nb(x, y, std::min(z + 1, p.nz - 1));
+        V3 lap;
+        for (int k = 0; k < 3; ++k)
+            lap[k] = (xp[k] - 2.0 * c[k] + xm[k]) / dx2 + (yp[k] - 2.0 * c[k] + ym[k]) / dy2 +
+                     (zp[k] - 2.0 * c[k] + zm[k]) / dz2;
+        h[f] = scale(lap, prefactor);
+    }
+    return h;
+}
+
+// lib.rs:2306-2400
+std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) {
+    const size_t px = p.px, py = p.py, pz = p.pz, plen = px * py * pz;
+    for (int c = 0; c < 3; ++c) {
+        std::fill(p.bm[c].begin(), p.bm[c].end(), cd(0, 0));
+        std::fill(p.bh[c].begin(), p.bh[c].end(), cd(0, 0));
+    }
+    for (size_t z = 0; z < p.nz; ++z)
+        for (size_t y = 0; y < p.ny; ++y)
+            for (size_t x = 0; x < p.nx; ++x) {
+                size_t s = x + p.nx * (y + p.ny * z), d = padded_index(px, py, x, y, z);
+                V3 mo = p.active(s) ? scale(m[s], p.ms_at(s)) : V3{0, 0, 0};
+                for (int c = 0; c < 3; ++c) p.bm[c][d] = cd(mo[c], 0.0);
+            }
+    for (int c = 0; c < 3; ++c) fft3_core(p.bm[c].data(), (int)px, (int)py, (int)pz, false);
+    const auto &kxx = p.kern[0], &kyy = p.kern[1], &kzz = p.kern[2];
+    const auto &kxy = p.kern[3], &kxz = p.kern[4], &kyz = p.kern[5];
+#pragma omp parallel for schedule(static)
+    for (long il = 0; il < (long)plen; ++il) {
+        size_t i = (size_t)il;
+        cd mx = p.bm[0][i], my = p.bm[1][i], mz = p.bm[2][i];
+        p.bh[0][i] = -(kxx[i] * mx + kxy[i] * my + kxz[i] * mz);
+        p.bh[1][i] = -(kxy[i] * mx + kyy[i] * my + kyz[i] * mz);
+        p.bh[2][i] = -(kxz[i] * mx + kyz[i] * my + kzz[i] * mz);
+    }
+    for (int c = 0; c < 3; ++c) fft3_core(p.bh[c].data(), (int)px, (int)py, (int)pz, true);
+    const double norm_ = 1.0 / (double)plen;
+    std::vector<V3> h(p.n);
+    for (size_t z = 0; z < p.nz; ++z)
+        for (size_t y = 0; y < p.ny; ++y)
+            for (size_t x = 0; x < p.nx; ++x) {
+                size_t s = padded_index(px, py, x, y, z), d = x + p.nx * (y + p.ny * z);
+                h[d] = p.active(d) ? V3{p.bh[0][s].real() * norm_, p.bh[1][s].real() * norm_,
+                                        p.bh[2][s].real() * norm_}
+                                   : V3{0, 0, 0};
+            }
+    return h;
+}
+
+// lib.rs:2584-2595
+inline V3 llg_rhs_from_field(const Problem &p, const V3 &m, const V3 &h) {
+    double gamma_bar = p.gamma / (1.0 + p.alpha * p.alpha);
+    V3 prec = cross(m, h);
+    V3 damp = cross(m, prec);
+    V3 pt = p.precession ? prec : V3{0, 0, 0};
+    return scale(add(pt, scale(damp, p.alpha)), -gamma_bar);
+}
+
+struct Eval {
+    std::vector<V3> h_ex, h_d, h_ext, h_eff, rhs;
+    Report r;
+};
+
+// lib.rs:2529-2582 (llg_rhs_full_ws) and energies lib.rs:2653-2703.
+Eval full_eval(Problem &p, const std::vector<V3> &m, bool want_stats) {
+    Eval e;
+    e.h_ex = p.exchange ? exchange_field(p, m) : std::vector<V3>(p.n, V3{0, 0, 0});
+    e.h_d = p.demag ? demag_field(p, m) : std::vector<V3>(p.n, V3{0, 0, 0});
+    e.h_ext.resize(p.n);
+    const V3 ext = p.has_ext ? p.hext : V3{0, 0, 0};
+    for (size_t i = 0; i < p.n; ++i) e.h_ext[i] = p.active(i) ? ext : V3{0, 0, 0};
+    e.h_eff.resize(p.n);
+    e.rhs.resize(p.n);
+#pragma omp parallel for schedule(static)
+    for (long il = 0; il < (long)p.n; ++il) {
+        size_t i = (size_t)il;
+        e.h_eff[i] = add(add(e.h_ex[i], e.h_d[i]), e.h_ext[i]);  // lib.rs:2849-2869
+        e.rhs[i] = llg_rhs_from_field(p, m[i], e.h_eff[i]);
+    }
+    if (!want_stats) return e;
+    const double v = p.dx * p.dy * p.dz;
+    double ex = 0, ed = 0, eext = 0;
+    for (size_t i = 0; i < p.n; ++i) {
+        double ms = p.ms_at(i);
+        if (p.exchange) ex += -0.5 * MU0 * ms * dot(m[i], e.h_ex[i]) * v;
+        if (p.demag) ed += -0.5 * MU0 * ms * dot(m[i], e.h_d[i]) * v;
+        if (p.has_ext) eext += -MU0 * ms * dot(m[i], e.h_ext[i]) * v;
+    }
+    e.r.e_ex = ex; e.r.e_demag = ed; e.r.e_ext = eext;
+    e.r.e_total = ex + ed + eext;
+    e.r.max_h_eff = max_norm(e.h_eff);
+    e.r.max_h_demag = max_norm(e.h_d);
+    e.r.max_rhs = max_norm(e.rhs);
+    return e;
+}
+
+std::vector<V3> rhs(Problem &p, const std::vector<V3> &m) { return full_eval(p, m, false).rhs; }
+
+Report finish(Problem &p, double dt, double suggested) {
+    Eval e = full_eval(p, p.m, true);
+    Report r = e.r;
+    r.time_seconds = p.time;
+    r.dt_used = dt;
+    r.suggested_next_dt = suggested;
+    return r;
+}
+
+// lib.rs:1217-1249
+Report heun_step(Problem &p, double dt) {
+    const size_t n = p.n;
+    std::vector<V3> m0 = p.m;
+    std::vector<V3> k1 = rhs(p, m0);
+    std::vector<V3> ms(n);
+    for (size_t i = 0; i < n; ++i) ms[i] = normalized(add(m0[i], scale(k1[i], dt)));
+    std::vector<V3> k2 = rhs(p, ms);
+    for (size_t i = 0; i < n; ++i) p.m[i] = normalized(add(m0[i], scale(add(k1[i], k2[i]), 0.5 * dt)));
+    p.time += dt;
+    return finish(p, dt, 0.0);
+}
+
+// lib.rs:1254-1302
+Report rk4_step(Problem &p, double dt) {
+    const size_t n = p.n;
+    std::vector<V3> m0 = p.m, st(n);
+    auto k1 = rhs(p, m0);
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], 0.5 * dt)));
+    auto k2 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k2[i], 0.5 * dt)));
+    auto k3 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k3[i], dt)));
+    auto k4 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        p.m[i] = normalized(add(m0[i], scale(add(add(k1[i], scale(k2[i], 2.0)), add(scale(k3[i], 2.0), k4[i])),
+                                             dt / 6.0)));
+    p.time += dt;
+    return finish(p, dt, 0.0);
+}
+
+// lib.rs:1653-1674
+double max_error_norm(const std::vector<std::pair<const std::vector<V3> *, double>> &ws, double dt, size_t n) {
+    double mx = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        V3 e = {0, 0, 0};
+        for (auto &w : ws) {
+            const V3 &k = (*w.first)[i];
+            e[0] += w.second * k[0];
+            e[1] += w.second * k[1];
+            e[2] += w.second * k[2];
+        }
+        e[0] *= dt; e[1] *= dt; e[2] *= dt;
+        mx = std::max(mx, norm(e));
+    }
+    return mx;
+}
+
+// lib.rs:1332-1407
+Report rk23_step(Problem &p, double dt) {
+    dt = std::max(std::min(dt, p.dt_max), p.dt_min);
+    const size_t n = p.n;
+    std::vector<V3> m0 = p.m, st(n);
+    for (;;) {
+        auto k1 = rhs(p, m0);
+        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], 0.5 * dt)));
+        auto k2 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k2[i], 0.75 * dt)));
+        auto k3 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i) {
+            V3 d = scale(add(add(scale(k1[i], 2.0 / 9.0), scale(k2[i], 1.0 / 3.0)), scale(k3[i], 4.0 / 9.0)), dt);
+            st[i] = normalized(add(m0[i], d));
+        }
+        auto k4 = rhs(p, st);
+        double err = max_error_norm({{&k1, -5.0 / 72.0}, {&k2, 1.0 / 12.0}, {&k3, 1.0 / 9.0}, {&k4, -1.0 / 8.0}}, dt, n);
+        if (err <= p.max_error || dt <= p.dt_min) {
+            p.m = st;
+            p.time += dt;
+            double dt_next = std::min(std::max(p.headroom * dt * std::pow(p.max_error / std::max(err, 1e-30), 1.0 / 3.0),
+                                               p.dt_min),
+                                      p.dt_max);
+            return finish(p, dt, dt_next);
+        }
+        double dt_new = p.headroom * dt * std::pow(p.max_error / err, 1.0 / 3.0);
+        dt = std::min(std::max(dt_new, p.dt_min), p.dt_max);
+    }
+}
+
+// lib.rs:1412-1568
+Report dp45_step(Problem &p, double dt) {
+    dt = std::max(std::min(dt, p.dt_max), p.dt_min);
+    const size_t n = p.n;
+    const double A21 = 1.0 / 5.0;
+    const double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+    const double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+    const double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0, A54 = -212.0 / 729.0;
+    const double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0, A64 = 49.0 / 176.0,
+                 A65 = -5103.0 / 18656.0;
+    const double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0, B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+    const double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0, E5 = -17253.0 / 339200.0,
+                 E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+    std::vector<V3> m0 = p.m, st(n);
+    for (;;) {
+        std::vector<V3> k1;
+        if (p.has_fsal) { k1 = p.k_fsal; p.has_fsal = false; p.k_fsal.clear(); }
+        else k1 = rhs(p, m0);
+        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], A21 * dt)));
+        auto k2 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i)
+            st[i] = normalized(add(m0[i], scale(add(scale(k1[i], A31), scale(k2[i], A32)), dt)));
+        auto k3 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i)
+            st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A41), scale(k2[i], A42)), scale(k3[i], A43)), dt)));
+        auto k4 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i)
+            st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A51), scale(k2[i], A52)),
+                                                    add(scale(k3[i], A53), scale(k4[i], A54))),
+                                                dt)));
+        auto k5 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i)
+            st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], A61), scale(k2[i], A62)), scale(k3[i], A63)),
+                                                    add(scale(k4[i], A64), scale(k5[i], A65))),
+                                                dt)));
+        auto k6 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i)
+            st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], B1), scale(k3[i], B3)), scale(k4[i], B4)),
+                                                    add(scale(k5[i], B5), scale(k6[i], B6))),
+                                                dt)));
+        auto k7 = rhs(p, st);
+        double err = max_error_norm({{&k1, E1}, {&k3, E3}, {&k4, E4}, {&k5, E5}, {&k6, E6}, {&k7, E7}}, dt, n);
+        if (err <= p.max_error || dt <= p.dt_min) {
+            p.m = st;
+            p.time += dt;
+            p.k_fsal = k7;
+            p.has_fsal = true;
+            double dt_next = std::min(std::max(p.headroom * dt * std::pow(p.max_error / std::max(err, 1e-30), 0.2),
+                                               p.dt_min),
+                                      p.dt_max);
+            return finish(p, dt, dt_next);
+        }
+        double dt_new = p.headroom * dt * std::pow(p.max_error / err, 0.2);
+        dt = std::min(std::max(dt_new, p.dt_min), p.dt_max);
+    }
+}
+
+// lib.rs:663-674 AbmHistory::push (restart when dt changes by more than 10%)
+void abm_push(Problem &p, std::vector<V3> f, double dt) {
+    if (p.abm_last_dt > 0.0 && std::fabs(dt - p.abm_last_dt) / p.abm_last_dt > 0.1) {
+        p.f_n.clear(); p.f_n1.clear(); p.f_n2.clear();
+        p.abm_count = 0; p.startup_steps = 0; p.abm_last_dt = 0.0;
+    }
+    p.f_n2 = std::move(p.f_n1);
+    p.f_n1 = std::move(p.f_n);
+    p.f_n = std::move(f);
+    p.abm_count = std::min(p.abm_count + 1, 3);
+    p.startup_steps = std::min(p.startup_steps + 1, 3);
+    p.abm_last_dt = dt;
+}
+
+// lib.rs:1573-1648
+Report abm3_step(Problem &p, double dt) {
+    const size_t n = p.n;
+    bool ready = p.startup_steps >= 3 && !p.f_n.empty() && !p.f_n1.empty() && !p.f_n2.empty();
+    std::vector<V3> m0 = p.m, st(n);
+    if (!ready) {
+        auto k1 = rhs(p, m0);
+        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], dt)));
+        auto k2 = rhs(p, st);
+        for (size_t i = 0; i < n; ++i) p.m[i] = normalized(add(m0[i], scale(add(k1[i], k2[i]), 0.5 * dt)));
+        p.time += dt;
+        abm_push(p, rhs(p, p.m), dt);
+        return finish(p, dt, 0.0);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        V3 pred = add(add(scale(p.f_n[i], 23.0 / 12.0), scale(p.f_n1[i], -16.0 / 12.0)), scale(p.f_n2[i], 5.0 / 12.0));
+        st[i] = normalized(add(m0[i], scale(pred, dt)));
+    }
+    auto fs = rhs(p, st);
+    for (size_t i = 0; i < n; ++i) {
+        V3 corr = add(add(scale(fs[i], 5.0 / 12.0), scale(p.f_n[i], 8.0 / 12.0)), scale(p.f_n1[i], -1.0 / 12.0));
+        p.m[i] = normalized(add(m0[i], scale(corr, dt)));
+    }
+    p.time += dt;
+    abm_push(p, fs, dt);
+    return finish(p, dt, 0.0);
+}
+
+void report_to(const Report &r, double *out) {
+    out[0] = r.time_seconds; out[1] = r.dt_used; out[2] = r.suggested_next_dt;
+    out[3] = r.e_ex; out[4] = r.e_demag; out[5] = r.e_ext; out[6] = r.e_total;
+    out[7] = r.max_h_eff; out[8] = r.max_h_demag; out[9] = r.max_rhs;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C entry points (ctypes from tests/ and bench.py's cpu_baseline leg)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void oracle_set_num_threads(int t) {
+#ifdef _OPENMP
+    if (t > 0) omp_set_num_threads(t);
+#else
+    (void)t;
+#endif
+}
+
+double oracle_newell_f(double x, double y, double z) { return newell_f(x, y, z); }
+double oracle_newell_g(double x, double y, double z) { return newell_g(x, y, z); }
+
+// out: 6 arrays of (2nx)(2ny)(2nz) doubles, order xx,yy,zz,xy,xz,yz
+void oracle_newell_kernels(uint64_t nx, uint64_t ny, uint64_t nz, double dx, double dy, double dz, double *out) {
+    std::vector<double> k[6];
+    compute_newell_kernels(nx, ny, nz, dx, dy, dz, k);
+    size_t plen = 8 * nx * ny * nz;
+    for (int c = 0; c < 6; ++c) std::memcpy(out + c * plen, k[c].data(), plen * sizeof(double));
+}
+
+// lib.rs:511-610.  thin_film_2d != 0 requires nz == 1 and produces the 2-D
+// (2nx)(2ny) spectra; otherwise the full 3-D (2nx)(2ny)(2nz) spectra.
+// out: 6 interleaved complex arrays [re,im,...] back to back.
+int oracle_newell_spectra(uint64_t nx, uint64_t ny, uint64_t nz, double dx, double dy, double dz,
+                          int thin_film_2d, double *out) {
+    if (thin_film_2d && nz != 1) return -1;
+    std::vector<double> k[6];
+    compute_newell_kernels(nx, ny, nz, dx, dy, dz, k);
+    size_t px = 2 * nx, py = 2 * ny, pz = 2 * nz;
+    if (thin_film_2d) {
+        size_t plane = px * py;
+        for (int c = 0; c < 6; ++c) {
+            std::vector<cd> b(plane);
+            for (size_t y = 0; y < py; ++y)
+                for (size_t x = 0; x < px; ++x) b[x + px * y] = cd(k[c][padded_index(px, py, x, y, 0)], 0.0);
+            fft3_core(b.data(), (int)px, (int)py, 1, false);
+            std::memcpy(out + c * 2 * plane, b.data(), plane * sizeof(cd));
+        }
+    } else {
+        size_t plen = px * py * pz;
+        for (int c = 0; c < 6; ++c) {
+            std::vector<cd> b(plen);
+            for (size_t i = 0; i < plen; ++i) b[i] = cd(k[c][i], 0.0);
+            fft3_core(b.data(), (int)px, (int)py, (int)pz, false);
+            std::memcpy(out + c * 2 * plen, b.data(), plen * sizeof(cd));
+        }
+    }
+    return 0;
+}
+
+void oracle_fft(double *interleaved, uint64_t n, int inverse) {
+    fft1(reinterpret_cast<cd *>(interleaved), (int)n, inverse != 0);
+}
+
+void oracle_fft3(double *interleaved, uint64_t nx, uint64_t ny, uint64_t nz, int inverse) {
+    fft3_core(reinterpret_cast<cd *>(interleaved), (int)nx, (int)ny, (int)nz, inverse != 0);
+}
+
+// fullmag-plan/src/lib.rs:1927-1957
+void oracle_random_unit_vectors(uint64_t seed, uint64_t count, double *out) {
+    uint64_t state = seed;
+    for (uint64_t i = 0; i < count; ++i) {
+        double c[3];
+        for (;;) {
+            for (int k = 0; k < 3; ++k) {
+                state ^= state << 13;
+                state ^= state >> 7;
+                state ^= state << 17;
+                c[k] = ((double)state / (double)UINT64_MAX) * 2.0 - 1.0;
+            }
+            double nrm = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            if (nrm > 1e-10) {
+                c[0] /= nrm; c[1] /= nrm; c[2] /= nrm;
+                break;
+            }
+        }
+        out[3 * i + 0] = c[0]; out[3 * i + 1] = c[1]; out[3 * i + 2] = c[2];
+    }
+}
+
+// params: [dx,dy,dz, Ms, A, alpha, gamma, hx,hy,hz, max_error, dt_min, dt_max, headroom]
+// flags:  [integrator, precession, exchange, demag, has_ext]
+void *oracle_problem_create(uint64_t nx, uint64_t ny, uint64_t nz, const double *params, const int *flags,
+                            const uint8_t *active_mask, const double *ms_cell, const double *m0_aos) {
+    auto *p = new Problem();
+    p->nx = nx; p->ny = ny; p->nz = nz; p->n = nx * ny * nz;
+    p->dx = params[0]; p->dy = params[1]; p->dz = params[2];
+    p->ms = params[3]; p->a = params[4]; p->alpha = params[5]; p->gamma = params[6];
+    p->hext = {params[7], params[8], params[9]};
+    // AdaptiveStepConfig defaults lib.rs:217-226
+    p->max_error = params[10] > 0 ? params[10] : 1e-5;
+    p->dt_min = params[11] > 0 ? params[11] : 1e-18;
+    p->dt_max = params[12] > 0 ? params[12] : 1e-10;
+    p->headroom = params[13] > 0 ? params[13] : 0.8;
+    p->integrator = flags[0];
+    p->precession = flags[1] != 0;
+    p->exchange = flags[2] != 0;
+    p->demag = flags[3] != 0;
+    p->has_ext = flags[4] != 0;
+    if (active_mask) p->mask.assign(active_mask, active_mask + p->n);
+    if (ms_cell) p->ms_cell.assign(ms_cell, ms_cell + p->n);
+    p->m.resize(p->n);
+    // ExchangeLlgState::new normalises; new_state zeroes inactive cells (lib.rs:1009-1019)
+    for (size_t i = 0; i < p->n; ++i) {
+        V3 v = {m0_aos[3 * i], m0_aos[3 * i + 1], m0_aos[3 * i + 2]};
+        p->m[i] = p->active(i) ? normalized(v) : V3{0, 0, 0};
+    }
+    build_workspace(*p);
+    return p;
+}
+
+void oracle_problem_destroy(void *h) { delete static_cast<Problem *>(h); }
+
+// report (10 doubles): time, dt_used, suggested_next_dt (0 = none),
+// E_ex, E_demag, E_ext, E_total, max|H_eff|, max|H_demag|, max|dm/dt|
+int oracle_problem_step(void *h, double dt, double *report) {
+    auto *p = static_cast<Problem *>(h);
+    if (!(dt > 0.0)) return -1;
+    Report r{};
+    switch (p->integrator) {
+        case RK4: r = rk4_step(*p, dt); break;
+        case RK23: r = rk23_step(*p, dt); break;
+        case DP45: r = dp45_step(*p, dt); break;
+        case ABM3: r = abm3_step(*p, dt); break;
+        case HEUN:
+        default: r = heun_step(*p, dt); break;
+    }
+    report_to(r, report);
+    return 0;
+}
+
+void oracle_problem_get_m(void *h, double *out) {
+    auto *p = static_cast<Problem *>(h);
+    for (size_t i = 0; i < p->n; ++i)
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = p->m[i][k];
+}
+
+// set_magnetization (lib.rs:739-752) normalises and keeps FSAL / ABM history.
+void oracle_problem_set_m(void *h, const double *in) {
+    auto *p = static_cast<Problem *>(h);
+    for (size_t i = 0; i < p->n; ++i) {
+        V3 v = {in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+        p->m[i] = p->active(i) ? normalized(v) : V3{0, 0, 0};
+    }
+}
+
+double oracle_problem_time(void *h) { return static_cast<Problem *>(h)->time; }
+
+// fields: 5 AoS arrays of 3N doubles back to back: H_ex, H_demag, H_ext, H_eff, rhs.
+// scalars (10 doubles, same layout as the step report with time/dt fields zero).
+void oracle_problem_observe(void *h, double *fields, double *scalars) {
+    auto *p = static_cast<Problem *>(h);
+    Eval e = full_eval(*p, p->m, true);
+    const std::vector<V3> *arr[5] = {&e.h_ex, &e.h_d, &e.h_ext, &e.h_eff, &e.rhs};
+    if (fields)
+        for (int a = 0; a < 5; ++a)
+            for (size_t i = 0; i < p->n; ++i)
+                for (int k = 0; k < 3; ++k) fields[a * 3 * p->n + 3 * i + k] = (*arr[a])[i][k];
+    e.r.time_seconds = p->time;
+    e.r.dt_used = 0; e.r.suggested_next_dt = 0;
+    if (scalars) report_to(e.r, scalars);
+}
+
+}  // extern "C"

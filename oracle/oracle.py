@@ -1,0 +1,224 @@
+"""ctypes wrapper around the CPU FP64 oracle (oracle/fdm_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the fullmag_amd package.
+
+The oracle restates fullmag-engine's CPU reference (crates/fullmag-engine/
+src/lib.rs), fullmag-fdm-demag's Newell tensor (crates/fullmag-fdm-demag/src/
+newell.rs) and the planner's m0 generator (crates/fullmag-plan/src/lib.rs:
+1927-1957).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+MU0 = 4.0 * np.pi * 1e-7
+INTEGRATORS = {"heun": 1, "dp45": 2, "rk45": 2, "abm3": 3, "rk4": 4, "rk23": 5}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++ + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u64, d, i = ctypes.c_uint64, ctypes.c_double, ctypes.c_int
+        L.oracle_newell_f.restype = d
+        L.oracle_newell_f.argtypes = [d, d, d]
+        L.oracle_newell_g.restype = d
+        L.oracle_newell_g.argtypes = [d, d, d]
+        L.oracle_newell_kernels.argtypes = [u64, u64, u64, d, d, d, _dp]
+        L.oracle_newell_spectra.restype = i
+        L.oracle_newell_spectra.argtypes = [u64, u64, u64, d, d, d, i, _dp]
+        L.oracle_fft.argtypes = [_dp, u64, i]
+        L.oracle_fft3.argtypes = [_dp, u64, u64, u64, i]
+        L.oracle_random_unit_vectors.argtypes = [u64, u64, _dp]
+        L.oracle_problem_create.restype = ctypes.c_void_p
+        L.oracle_problem_create.argtypes = [u64, u64, u64, _dp, ctypes.POINTER(i),
+                                            ctypes.c_void_p, ctypes.c_void_p, _dp]
+        L.oracle_problem_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_problem_step.restype = i
+        L.oracle_problem_step.argtypes = [ctypes.c_void_p, d, _dp]
+        L.oracle_problem_get_m.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_problem_set_m.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_problem_time.restype = d
+        L.oracle_problem_time.argtypes = [ctypes.c_void_p]
+        L.oracle_problem_observe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_num_threads.restype = i
+        L.oracle_set_num_threads.argtypes = [i]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def set_num_threads(n: int) -> None:
+    lib().oracle_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return lib().oracle_num_threads()
+
+
+def newell_f(x, y, z) -> float:
+    return lib().oracle_newell_f(x, y, z)
+
+
+def newell_g(x, y, z) -> float:
+    return lib().oracle_newell_g(x, y, z)
+
+
+def newell_kernels(nx, ny, nz, dx, dy, dz) -> np.ndarray:
+    """Real-space kernels, shape (6, 2nz, 2ny, 2nx), order xx,yy,zz,xy,xz,yz."""
+    out = np.zeros((6, 2 * nz, 2 * ny, 2 * nx))
+    lib().oracle_newell_kernels(nx, ny, nz, dx, dy, dz, _ptr(out))
+    return out
+
+
+def newell_spectra(nx, ny, nz, dx, dy, dz, thin_film_2d: bool = False) -> np.ndarray:
+    """Interleaved complex spectra as handed to the C ABI: shape (6, 2*Npad).
+
+    thin_film_2d follows compute_newell_kernel_spectra_thin_film_2d
+    (crates/fullmag-engine/src/lib.rs:553-610): nz must be 1, Npad = 4 nx ny.
+    Otherwise compute_newell_kernel_spectra (lib.rs:511-551), Npad = 8 nx ny nz.
+    """
+    npad = 4 * nx * ny if thin_film_2d else 8 * nx * ny * nz
+    out = np.zeros((6, 2 * npad))
+    rc = lib().oracle_newell_spectra(nx, ny, nz, dx, dy, dz, 1 if thin_film_2d else 0, _ptr(out))
+    if rc != 0:
+        raise ValueError("thin-film 2-D spectra need nz == 1")
+    return out
+
+
+def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    lib().oracle_fft(a.ctypes.data_as(_dp), a.size, 1 if inverse else 0)
+    return a
+
+
+def fft3(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """x has shape (pz, py, px) (x fastest)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    pz, py, px = a.shape
+    lib().oracle_fft3(a.ctypes.data_as(_dp), px, py, pz, 1 if inverse else 0)
+    return a
+
+
+def random_unit_vectors(seed: int, count: int) -> np.ndarray:
+    out = np.zeros((count, 3))
+    lib().oracle_random_unit_vectors(ctypes.c_uint64(seed), count, _ptr(out))
+    return out
+
+
+@dataclass
+class OracleConfig:
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    ms: float
+    a: float
+    alpha: float
+    gamma: float = 2.211e5
+    integrator: str = "heun"
+    precession: bool = True
+    exchange: bool = True
+    demag: bool = True
+    h_ext: Optional[tuple] = None  # A/m
+    active_mask: Optional[np.ndarray] = None
+    ms_cell: Optional[np.ndarray] = None  # extension: per-cell Ms
+    max_error: float = 0.0
+    dt_min: float = 0.0
+    dt_max: float = 0.0
+    headroom: float = 0.0
+
+
+REPORT_FIELDS = ("time", "dt_used", "suggested_next_dt", "e_ex", "e_demag", "e_ext",
+                 "e_total", "max_h_eff", "max_h_demag", "max_dm_dt")
+
+
+class OracleProblem:
+    """ExchangeLlgProblem + ExchangeLlgState + workspace (lib.rs:869-1147)."""
+
+    def __init__(self, cfg: OracleConfig, m0: np.ndarray):
+        self.cfg = cfg
+        self.n = cfg.nx * cfg.ny * cfg.nz
+        h = cfg.h_ext if cfg.h_ext is not None else (0.0, 0.0, 0.0)
+        params = np.array([cfg.dx, cfg.dy, cfg.dz, cfg.ms, cfg.a, cfg.alpha, cfg.gamma,
+                           h[0], h[1], h[2], cfg.max_error, cfg.dt_min, cfg.dt_max,
+                           cfg.headroom], dtype=np.float64)
+        flags = (ctypes.c_int * 5)(INTEGRATORS[cfg.integrator], int(cfg.precession),
+                                   int(cfg.exchange), int(cfg.demag), int(cfg.h_ext is not None))
+        m0 = np.ascontiguousarray(m0, dtype=np.float64).reshape(self.n, 3)
+        self._mask = None
+        if cfg.active_mask is not None:
+            self._mask = np.ascontiguousarray(cfg.active_mask, dtype=np.uint8).ravel()
+        self._ms = None
+        if cfg.ms_cell is not None:
+            self._ms = np.ascontiguousarray(cfg.ms_cell, dtype=np.float64).ravel()
+        self._h = lib().oracle_problem_create(
+            cfg.nx, cfg.ny, cfg.nz, _ptr(params), flags,
+            None if self._mask is None else self._mask.ctypes.data_as(ctypes.c_void_p),
+            None if self._ms is None else self._ms.ctypes.data_as(ctypes.c_void_p),
+            _ptr(m0))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().oracle_problem_destroy(h)
+            self._h = None
+
+    def step(self, dt: float) -> dict:
+        r = np.zeros(10)
+        rc = lib().oracle_problem_step(self._h, dt, _ptr(r))
+        if rc != 0:
+            raise ValueError("dt must be positive")
+        return dict(zip(REPORT_FIELDS, r.tolist()))
+
+    @property
+    def m(self) -> np.ndarray:
+        out = np.zeros((self.n, 3))
+        lib().oracle_problem_get_m(self._h, _ptr(out))
+        return out
+
+    @m.setter
+    def m(self, value: np.ndarray) -> None:
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(self.n, 3)
+        lib().oracle_problem_set_m(self._h, _ptr(v))
+
+    @property
+    def time(self) -> float:
+        return lib().oracle_problem_time(self._h)
+
+    def observe(self) -> dict:
+        """Fields (AoS, shape (N,3)) and scalars at the current state."""
+        f = np.zeros((5, self.n, 3))
+        s = np.zeros(10)
+        lib().oracle_problem_observe(self._h, f.ctypes.data_as(ctypes.c_void_p),
+                                     s.ctypes.data_as(ctypes.c_void_p))
+        out = dict(zip(REPORT_FIELDS, s.tolist()))
+        out.update(h_ex=f[0], h_demag=f[1], h_ext=f[2], h_eff=f[3], rhs=f[4])
+        return out
