@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Heun cell-steps/s of the FDM LLG step on one MI355X.
+
+Workload (the metric's configuration): thin film 512x512x8 cells of 4x4x2 nm,
+Ms 8.6e5 A/m, A 1.3e-11 J/m, alpha 0.02, exchange + Newell demag (padded
+1024x1024x16) + Zeeman with B_ext of seeded random direction and 0-50 mT
+magnitude, m0 = seeded random unit vectors (planner generator), Heun dt 5e-14 s.
+
+A step is one fullmag_fdm_backend_step() call through the C ABI, including the
+per-step diagnostics it returns, exactly as the runner drives it.  The timed
+region is bracketed by torch.cuda.synchronize(); inputs are resident on the
+device before timing starts.  Prints one JSON line.
+
+Also reported: the plain FP64 path's throughput on the same workload, the
+mixed-vs-FP64 accuracy after --acc-steps steps from the same m0, the roofline
+position of the dominant kernel (per-kernel HIP-event timing on the backend's
+stream), and the CPU oracle timed on the host (bounded sample).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MU0 = 4.0 * np.pi * 1e-7
+HBM_PEAK_GBS = 8000.0
+METRIC = "Heun cell-steps/s (exch+demag+Zeeman) mixed-FP32 vs FP64 at max|Δm|≤1e-5"
+
+
+def workload():
+    import fullmag_amd as fa
+
+    nx, ny, nz = 512, 512, 8
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(42, n)
+    v = fa.generate_random_unit_vectors(7, 2)
+    b_tesla = 0.050 * (v[1, 0] + 1.0) / 2.0  # magnitude in [0, 50] mT from the same stream
+    h_ext = tuple((b_tesla / MU0) * v[0])
+    base = dict(cells=(nx, ny, nz), cell_size=(4e-9, 4e-9, 2e-9), initial_magnetization=m0,
+                saturation_magnetisation=8.6e5, exchange_stiffness=1.3e-11, damping=0.02,
+                external_field=h_ext, integrator="heun", demag_kernel="device")
+    return base, {"b_ext_mT": 1e3 * b_tesla, "b_dir": [float(x) for x in v[0]]}
+
+
+def alg_bytes(nx, ny, nz, q, r):
+    """Compulsory HBM bytes per launch of each kernel of one Heun step
+    (each array read or written once per launch; the exchange stencil's
+    neighbour re-reads are not counted).  q: state bytes, r: field bytes."""
+    n = nx * ny * nz
+    c = 2 * r
+    h = nx + 1
+    a1 = h * ny * nz
+    a2 = h * 2 * ny * nz
+    kern = 6 * (nx + 1) * (ny + 1) * (nz + 1)
+    return {
+        # one launch reads the state m, the other the stage point
+        "demag_x_r2c": 3 * n * (q + r) / 2 + 3 * a1 * c,
+        "demag_y_fwd": 3 * a1 * c + 3 * a2 * c,
+        "demag_z_conv": 3 * a2 * c + kern * r + 3 * a2 * c,
+        "demag_y_inv": 3 * a2 * c + 3 * a1 * c,
+        "demag_x_c2r": 3 * a1 * c + 3 * n * r,
+        # predictor: m, Hd -> k1, mt;  corrector: mt, Hd, m0, k1 -> m
+        "llg_stage": 3 * n * (q + 3 * r + 2 * q + 3 * r) / 2,
+        "llg_stats": 3 * n * (q + r),
+    }
+
+
+def step_bytes(nx, ny, nz, q, r):
+    """Algorithmic bytes of one Heun step: two demag evaluations plus the
+    torque/predictor and corrector passes, stats fused."""
+    n = nx * ny * nz
+    c = 2 * r
+    h = nx + 1
+    a1, a2 = h * ny * nz, h * 2 * ny * nz
+    kern = 6 * (nx + 1) * (ny + 1) * (nz + 1)
+    d = (3 * n * q + 3 * a1 * c) + (3 * a1 * c + 3 * a2 * c) + (3 * a2 * c + kern * r + 3 * a2 * c) \
+        + (3 * a2 * c + 3 * a1 * c) + (3 * a1 * c + 3 * n * r)
+    return 2 * d + 3 * n * (2 * q + 2 * r) + 3 * n * (3 * q + 2 * r)
+
+
+def time_steps(backend, steps, warmup, dt):
+    import torch
+
+    for _ in range(warmup):
+        backend.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        backend.step(dt)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(backend, dt, steps=10):
+    backend.set_timing(True)
+    for _ in range(steps):
+        backend.step(dt)
+    t = backend.timings()
+    backend.set_timing(False)
+    return t
+
+
+def cpu_baseline(base, dt):
+    """The oracle (FP64 C++ restatement of fullmag-engine: OpenMP per-cell
+    loops, serial FFT) timed for one Heun step of the same workload."""
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    O.set_num_threads(threads)
+    nx, ny, nz = base["cells"]
+    dx, dy, dz = base["cell_size"]
+    cfg = O.OracleConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ms=base["saturation_magnetisation"],
+                         a=base["exchange_stiffness"], alpha=base["damping"], h_ext=base["external_field"])
+    p = O.OracleProblem(cfg, base["initial_magnetization"])
+    t0 = time.perf_counter()
+    p.step(dt)
+    el = time.perf_counter() - t0
+    return {"value": nx * ny * nz / el, "unit": "cell-steps/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"1 Heun step of the {nx}x{ny}x{nz} workload ({nx * ny * nz} cells, {el:.1f} s), "
+                      "FP64 oracle restating crates/fullmag-engine (setup and Newell build excluded)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=100)
+    ap.add_argument("--acc-steps", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
+              "build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import fullmag_amd as fa
+
+    torch.cuda.init()
+    dt = 5e-14
+    base, extra = workload()
+    nx, ny, nz = base["cells"]
+    n = nx * ny * nz
+
+    mixed = fa.NativeFdmBackend(fa.FdmPlan(precision="mixed", **base))
+    t_mixed = time_steps(mixed, args.steps, args.warmup, dt)
+    value = n * args.steps / t_mixed
+    ms_per_step = 1e3 * t_mixed / args.steps
+    prof = {} if args.no_profile else kernel_profile(mixed, dt)
+    mixed.close()
+
+    fp64 = fa.NativeFdmBackend(fa.FdmPlan(precision="double", **base))
+    t64 = time_steps(fp64, args.fp64_steps, 5, dt)
+    fp64_value = n * args.fp64_steps / t64
+    prof64 = {} if args.no_profile else kernel_profile(fp64, dt)
+    fp64.close()
+
+    # accuracy of the mixed path against the FP64 GPU path from the same m0
+    a = fa.NativeFdmBackend(fa.FdmPlan(precision="double", **base))
+    b = fa.NativeFdmBackend(fa.FdmPlan(precision="mixed", **base))
+    for _ in range(args.acc_steps):
+        sa = a.step(dt)
+        sb = b.step(dt)
+    max_dm = float(np.abs(a.copy_m() - b.copy_m()).max())
+    rel_e = abs(sb.e_total - sa.e_total) / abs(sa.e_total)
+    a.close()
+    b.close()
+
+    roofline = None
+    if prof:
+        ab = alg_bytes(nx, ny, nz, 8, 4)
+        name, (tot_ms, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
+        avg_s = tot_ms / cnt / 1e3
+        achieved = ab[name] / avg_s / 1e9 if name in ab else None
+        roofline = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                    "avg_launch_us": 1e6 * avg_s, "alg_bytes_per_launch": ab.get(name),
+                    "kernels_us": {k: round(1e3 * v[0] / v[1], 2) for k, v in sorted(prof.items())}}
+
+    sb_bytes = step_bytes(nx, ny, nz, 8, 4)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields / f64 state",
+        "data": "synthetic",
+        "config": {"workload": "thin film 512x512x8 cells of 4x4x2 nm, Heun dt 5e-14 s, exchange + Newell demag "
+                               "(padded 1024x1024x16) + Zeeman, Ms 8.6e5, A 1.3e-11, alpha 0.02, seeded random "
+                               "m0 and B_ext", "cells": n, "b_ext_mT": extra["b_ext_mT"],
+                   "precision": "mixed (FP32 fields/FFT/torque, FP64 state and Newell tensor)",
+                   "fp64_cell_steps_per_s": fp64_value, "speedup_vs_fp64": value / fp64_value,
+                   "fp64_ms_per_step": 1e3 * t64 / args.fp64_steps,
+                   "accuracy_vs_fp64": {"steps": args.acc_steps, "max_abs_dm": max_dm, "rel_energy_err": rel_e},
+                   "step_alg_bytes": sb_bytes, "step_achieved_GBs": sb_bytes / (ms_per_step / 1e3) / 1e9},
+        "roofline": roofline,
+        "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(base, dt),
+    }
+    if prof64:
+        out["config"]["fp64_kernels_us"] = {k: round(1e3 * v[0] / v[1], 2) for k, v in sorted(prof64.items())}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+// context.hpp -- per-backend device state of the HIP FDM library.
+//
+// Layout in HBM (all vector fields are SoA, component-major, 3 * n elements,
+// index x + nx (y + ny z) as crates/fullmag-engine/src/lib.rs:139-141):
+//   m        State   magnetisation (FP64 in double/mixed, FP32 in single)
+//   mst      State   candidate state of adaptive integrators
+//   hd       Real    demag field at the most recently evaluated point
+//   mt[2]    Real    stage points of multi-stage integrators
+//   k[i]     Real    stage slopes dm/dt
+//   spec     cplx<Real>  half spectrum of the padded convolution, per
+//                    component [nz][rows][nx+1] (rows = 2ny in 3-D, ny in 2-D)
+//   kern     Real    Newell tensor spectrum on the non-negative octant
+//                    [comp][nz+1 or 1][ny+1][nx+1] (real by parity)
+#pragma once
+
+#include "common.hpp"
+#include "fft.hpp"
+#include "fullmag_fdm.h"
+#include "fullmag_fdm_ext.h"
+
+#include <string>
+#include <vector>
+
+namespace fm {
+
+enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
+
+struct DemagPlan {
+    bool enabled = false;
+    bool three_d = false;     // false: nz == 1 thin film, 2-D padded transform
+    bool projection = false;  // no tensor: k-space projection fallback
+    int nx = 0, ny = 0, nz = 0;
+    int px = 0, py = 0, pz = 1;
+    int hx = 0;    // nx + 1 columns of the half spectrum
+    int rows = 0;  // rows per plane of `spec`
+    int kcomp = 0; // 6 (3-D) or 4 (2-D: xz = yz = 0 on the film plane)
+    int kz_n = 1;  // octant planes in kz: nz + 1 (3-D) or 1
+    double inv_npad = 0.0;
+    LinePlan lx, ly, lz;
+    // launch geometry
+    int x_lines_per_wg = 1;
+    int y_tile = 1, yconv_tile = 1, z_tile = 1;
+    // device buffers (element type depends on Real)
+    void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
+    void *spec = nullptr;
+    void *kern = nullptr;
+    uint64_t spec_elems = 0;  // complex elements
+    uint64_t kern_elems = 0;  // real elements
+};
+
+struct TimingEntry {
+    std::string name;
+    double total_ms = 0.0;
+    uint64_t launches = 0;
+};
+
+struct PendingEvent {
+    int entry;
+    hipEvent_t a, b;
+};
+
+struct Context {
+    // grid and material
+    uint32_t nx = 0, ny = 0, nz = 0;
+    uint64_t n = 0;
+    double dx = 0, dy = 0, dz = 0;
+    double ms = 0, a = 0, alpha = 0, gamma = 0;
+    int mode = MODE_DOUBLE;
+    int integrator = FULLMAG_FDM_INTEGRATOR_HEUN;
+    bool precession = true, exchange = true, demag = false, has_ext = false;
+    double hext[3] = {0, 0, 0};
+
+    bool has_mask = false;
+    std::vector<uint8_t> mask_host;
+    uint8_t *d_mask = nullptr;
+    bool use_lut = false;        // per-pair A_ij from the region table
+    double a_exchange = 0;       // uniform A used when !use_lut
+    uint32_t *d_region = nullptr;
+    double *d_lut = nullptr;
+    bool has_ms_cell = false;
+    double *d_ms_cell = nullptr;
+
+    double max_err = 1e-5, dt_min = 1e-18, dt_max = 1e-10, headroom = 0.8;
+
+    // time stepping
+    uint64_t step_count = 0;
+    double time = 0.0;
+    bool fsal_valid = false;
+    int abm_count = 0, abm_startup = 0;
+    double abm_last_dt = 0.0;
+    int abm_slot[3] = {4, 5, 6};  // k[] indices of f_n, f_{n-1}, f_{n-2}
+
+    // device
+    int device = 0;
+    hipStream_t stream = nullptr;
+    void *m = nullptr, *mst = nullptr, *hd = nullptr;
+    void *mt[2] = {nullptr, nullptr};
+    void *k[7] = {};
+    void *h_ex_obs = nullptr, *h_eff_obs = nullptr;
+    bool hd_at_m = false;     // hd holds H_demag(m)
+    bool obs_frozen = false;  // observables kept from before an upload
+    bool obs_valid = false;   // h_ex_obs / h_eff_obs hold the fields at m
+    DemagPlan dp;
+
+    // reductions
+    double *d_partials = nullptr;
+    int max_partial_blocks = 0;
+    double *d_result = nullptr;
+    double *h_result = nullptr;  // pinned
+
+    // scratch for previews
+    void *d_preview = nullptr;
+    uint64_t preview_bytes = 0;
+
+    // timing instrumentation
+    bool timing = false;
+    std::vector<TimingEntry> timings;
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    std::string last_error;
+    fullmag_fdm_device_info info{};
+    bool info_valid = false;
+    uint64_t device_bytes = 0;
+    std::vector<void *> allocations;
+    bool ready = false;  // create() finished successfully
+
+    size_t state_bytes() const { return mode == MODE_SINGLE ? 4 : 8; }
+    size_t real_bytes() const { return mode == MODE_DOUBLE ? 8 : 4; }
+};
+
+// context.cpp
+bool ctx_alloc(Context &c, void **ptr, size_t bytes, const char *what);
+void ctx_free_all(Context &c);
+bool ctx_upload_m(Context &c, const void *host, bool host_is_f64);
+bool ctx_query_device_info(Context &c);
+int timing_entry(Context &c, const char *name);
+void timing_begin(Context &c, int entry, hipEvent_t *a);
+void timing_end(Context &c, int entry, hipEvent_t a);
+void timing_resolve(Context &c);
+
+// demag.hip
+bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_device);
+// Evaluates H_demag of the field `src` (3n elements, FP64 when src_f64) into
+// `out` (Real).  Inactive cells get zero.
+void demag_eval(Context &c, const void *src, bool src_f64, void *out);
+bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double dz, bool thin_film_2d,
+                         double *out, std::string &err);
+
+// newell.cpp: Newell tensor on the non-negative displacement octant,
+// out[c][k][j][i] for i < nx, j < ny, k < nz, c = xx, yy, zz, xy, xz, yz.
+void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz, std::vector<double> &out);
+
+// llg.hip
+int integrator_step(Context &c, double dt, fullmag_fdm_step_stats *st);
+bool refresh_observables(Context &c);
+bool materialize_observables(Context &c);  // fills h_ex_obs / h_eff_obs at m
+bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
+                   uint32_t z0, uint32_t zs, void *dev_out, bool out_f64);
+bool alloc_integrator_buffers(Context &c);
+
+}  // namespace fm

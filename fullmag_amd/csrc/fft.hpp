@@ -1,0 +1,213 @@
+// fft.hpp -- line FFTs staged in LDS for the demag convolution.
+//
+// A "line" is one 1-D transform of length n held in LDS.  A workgroup holds
+// several lines; each line is worked on by `threads` = n / (smallest radix)
+// lanes, one butterfly per lane per pass, so every pass is read-all /
+// barrier / write-all in place and a line needs only n complex slots of LDS.
+//
+// The passes are Stockham autosort DIT (natural-order output, no bit
+// reversal): for radix R and span ns (product of the radices already
+// applied) butterfly j reads x[j + r n/R], multiplies by w^(r k n/(ns R))
+// with k = j mod ns, applies the R-point DFT and writes x[(j/ns) ns R + k + r ns].
+// Twiddles come from a per-length table w[t] = exp(-2 pi i t / n) rounded
+// from long double on the host, so FP64 transforms carry only butterfly
+// rounding.  Forward is unnormalised exp(-i...), inverse unnormalised
+// exp(+i...), the convention of crates/fullmag-engine/src/lib.rs:343-386.
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace fm {
+
+constexpr int kMaxPass = 16;
+
+struct LinePlan {
+    int n = 0;        // line length
+    int npass = 0;    // number of Stockham passes
+    int threads = 0;  // lanes per line = n / min radix
+    int radix[kMaxPass] = {};
+};
+
+// Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13}.  prefer_wide first
+// tries power-of-two radices of 8 and 16 only.  Returns false (with `err`)
+// for lengths with a prime factor above 13 or needing more than 1024 lanes.
+bool plan_line(int n, LinePlan &lp, std::string &err, bool prefer_wide = false);
+
+// w[t] = exp(-2 pi i t / n), t in [0, n), correctly rounded from long double.
+std::vector<cplx<double>> twiddle_table(int n);
+
+// ---------------------------------------------------------------------------
+// Device butterflies
+// ---------------------------------------------------------------------------
+
+// cos/sin of 2 pi m / 16, m = 0..15 (exact decimal expansions).
+#define FM_C1 0.92387953251128675613
+#define FM_S1 0.38268343236508977173
+#define FM_H 0.70710678118654752440
+
+template <typename T>
+__device__ inline cplx<T> w16(int m, bool inv) {
+    // returns exp(-+ 2 pi i m / 16) (forward: minus sign)
+    const T c[16] = {T(1), T(FM_C1), T(FM_H), T(FM_S1), T(0), T(-FM_S1), T(-FM_H), T(-FM_C1),
+                     T(-1), T(-FM_C1), T(-FM_H), T(-FM_S1), T(0), T(FM_S1), T(FM_H), T(FM_C1)};
+    const T s[16] = {T(0), T(FM_S1), T(FM_H), T(FM_C1), T(1), T(FM_C1), T(FM_H), T(FM_S1),
+                     T(0), T(-FM_S1), T(-FM_H), T(-FM_C1), T(-1), T(-FM_C1), T(-FM_H), T(-FM_S1)};
+    m &= 15;
+    return {c[m], inv ? s[m] : -s[m]};
+}
+
+template <typename T>
+__device__ inline void dft2(cplx<T> &a, cplx<T> &b) {
+    cplx<T> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+// In-place 4-point DFT of (v[0], v[s], v[2s], v[3s]).
+template <typename T>
+__device__ inline void dft4(cplx<T> &a0, cplx<T> &a1, cplx<T> &a2, cplx<T> &a3, bool inv) {
+    cplx<T> s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    cplx<T> s1 = cadd(a1, a3), d1 = csub(a1, a3);
+    // forward: -i * d1 ; inverse: +i * d1
+    cplx<T> jd1 = inv ? cplx<T>{-d1.y, d1.x} : cplx<T>{d1.y, -d1.x};
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = cadd(d0, jd1);
+    a3 = csub(d0, jd1);
+}
+
+// 8-point DFT: inner 4-point DFTs over even/odd samples, twiddle w8^(r2 q1),
+// outer 2-point DFTs: X[q1 + 4 q2].
+template <typename T>
+__device__ inline void dft8(cplx<T> *v, bool inv) {
+    cplx<T> u0[4] = {v[0], v[2], v[4], v[6]};
+    cplx<T> u1[4] = {v[1], v[3], v[5], v[7]};
+    dft4(u0[0], u0[1], u0[2], u0[3], inv);
+    dft4(u1[0], u1[1], u1[2], u1[3], inv);
+#pragma unroll
+    for (int q1 = 1; q1 < 4; ++q1) u1[q1] = cmul(u1[q1], w16<T>(2 * q1, inv));
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1) {
+        cplx<T> a = u0[q1], b = u1[q1];
+        dft2(a, b);
+        v[q1] = a;
+        v[q1 + 4] = b;
+    }
+}
+
+// 16-point DFT as 4 x 4: inner DFTs over r1 (stride 4), twiddle w16^(r2 q1),
+// outer DFTs over r2: X[q1 + 4 q2].
+template <typename T>
+__device__ inline void dft16(cplx<T> *v, bool inv) {
+    cplx<T> u[4][4];
+#pragma unroll
+    for (int r2 = 0; r2 < 4; ++r2) {
+        cplx<T> a0 = v[r2], a1 = v[r2 + 4], a2 = v[r2 + 8], a3 = v[r2 + 12];
+        dft4(a0, a1, a2, a3, inv);
+        u[r2][0] = a0; u[r2][1] = a1; u[r2][2] = a2; u[r2][3] = a3;
+    }
+#pragma unroll
+    for (int r2 = 1; r2 < 4; ++r2)
+#pragma unroll
+        for (int q1 = 1; q1 < 4; ++q1) u[r2][q1] = cmul(u[r2][q1], w16<T>(r2 * q1, inv));
+#pragma unroll
+    for (int q1 = 0; q1 < 4; ++q1) {
+        cplx<T> a0 = u[0][q1], a1 = u[1][q1], a2 = u[2][q1], a3 = u[3][q1];
+        dft4(a0, a1, a2, a3, inv);
+        v[q1] = a0; v[q1 + 4] = a1; v[q1 + 8] = a2; v[q1 + 12] = a3;
+    }
+}
+
+// Odd prime R: direct DFT with roots from the line's twiddle table.
+template <typename T, int R>
+__device__ inline void dft_prime(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
+    cplx<T> out[R];
+    const int step = n / R;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        cplx<T> acc = v[0];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            cplx<T> w = tw[((r * q) % R) * step];
+            if (inv) w.y = -w.y;
+            acc = cadd(acc, cmul(v[r], w));
+        }
+        out[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+}
+
+template <typename T, int R>
+__device__ inline void dft_any(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
+    if constexpr (R == 2) {
+        dft2(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], inv);
+    } else if constexpr (R == 8) {
+        dft8(v, inv);
+    } else if constexpr (R == 16) {
+        dft16(v, inv);
+    } else {
+        dft_prime<T, R>(v, tw, n, inv);
+    }
+}
+
+// One Stockham pass over a line in LDS.  Every lane of the workgroup must
+// call it (it contains barriers); `t` is the lane's index within its line.
+template <typename T, int R>
+__device__ inline void stockham_pass(cplx<T> *line, int n, int ns, int t, const cplx<T> *tw, bool inv) {
+    const int nr = n / R;
+    const bool act = t < nr;
+    cplx<T> v[R];
+    int base = 0;
+    if (act) {
+        const int k = t % ns;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = line[t + r * nr];
+        if (k != 0) {
+            // r k n/(ns R) < n for r < R, k < ns: no modular reduction needed
+            const int stride = n / (ns * R);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                cplx<T> w = tw[r * k * stride];
+                if (inv) w.y = -w.y;
+                v[r] = cmul(v[r], w);
+            }
+        }
+        dft_any<T, R>(v, tw, n, inv);
+        base = (t / ns) * ns * R + k;
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) line[base + r * ns] = v[r];
+    }
+    __syncthreads();
+}
+
+// Full transform of one line.  Uniform across the workgroup.
+template <typename T>
+__device__ inline void line_fft(cplx<T> *line, const LinePlan &lp, int t, const cplx<T> *tw, bool inv) {
+    int ns = 1;
+    for (int p = 0; p < lp.npass; ++p) {
+        const int R = lp.radix[p];
+        switch (R) {
+            case 2: stockham_pass<T, 2>(line, lp.n, ns, t, tw, inv); break;
+            case 3: stockham_pass<T, 3>(line, lp.n, ns, t, tw, inv); break;
+            case 4: stockham_pass<T, 4>(line, lp.n, ns, t, tw, inv); break;
+            case 5: stockham_pass<T, 5>(line, lp.n, ns, t, tw, inv); break;
+            case 7: stockham_pass<T, 7>(line, lp.n, ns, t, tw, inv); break;
+            case 8: stockham_pass<T, 8>(line, lp.n, ns, t, tw, inv); break;
+            case 11: stockham_pass<T, 11>(line, lp.n, ns, t, tw, inv); break;
+            case 13: stockham_pass<T, 13>(line, lp.n, ns, t, tw, inv); break;
+            case 16: stockham_pass<T, 16>(line, lp.n, ns, t, tw, inv); break;
+            default: break;
+        }
+        ns *= R;
+    }
+}
+
+}  // namespace fm

@@ -1,0 +1,97 @@
+// fft_plan.cpp -- host planning for the LDS line FFTs (see fft.hpp).
+#include "fft.hpp"
+
+#include <cmath>
+
+namespace fm {
+
+namespace {
+
+void pow2_radices(int k, bool wide, std::vector<int> &out) {
+    // k = log2 of the power-of-two factor.  Default: as many radix-16 passes
+    // as possible with a radix-8/4 tail (few LDS round trips, >= 4 lanes of
+    // work per pass).  `wide` restricts to radices 8 and 16 so that long
+    // lines fit in 1024 lanes.
+    if (k == 0) return;
+    if (wide) {
+        int y = k / 4;
+        while (y >= 0 && (k - 4 * y) % 3 != 0) --y;
+        if (y < 0) { out.push_back(-1); return; }
+        for (int i = 0; i < y; ++i) out.push_back(16);
+        for (int i = 0; i < (k - 4 * y) / 3; ++i) out.push_back(8);
+        return;
+    }
+    int a = k / 4, r = k % 4;
+    switch (r) {
+        case 0: for (int i = 0; i < a; ++i) out.push_back(16); break;
+        case 1:
+            if (a >= 1) {
+                for (int i = 0; i < a - 1; ++i) out.push_back(16);
+                out.push_back(8);
+                out.push_back(4);
+            } else {
+                out.push_back(2);
+            }
+            break;
+        case 2: for (int i = 0; i < a; ++i) out.push_back(16); out.push_back(4); break;
+        case 3: for (int i = 0; i < a; ++i) out.push_back(16); out.push_back(8); break;
+    }
+}
+
+}  // namespace
+
+bool plan_line(int n, LinePlan &lp, std::string &err, bool prefer_wide) {
+    lp = LinePlan{};
+    if (n < 1) {
+        err = "FFT line length must be >= 1";
+        return false;
+    }
+    lp.n = n;
+    if (n == 1) {
+        lp.npass = 0;
+        lp.threads = 1;
+        return true;
+    }
+    int m = n, k = 0;
+    while (m % 2 == 0) { m /= 2; ++k; }
+    std::vector<int> odd;
+    for (int p : {3, 5, 7, 11, 13})
+        while (m % p == 0) { odd.push_back(p); m /= p; }
+    if (m != 1) {
+        err = "padded FFT length " + std::to_string(n) +
+              " has a prime factor above 13, which the line FFT does not support";
+        return false;
+    }
+    const int order[2][2] = {{0, 1}, {1, 0}};
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const int wide = order[prefer_wide ? 1 : 0][attempt];
+        std::vector<int> rad;
+        pow2_radices(k, wide != 0, rad);
+        if (!rad.empty() && rad.back() < 0) continue;
+        for (int p : odd) rad.push_back(p);
+        if ((int)rad.size() > kMaxPass) continue;
+        int rmin = 1 << 30;
+        for (int r : rad) rmin = r < rmin ? r : rmin;
+        int threads = n / rmin;
+        if (threads > 1024) continue;
+        lp.npass = (int)rad.size();
+        for (int i = 0; i < lp.npass; ++i) lp.radix[i] = rad[i];
+        lp.threads = threads;
+        return true;
+    }
+    err = "padded FFT length " + std::to_string(n) + " needs more than 1024 lanes per line";
+    return false;
+}
+
+std::vector<cplx<double>> twiddle_table(int n) {
+    std::vector<cplx<double>> w(n > 0 ? n : 1);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int t = 0; t < n; ++t) {
+        long double a = -two_pi * (long double)t / (long double)n;
+        w[t] = {(double)cosl(a), (double)sinl(a)};
+    }
+    if (n <= 0) w[0] = {1.0, 0.0};
+    return w;
+}
+
+}  // namespace fm

@@ -1,0 +1,832 @@
+// llg.hip -- fused field/torque/stage kernels and the time integrators.
+//
+// Replaces native/backends/fdm/src/{exchange,llg,llg_dp45,llg_rk4,llg_rk23,
+// llg_abm3,reductions}_fp64.cu and their fp32 twins.  One "stage" kernel does
+// per cell what the reference spreads over exchange, combine, rhs, predictor
+// and reduction kernels:
+//   H_eff = H_ex(src) + H_demag + H_ext          (stencil read from `src`)
+//   k     = -gamma/(1+alpha^2) (p m x H + alpha m x (m x H))
+//   out   = normalise(m0 + dt (sum_j a_j k_j + a_new k))
+// optionally accumulating the step diagnostics (compensated energy sums and
+// max norms) or the adaptive error norm.  Semantics follow the CPU reference
+// crates/fullmag-engine/src/lib.rs (exchange :2247-2299, rhs :2584-2595,
+// energies :2653-2703, Heun :1217-1249, RK4 :1254-1302, RK23 :1332-1407,
+// DP45 :1412-1568, ABM3 :1573-1648, error norm :1653-1674).
+#include "context.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <utility>
+
+namespace fm {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kNStat = 9;  // s_ex, c_ex, s_d, c_d, s_ext, c_ext, max|Heff|^2, max|Hd|^2, max|k|^2
+
+struct FieldArgs {
+    int nx, ny, nz;
+    int64_t n;
+    double idx2, idy2, idz2;  // 1 / d^2
+    double pref;              // 2 A / (mu0 Ms) for uniform A and Ms
+    double a_uniform;         // A when ms_cell is set and no table
+    double two_over_mu0;
+    double pre_lut;           // 2 / (mu0 Ms) for the per-pair table
+    const uint8_t *mask;
+    const uint32_t *region;  // non-null: per-pair A from `lut`
+    const double *lut;
+    const double *ms_cell;
+    int exchange, has_ext, precession;
+    double hx, hy, hz;
+    double gamma_bar, alpha;
+};
+
+// Exchange field at cell i in the arithmetic type C of the source, with
+// clamped (Neumann) neighbours and inactive neighbours replaced by the centre
+// (lib.rs:2270-2283).
+template <typename C, typename S>
+__device__ inline void exchange_at(const FieldArgs &f, const S *__restrict__ m, int64_t i, C h[3]) {
+    const int64_t nxy = (int64_t)f.nx * f.ny;
+    const int z = (int)(i / nxy);
+    const int64_t r = i - (int64_t)z * nxy;
+    const int y = (int)(r / f.nx);
+    const int x = (int)(r - (int64_t)y * f.nx);
+    int64_t xm = x > 0 ? i - 1 : i, xp = x < f.nx - 1 ? i + 1 : i;
+    int64_t ym = y > 0 ? i - f.nx : i, yp = y < f.ny - 1 ? i + f.nx : i;
+    int64_t zm = z > 0 ? i - nxy : i, zp = z < f.nz - 1 ? i + nxy : i;
+    if (f.mask) {
+        if (!f.mask[xm]) xm = i;
+        if (!f.mask[xp]) xp = i;
+        if (!f.mask[ym]) ym = i;
+        if (!f.mask[yp]) yp = i;
+        if (!f.mask[zm]) zm = i;
+        if (!f.mask[zp]) zp = i;
+    }
+    const C ix = C(f.idx2), iy = C(f.idy2), iz = C(f.idz2);
+    if (!f.region) {
+        const C pref = f.ms_cell ? C(f.two_over_mu0 * f.a_uniform / f.ms_cell[i]) : C(f.pref);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(mc[i]);
+            const C lap = (C(mc[xp]) - C(2) * cc + C(mc[xm])) * ix + (C(mc[yp]) - C(2) * cc + C(mc[ym])) * iy +
+                          (C(mc[zp]) - C(2) * cc + C(mc[zm])) * iz;
+            h[c] = pref * lap;
+        }
+    } else {
+        // per-pair coupling (native/backends/fdm/src/exchange_fp64.cu:80-114)
+        const uint32_t ri = f.region[i];
+        const double *row = f.lut + (size_t)ri * FULLMAG_FDM_MAX_EXCHANGE_REGIONS;
+        const C axm = C(row[f.region[xm]]), axp = C(row[f.region[xp]]);
+        const C aym = C(row[f.region[ym]]), ayp = C(row[f.region[yp]]);
+        const C azm = C(row[f.region[zm]]), azp = C(row[f.region[zp]]);
+        const double msi = f.ms_cell ? f.ms_cell[i] : 0.0;
+        const C pre = f.ms_cell ? C(f.two_over_mu0 / msi) : C(f.pre_lut);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(mc[i]);
+            const C s = (axp * (C(mc[xp]) - cc) + axm * (C(mc[xm]) - cc)) * ix +
+                        (ayp * (C(mc[yp]) - cc) + aym * (C(mc[ym]) - cc)) * iy +
+                        (azp * (C(mc[zp]) - cc) + azm * (C(mc[zm]) - cc)) * iz;
+            h[c] = pre * s;
+        }
+    }
+}
+
+template <typename R>
+__device__ inline void torque(const FieldArgs &f, const R m[3], const R h[3], R k[3]) {
+    const R p0 = m[1] * h[2] - m[2] * h[1];
+    const R p1 = m[2] * h[0] - m[0] * h[2];
+    const R p2 = m[0] * h[1] - m[1] * h[0];
+    const R d0 = m[1] * p2 - m[2] * p1;
+    const R d1 = m[2] * p0 - m[0] * p2;
+    const R d2 = m[0] * p1 - m[1] * p0;
+    const R a = R(f.alpha), g = R(-f.gamma_bar);
+    const R pe = f.precession ? R(1) : R(0);
+    k[0] = (pe * p0 + d0 * a) * g;
+    k[1] = (pe * p1 + d1 * a) * g;
+    k[2] = (pe * p2 + d2 * a) * g;
+}
+
+__device__ inline void neu_add(double &s, double &c, double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+}
+
+enum { STAGE_PLAIN = 0, STAGE_STATS = 1, STAGE_ERROR = 2 };
+
+template <typename State, typename Real, typename Src, typename Out>
+struct StageArgs {
+    FieldArgs f;
+    const Src *src;
+    const Real *hd;    // demag field at src, or null
+    const State *m0;   // base point of the update (MODE_PLAIN with out)
+    const Real *kp[6];
+    double ap[6];
+    int nprev;
+    double anew, dt;
+    Real *kout;
+    Out *out;
+    double *partials;
+};
+
+template <typename State, typename Real, typename Src, typename Out, int MODE>
+__global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Src, Out> a) {
+    const FieldArgs &f = a.f;
+    const int64_t n = f.n;
+    double acc[kNStat];
+#pragma unroll
+    for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool act = !f.mask || f.mask[i];
+        Real m[3], hex[3] = {Real(0), Real(0), Real(0)}, hdv[3] = {Real(0), Real(0), Real(0)}, h[3], k[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) m[c] = Real(a.src[c * n + i]);
+        if (act) {
+            if (f.exchange) {
+                Src he[3];
+                exchange_at<Src, Src>(f, a.src, i, he);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
+            }
+            if (a.hd) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hdv[c] = a.hd[c * n + i];
+            }
+            const Real he0 = f.has_ext ? Real(f.hx) : Real(0), he1 = f.has_ext ? Real(f.hy) : Real(0),
+                       he2 = f.has_ext ? Real(f.hz) : Real(0);
+            h[0] = (hex[0] + hdv[0]) + he0;
+            h[1] = (hex[1] + hdv[1]) + he1;
+            h[2] = (hex[2] + hdv[2]) + he2;
+            torque(f, m, h, k);
+        } else {
+            h[0] = h[1] = h[2] = Real(0);
+            k[0] = k[1] = k[2] = Real(0);
+        }
+        if (a.kout) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.kout[c * n + i] = k[c];
+        }
+        if (MODE == STAGE_PLAIN && a.out) {
+            State v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                State s = State(a.anew) * State(k[c]);
+                for (int j = 0; j < a.nprev; ++j) s += State(a.ap[j]) * State(a.kp[j][c * n + i]);
+                v[c] = a.m0[c * n + i] + State(a.dt) * s;
+            }
+            const State nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            const State inv = nrm > State(0) ? State(1) / nrm : State(0);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.out[c * n + i] = Out(v[c] * inv);
+        }
+        if (MODE == STAGE_STATS && act) {
+            // m . H per term in FP64 with the state's own m
+            double md[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) md[c] = double(a.src[c * n + i]);
+            const double w = f.ms_cell ? f.ms_cell[i] : 1.0;
+            neu_add(acc[0], acc[1], w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2])));
+            neu_add(acc[2], acc[3], w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2])));
+            if (f.has_ext) neu_add(acc[4], acc[5], w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz));
+            const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
+            const double d2 = double(hdv[0]) * hdv[0] + double(hdv[1]) * hdv[1] + double(hdv[2]) * hdv[2];
+            const double k2 = double(k[0]) * k[0] + double(k[1]) * k[1] + double(k[2]) * k[2];
+            acc[6] = fmax(acc[6], h2);
+            acc[7] = fmax(acc[7], d2);
+            acc[8] = fmax(acc[8], k2);
+        }
+        if (MODE == STAGE_ERROR) {
+            // lib.rs:1653-1674: |dt sum_j E_j k_j| in FP64
+            double e[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double s = a.anew * double(k[c]);
+                for (int j = 0; j < a.nprev; ++j) s += a.ap[j] * double(a.kp[j][c * n + i]);
+                e[c] = s * a.dt;
+            }
+            acc[0] = fmax(acc[0], e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        }
+    }
+    if (MODE == STAGE_PLAIN) return;
+    __shared__ double red[kNStat][kBlock];
+    const int tid = threadIdx.x;
+    const int nq = MODE == STAGE_STATS ? kNStat : 1;
+    for (int q = 0; q < nq; ++q) red[q][tid] = acc[q];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            if (MODE == STAGE_STATS) {
+                for (int e = 0; e < 3; ++e) {
+                    double s = red[2 * e][tid], c = red[2 * e + 1][tid];
+                    neu_add(s, c, red[2 * e][tid + off]);
+                    red[2 * e][tid] = s;
+                    red[2 * e + 1][tid] = c + red[2 * e + 1][tid + off];
+                }
+                for (int q = 6; q < 9; ++q) red[q][tid] = fmax(red[q][tid], red[q][tid + off]);
+            } else {
+                red[0][tid] = fmax(red[0][tid], red[0][tid + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int q = 0; q < nq; ++q) a.partials[(int64_t)blockIdx.x * nq + q] = red[q][0];
+}
+
+// Reduces per-block partials into out[0..nq).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) finalize_kernel(const double *partials, int nblocks, double *out) {
+    constexpr int nq = MODE == STAGE_STATS ? kNStat : 1;
+    __shared__ double red[nq][kBlock];
+    const int tid = threadIdx.x;
+    double acc[nq];
+    for (int q = 0; q < nq; ++q) acc[q] = 0.0;
+    for (int b = tid; b < nblocks; b += kBlock) {
+        const double *p = partials + (int64_t)b * nq;
+        if (MODE == STAGE_STATS) {
+            for (int e = 0; e < 3; ++e) {
+                neu_add(acc[2 * e], acc[2 * e + 1], p[2 * e]);
+                acc[2 * e + 1] += p[2 * e + 1];
+            }
+            for (int q = 6; q < 9; ++q) acc[q] = fmax(acc[q], p[q]);
+        } else {
+            acc[0] = fmax(acc[0], p[0]);
+        }
+    }
+    for (int q = 0; q < nq; ++q) red[q][tid] = acc[q];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            if (MODE == STAGE_STATS) {
+                for (int e = 0; e < 3; ++e) {
+                    double s = red[2 * e][tid], c = red[2 * e + 1][tid];
+                    neu_add(s, c, red[2 * e][tid + off]);
+                    red[2 * e][tid] = s;
+                    red[2 * e + 1][tid] = c + red[2 * e + 1][tid + off];
+                }
+                for (int q = 6; q < 9; ++q) red[q][tid] = fmax(red[q][tid], red[q][tid + off]);
+            } else {
+                red[0][tid] = fmax(red[0][tid], red[0][tid + off]);
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int q = 0; q < nq; ++q) out[q] = red[q][0];
+}
+
+// out = normalise(m0 + dt sum_j a_j k_j), no field evaluation (DP45 FSAL
+// first stage, ABM3 predictor).
+template <typename State, typename Real, typename Out>
+__global__ void __launch_bounds__(kBlock) combine_kernel(const State *m0, StageArgs<State, Real, State, Out> a) {
+    const int64_t n = a.f.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        State v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            State s = State(0);
+            for (int j = 0; j < a.nprev; ++j) s += State(a.ap[j]) * State(a.kp[j][c * n + i]);
+            v[c] = m0[c * n + i] + State(a.dt) * s;
+        }
+        const State nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        const State inv = nrm > State(0) ? State(1) / nrm : State(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.out[c * n + i] = Out(v[c] * inv);
+    }
+}
+
+// Field observables at m: H_ex and H_eff (Real).
+template <typename State, typename Real>
+__global__ void __launch_bounds__(kBlock) observe_kernel(FieldArgs f, const State *m, const Real *hd, Real *hex_out,
+                                                         Real *heff_out) {
+    const int64_t n = f.n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool act = !f.mask || f.mask[i];
+        Real hex[3] = {Real(0), Real(0), Real(0)}, heff[3] = {Real(0), Real(0), Real(0)};
+        if (act) {
+            if (f.exchange) {
+                State he[3];
+                exchange_at<State, State>(f, m, i, he);
+                for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
+            }
+            const Real ext[3] = {f.has_ext ? Real(f.hx) : Real(0), f.has_ext ? Real(f.hy) : Real(0),
+                                 f.has_ext ? Real(f.hz) : Real(0)};
+            for (int c = 0; c < 3; ++c) heff[c] = (hex[c] + (hd ? hd[c * n + i] : Real(0))) + ext[c];
+        }
+        for (int c = 0; c < 3; ++c) {
+            hex_out[c * n + i] = hex[c];
+            heff_out[c * n + i] = heff[c];
+        }
+    }
+}
+
+// Box-average preview (native/backends/fdm/src/context.cu:222-286 semantics).
+template <typename In, typename OutT>
+__global__ void preview_kernel(const In *fx, const In *fy, const In *fz, uint32_t nx, uint32_t ny, uint32_t nz,
+                               uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, OutT *out) {
+    const uint64_t count = (uint64_t)pnx * pny * pnz;
+    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= count) return;
+    const uint32_t bx = (uint32_t)(p % pnx), by = (uint32_t)((p / pnx) % pny), bz = (uint32_t)(p / ((uint64_t)pnx * pny));
+    uint32_t x0 = (uint32_t)((uint64_t)bx * nx / pnx), x1 = (uint32_t)((uint64_t)(bx + 1) * nx / pnx);
+    if (x1 <= x0) x1 = x0 + 1;
+    if (x1 > nx) x1 = nx;
+    uint32_t y0 = (uint32_t)((uint64_t)by * ny / pny), y1 = (uint32_t)((uint64_t)(by + 1) * ny / pny);
+    if (y1 <= y0) y1 = y0 + 1;
+    if (y1 > ny) y1 = ny;
+    uint32_t za = z0 + bz * zs;
+    if (za >= nz) za = nz - 1;
+    uint32_t zb = z0 + (bz + 1) * zs;
+    if (zb <= za) zb = za + 1;
+    if (zb > nz) zb = nz;
+    double sx = 0, sy = 0, sz = 0, cnt = 0;
+    for (uint32_t z = za; z < zb; ++z)
+        for (uint32_t y = y0; y < y1; ++y)
+            for (uint32_t x = x0; x < x1; ++x) {
+                const uint64_t i = ((uint64_t)z * ny + y) * nx + x;
+                sx += double(fx[i]);
+                sy += double(fy[i]);
+                sz += double(fz[i]);
+                cnt += 1.0;
+            }
+    out[p * 3 + 0] = OutT(sx / cnt);
+    out[p * 3 + 1] = OutT(sy / cnt);
+    out[p * 3 + 2] = OutT(sz / cnt);
+}
+
+FieldArgs field_args(const Context &c) {
+    FieldArgs f{};
+    f.nx = (int)c.nx;
+    f.ny = (int)c.ny;
+    f.nz = (int)c.nz;
+    f.n = (int64_t)c.n;
+    f.idx2 = 1.0 / (c.dx * c.dx);
+    f.idy2 = 1.0 / (c.dy * c.dy);
+    f.idz2 = 1.0 / (c.dz * c.dz);
+    f.a_uniform = c.use_lut ? c.a : c.a_exchange;
+    f.pref = 2.0 * f.a_uniform / (kMu0 * c.ms);
+    f.two_over_mu0 = 2.0 / kMu0;
+    f.pre_lut = 2.0 / (kMu0 * c.ms);
+    f.mask = c.d_mask;
+    f.region = c.use_lut ? c.d_region : nullptr;
+    f.lut = c.use_lut ? c.d_lut : nullptr;
+    f.ms_cell = c.d_ms_cell;
+    f.exchange = c.exchange ? 1 : 0;
+    f.has_ext = c.has_ext ? 1 : 0;
+    f.precession = c.precession ? 1 : 0;
+    f.hx = c.hext[0];
+    f.hy = c.hext[1];
+    f.hz = c.hext[2];
+    f.gamma_bar = c.gamma / (1.0 + c.alpha * c.alpha);
+    f.alpha = c.alpha;
+    return f;
+}
+
+struct Timed {
+    Context &c;
+    int entry = -1;
+    hipEvent_t ev{};
+    Timed(Context &cc, const char *name) : c(cc) {
+        if (c.timing) {
+            entry = timing_entry(c, name);
+            timing_begin(c, entry, &ev);
+        }
+    }
+    ~Timed() {
+        if (entry >= 0) timing_end(c, entry, ev);
+    }
+};
+
+// Typed driver for one precision mode.
+template <typename State, typename Real>
+struct Driver {
+    Context &c;
+    explicit Driver(Context &cc) : c(cc) {}
+
+    State *m() { return static_cast<State *>(c.m); }
+    State *mst() { return static_cast<State *>(c.mst); }
+    Real *hd() { return static_cast<Real *>(c.hd); }
+    Real *mt(int i) { return static_cast<Real *>(c.mt[i]); }
+    Real *k(int i) { return static_cast<Real *>(c.k[i]); }
+    static constexpr bool kStateF64 = sizeof(State) == 8;
+    static constexpr bool kRealF64 = sizeof(Real) == 8;
+
+    unsigned grid_plain() const { return (unsigned)std::max<uint64_t>(1, cdiv(c.n, kBlock)); }
+    unsigned grid_reduce() const {
+        return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cdiv(c.n, kBlock), (uint64_t)c.max_partial_blocks));
+    }
+
+    template <typename Src, typename Out>
+    StageArgs<State, Real, Src, Out> args(const Src *src, const Real *hdp) {
+        StageArgs<State, Real, Src, Out> a{};
+        a.f = field_args(c);
+        a.src = src;
+        a.hd = c.demag ? hdp : nullptr;
+        a.m0 = m();
+        a.nprev = 0;
+        a.anew = 0;
+        a.dt = 0;
+        a.kout = nullptr;
+        a.out = nullptr;
+        a.partials = c.d_partials;
+        return a;
+    }
+
+    template <typename Src, typename Out>
+    void stage(const Src *src, Real *kout, Out *out, std::initializer_list<std::pair<int, double>> prev, double anew,
+               double dt) {
+        StageArgs<State, Real, Src, Out> a = args<Src, Out>(src, hd());
+        a.kout = kout;
+        a.out = out;
+        for (auto &p : prev) {
+            a.kp[a.nprev] = k(p.first);
+            a.ap[a.nprev] = p.second;
+            ++a.nprev;
+        }
+        a.anew = anew;
+        a.dt = dt;
+        Timed t(c, "llg_stage");
+        hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), dim3(grid_plain()), dim3(kBlock), 0,
+                           c.stream, a);
+    }
+
+    template <typename Out>
+    void combine(Out *out, std::initializer_list<std::pair<int, double>> prev, double dt) {
+        StageArgs<State, Real, State, Out> a = args<State, Out>(m(), hd());
+        a.out = out;
+        for (auto &p : prev) {
+            a.kp[a.nprev] = k(p.first);
+            a.ap[a.nprev] = p.second;
+            ++a.nprev;
+        }
+        a.dt = dt;
+        Timed t(c, "llg_combine");
+        hipLaunchKernelGGL((combine_kernel<State, Real, Out>), dim3(grid_plain()), dim3(kBlock), 0, c.stream, m(), a);
+    }
+
+    // Max over cells of |dt sum_j w_j k_j|, evaluating the last slope at src.
+    double error_stage(const State *src, Real *kout, std::initializer_list<std::pair<int, double>> prev, double wnew,
+                       double dt) {
+        StageArgs<State, Real, State, Real> a = args<State, Real>(src, hd());
+        a.kout = kout;
+        for (auto &p : prev) {
+            a.kp[a.nprev] = k(p.first);
+            a.ap[a.nprev] = p.second;
+            ++a.nprev;
+        }
+        a.anew = wnew;
+        a.dt = dt;
+        const unsigned g = grid_reduce();
+        {
+            Timed t(c, "llg_error");
+            hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_ERROR>), dim3(g), dim3(kBlock), 0,
+                               c.stream, a);
+        }
+        hipLaunchKernelGGL((finalize_kernel<STAGE_ERROR>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
+                           c.d_result);
+        hipMemcpyAsync(c.h_result, c.d_result, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+        hipStreamSynchronize(c.stream);
+        timing_resolve(c);
+        return std::sqrt(c.h_result[0]);
+    }
+
+    void demag_at_state(const State *src) {
+        if (c.demag) demag_eval(c, src, kStateF64, hd());
+    }
+    void demag_at_stage(const Real *src) {
+        if (c.demag) demag_eval(c, src, kRealF64, hd());
+    }
+    void ensure_hd() {
+        if (!c.hd_at_m) demag_at_state(m());
+        c.hd_at_m = true;
+    }
+
+    // Diagnostics at m (hd must hold H_demag(m)); optionally stores dm/dt.
+    void stats(Real *kout) {
+        StageArgs<State, Real, State, Real> a = args<State, Real>(m(), hd());
+        a.kout = kout;
+        const unsigned g = grid_reduce();
+        {
+            Timed t(c, "llg_stats");
+            hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS>), dim3(g), dim3(kBlock), 0,
+                               c.stream, a);
+        }
+        hipLaunchKernelGGL((finalize_kernel<STAGE_STATS>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
+                           c.d_result);
+        hipMemcpyAsync(c.h_result, c.d_result, kNStat * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+    }
+
+    // lib.rs:1217-1249
+    void heun(double dt) {
+        ensure_hd();
+        stage<State, Real>(m(), k(0), mt(0), {}, 1.0, dt);
+        demag_at_stage(mt(0));
+        stage<Real, State>(mt(0), nullptr, m(), {{0, 0.5}}, 0.5, dt);
+        demag_at_state(m());
+        c.hd_at_m = true;
+    }
+
+    // lib.rs:1254-1302
+    void rk4(double dt) {
+        ensure_hd();
+        stage<State, Real>(m(), k(0), mt(0), {}, 0.5, dt);
+        demag_at_stage(mt(0));
+        stage<Real, Real>(mt(0), k(1), mt(1), {}, 0.5, dt);
+        demag_at_stage(mt(1));
+        stage<Real, Real>(mt(1), k(2), mt(0), {}, 1.0, dt);
+        demag_at_stage(mt(0));
+        stage<Real, State>(mt(0), nullptr, m(), {{0, 1.0 / 6.0}, {1, 2.0 / 6.0}, {2, 2.0 / 6.0}}, 1.0 / 6.0, dt);
+        demag_at_state(m());
+        c.hd_at_m = true;
+    }
+
+    // lib.rs:1332-1407 (no FSAL); returns the suggested next dt.
+    double rk23(double &dt) {
+        dt = std::max(std::min(dt, c.dt_max), c.dt_min);
+        for (;;) {
+            ensure_hd();
+            stage<State, Real>(m(), k(0), mt(0), {}, 0.5, dt);
+            demag_at_stage(mt(0));
+            stage<Real, Real>(mt(0), k(1), mt(1), {}, 0.75, dt);
+            demag_at_stage(mt(1));
+            stage<Real, State>(mt(1), k(2), mst(), {{0, 2.0 / 9.0}, {1, 1.0 / 3.0}}, 4.0 / 9.0, dt);
+            demag_at_state(mst());
+            const double err =
+                error_stage(mst(), nullptr, {{0, -5.0 / 72.0}, {1, 1.0 / 12.0}, {2, 1.0 / 9.0}}, -1.0 / 8.0, dt);
+            if (err <= c.max_err || dt <= c.dt_min) {
+                std::swap(c.m, c.mst);
+                c.hd_at_m = true;
+                return std::min(std::max(c.headroom * dt * std::pow(c.max_err / std::max(err, 1e-30), 1.0 / 3.0),
+                                         c.dt_min),
+                                c.dt_max);
+            }
+            c.hd_at_m = false;
+            dt = std::min(std::max(c.headroom * dt * std::pow(c.max_err / err, 1.0 / 3.0), c.dt_min), c.dt_max);
+        }
+    }
+
+    // lib.rs:1412-1568 (Dormand-Prince 5(4), FSAL)
+    double dp45(double &dt) {
+        constexpr double A21 = 1.0 / 5.0;
+        constexpr double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+        constexpr double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+        constexpr double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0,
+                         A54 = -212.0 / 729.0;
+        constexpr double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0, A64 = 49.0 / 176.0,
+                         A65 = -5103.0 / 18656.0;
+        constexpr double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0, B5 = -2187.0 / 6784.0,
+                         B6 = 11.0 / 84.0;
+        constexpr double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0, E5 = -17253.0 / 339200.0,
+                         E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+        dt = std::max(std::min(dt, c.dt_max), c.dt_min);
+        for (;;) {
+            if (c.fsal_valid) {
+                combine<Real>(mt(0), {{0, A21}}, dt);
+            } else {
+                ensure_hd();
+                stage<State, Real>(m(), k(0), mt(0), {}, A21, dt);
+            }
+            c.fsal_valid = false;
+            demag_at_stage(mt(0));
+            stage<Real, Real>(mt(0), k(1), mt(1), {{0, A31}}, A32, dt);
+            demag_at_stage(mt(1));
+            stage<Real, Real>(mt(1), k(2), mt(0), {{0, A41}, {1, A42}}, A43, dt);
+            demag_at_stage(mt(0));
+            stage<Real, Real>(mt(0), k(3), mt(1), {{0, A51}, {1, A52}, {2, A53}}, A54, dt);
+            demag_at_stage(mt(1));
+            stage<Real, Real>(mt(1), k(4), mt(0), {{0, A61}, {1, A62}, {2, A63}, {3, A64}}, A65, dt);
+            demag_at_stage(mt(0));
+            stage<Real, State>(mt(0), k(5), mst(), {{0, B1}, {2, B3}, {3, B4}, {4, B5}}, B6, dt);
+            demag_at_state(mst());
+            const double err = error_stage(mst(), k(6), {{0, E1}, {2, E3}, {3, E4}, {4, E5}, {5, E6}}, E7, dt);
+            if (err <= c.max_err || dt <= c.dt_min) {
+                std::swap(c.m, c.mst);
+                std::swap(c.k[0], c.k[6]);  // FSAL: k7 becomes the next k1
+                c.fsal_valid = true;
+                c.hd_at_m = true;
+                return std::min(std::max(c.headroom * dt * std::pow(c.max_err / std::max(err, 1e-30), 0.2),
+                                         c.dt_min),
+                                c.dt_max);
+            }
+            c.hd_at_m = false;
+            dt = std::min(std::max(c.headroom * dt * std::pow(c.max_err / err, 0.2), c.dt_min), c.dt_max);
+        }
+    }
+
+    // lib.rs:1573-1648; the history push restarts it when dt changed by more
+    // than 10% (lib.rs:663-674), after the step.  Also launches the stats
+    // (the startup branch stores f(m_new) from them).
+    void abm3(double dt) {
+        const bool ready = c.abm_startup >= 3 && c.abm_count >= 3;
+        const int fn = c.abm_slot[0], fn1 = c.abm_slot[1], fn2 = c.abm_slot[2];
+        if (!ready) {
+            heun(dt);
+            stats(k(fn2));  // f(m_new) into the slot that is about to be recycled
+        } else {
+            combine<Real>(mt(0), {{fn, 23.0 / 12.0}, {fn1, -16.0 / 12.0}, {fn2, 5.0 / 12.0}}, dt);
+            demag_at_stage(mt(0));
+            stage<Real, State>(mt(0), k(fn2), m(), {{fn, 8.0 / 12.0}, {fn1, -1.0 / 12.0}}, 5.0 / 12.0, dt);
+            demag_at_state(m());
+            c.hd_at_m = true;
+            stats(nullptr);
+        }
+        if (c.abm_last_dt > 0.0 && std::fabs(dt - c.abm_last_dt) / c.abm_last_dt > 0.1) {
+            c.abm_count = 0;
+            c.abm_startup = 0;
+        }
+        c.abm_slot[0] = fn2;
+        c.abm_slot[1] = fn;
+        c.abm_slot[2] = fn1;
+        c.abm_count = std::min(c.abm_count + 1, 3);
+        c.abm_startup = std::min(c.abm_startup + 1, 3);
+        c.abm_last_dt = dt;
+    }
+
+    int step(double dt_in, fullmag_fdm_step_stats *st) {
+        const auto t0 = std::chrono::steady_clock::now();
+        double dt = dt_in, suggested = 0.0;
+        bool stats_pending = false;
+        switch (c.integrator) {
+            case FULLMAG_FDM_INTEGRATOR_RK4: rk4(dt); break;
+            case FULLMAG_FDM_INTEGRATOR_RK23: suggested = rk23(dt); break;
+            case FULLMAG_FDM_INTEGRATOR_DP45: suggested = dp45(dt); break;
+            case FULLMAG_FDM_INTEGRATOR_ABM3: abm3(dt); stats_pending = true; break;
+            case FULLMAG_FDM_INTEGRATOR_HEUN:
+            default: heun(dt); break;
+        }
+        if (!stats_pending) stats(nullptr);
+        c.obs_valid = false;
+        c.obs_frozen = false;
+        if (!hip_ok(hipStreamSynchronize(c.stream), "integrator step", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+        if (!hip_ok(hipGetLastError(), "integrator step launch", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+        timing_resolve(c);
+        c.step_count++;
+        c.time += dt;
+        fill(st, dt, suggested);
+        st->wall_time_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                               std::chrono::steady_clock::now() - t0)
+                               .count();
+        return FULLMAG_FDM_OK;
+    }
+
+    // Runs n fixed-dt steps; diagnostics only after the last one.
+    int step_many(double dt, uint64_t nsteps, fullmag_fdm_step_stats *st) {
+        const auto t0 = std::chrono::steady_clock::now();
+        double suggested = 0.0;
+        for (uint64_t s = 0; s < nsteps; ++s) {
+            double d = dt;
+            switch (c.integrator) {
+                case FULLMAG_FDM_INTEGRATOR_RK4: rk4(d); break;
+                case FULLMAG_FDM_INTEGRATOR_RK23: suggested = rk23(d); break;
+                case FULLMAG_FDM_INTEGRATOR_DP45: suggested = dp45(d); break;
+                case FULLMAG_FDM_INTEGRATOR_ABM3: abm3(d); break;
+                default: heun(d); break;
+            }
+            c.step_count++;
+            c.time += d;
+            if (s + 1 == nsteps) {
+                if (c.integrator != FULLMAG_FDM_INTEGRATOR_ABM3) stats(nullptr);
+                fill_pending_dt_ = d;
+            }
+        }
+        c.obs_valid = false;
+        c.obs_frozen = false;
+        if (!hip_ok(hipStreamSynchronize(c.stream), "integrator steps", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+        if (!hip_ok(hipGetLastError(), "integrator steps launch", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+        timing_resolve(c);
+        if (st) {
+            fill(st, fill_pending_dt_, suggested);
+            st->wall_time_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                   std::chrono::steady_clock::now() - t0)
+                                   .count();
+        }
+        return FULLMAG_FDM_OK;
+    }
+    double fill_pending_dt_ = 0.0;
+
+    void fill(fullmag_fdm_step_stats *st, double dt, double suggested) {
+        const double *r = c.h_result;
+        const double v = c.dx * c.dy * c.dz;
+        const double w = c.has_ms_cell ? 1.0 : c.ms;
+        const double e_ex = c.exchange ? -0.5 * kMu0 * w * v * (r[0] + r[1]) : 0.0;
+        const double e_d = c.demag ? -0.5 * kMu0 * w * v * (r[2] + r[3]) : 0.0;
+        const double e_ext = c.has_ext ? -kMu0 * w * v * (r[4] + r[5]) : 0.0;
+        st->step = c.step_count;
+        st->time_seconds = c.time;
+        st->dt_seconds = dt;
+        st->exchange_energy_joules = e_ex;
+        st->demag_energy_joules = e_d;
+        st->external_energy_joules = e_ext;
+        st->total_energy_joules = e_ex + e_d + e_ext;
+        st->max_effective_field_amplitude = std::sqrt(r[6]);
+        st->max_demag_field_amplitude = c.demag ? std::sqrt(r[7]) : 0.0;
+        st->max_rhs_amplitude = std::sqrt(r[8]);
+        st->suggested_next_dt = suggested;
+    }
+
+    bool refresh() {
+        demag_at_state(m());
+        c.hd_at_m = true;
+        c.obs_valid = false;
+        c.obs_frozen = false;
+        return hip_ok(hipStreamSynchronize(c.stream), "refresh_observables", c.last_error);
+    }
+
+    bool observe() {
+        if (!c.h_ex_obs && !ctx_alloc(c, &c.h_ex_obs, 3 * c.n * sizeof(Real), "H_ex observable")) return false;
+        if (!c.h_eff_obs && !ctx_alloc(c, &c.h_eff_obs, 3 * c.n * sizeof(Real), "H_eff observable")) return false;
+        FieldArgs f = field_args(c);
+        hipLaunchKernelGGL((observe_kernel<State, Real>), dim3(grid_plain()), dim3(kBlock), 0, c.stream, f, m(),
+                           c.demag ? hd() : nullptr, static_cast<Real *>(c.h_ex_obs),
+                           static_cast<Real *>(c.h_eff_obs));
+        c.obs_valid = true;
+        return hip_ok(hipStreamSynchronize(c.stream), "observables", c.last_error);
+    }
+};
+
+}  // namespace
+
+bool alloc_integrator_buffers(Context &c) {
+    const size_t sb = 3 * c.n * c.state_bytes(), rb = 3 * c.n * c.real_bytes();
+    if (!ctx_alloc(c, &c.m, sb, "m")) return false;
+    if (c.demag && !ctx_alloc(c, &c.hd, rb, "H_demag")) return false;
+    int nk = 1, nmt = 1;
+    bool need_mst = false;
+    switch (c.integrator) {
+        case FULLMAG_FDM_INTEGRATOR_RK4: nk = 3; nmt = 2; break;
+        case FULLMAG_FDM_INTEGRATOR_RK23: nk = 3; nmt = 2; need_mst = true; break;
+        case FULLMAG_FDM_INTEGRATOR_DP45: nk = 7; nmt = 2; need_mst = true; break;
+        case FULLMAG_FDM_INTEGRATOR_ABM3: nk = 7; nmt = 1; break;
+        default: break;
+    }
+    for (int i = 0; i < nk; ++i)
+        if (!ctx_alloc(c, &c.k[i], rb, "stage slope")) return false;
+    for (int i = 0; i < nmt; ++i)
+        if (!ctx_alloc(c, &c.mt[i], rb, "stage point")) return false;
+    if (need_mst && !ctx_alloc(c, &c.mst, sb, "candidate state")) return false;
+    c.max_partial_blocks = 2048;
+    if (!ctx_alloc(c, (void **)&c.d_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double), "partials"))
+        return false;
+    if (!ctx_alloc(c, (void **)&c.d_result, 16 * sizeof(double), "result")) return false;
+    if (!hip_ok(hipHostMalloc((void **)&c.h_result, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc",
+                c.last_error))
+        return false;
+    return true;
+}
+
+int integrator_step(Context &c, double dt, fullmag_fdm_step_stats *st) {
+    if (c.mode == MODE_DOUBLE) return Driver<double, double>(c).step(dt, st);
+    if (c.mode == MODE_MIXED) return Driver<double, float>(c).step(dt, st);
+    return Driver<float, float>(c).step(dt, st);
+}
+
+int integrator_step_many(Context &c, double dt, uint64_t n, fullmag_fdm_step_stats *st) {
+    if (c.mode == MODE_DOUBLE) return Driver<double, double>(c).step_many(dt, n, st);
+    if (c.mode == MODE_MIXED) return Driver<double, float>(c).step_many(dt, n, st);
+    return Driver<float, float>(c).step_many(dt, n, st);
+}
+
+bool refresh_observables(Context &c) {
+    if (c.mode == MODE_DOUBLE) return Driver<double, double>(c).refresh();
+    if (c.mode == MODE_MIXED) return Driver<double, float>(c).refresh();
+    return Driver<float, float>(c).refresh();
+}
+
+bool materialize_observables(Context &c) {
+    if (c.obs_valid || c.obs_frozen) return true;
+    if (c.mode == MODE_DOUBLE) return Driver<double, double>(c).observe();
+    if (c.mode == MODE_MIXED) return Driver<double, float>(c).observe();
+    return Driver<float, float>(c).observe();
+}
+
+bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
+                   uint32_t z0, uint32_t zs, void *dev_out, bool out_f64) {
+    const uint64_t count = (uint64_t)pnx * pny * pnz;
+    const dim3 grid((unsigned)cdiv(count, 256)), block(256);
+    const uint64_t n = c.n;
+    if (field_f64) {
+        const double *f = static_cast<const double *>(field);
+        if (out_f64)
+            hipLaunchKernelGGL((preview_kernel<double, double>), grid, block, 0, c.stream, f, f + n, f + 2 * n, c.nx,
+                               c.ny, c.nz, pnx, pny, pnz, z0, zs, static_cast<double *>(dev_out));
+        else
+            hipLaunchKernelGGL((preview_kernel<double, float>), grid, block, 0, c.stream, f, f + n, f + 2 * n, c.nx,
+                               c.ny, c.nz, pnx, pny, pnz, z0, zs, static_cast<float *>(dev_out));
+    } else {
+        const float *f = static_cast<const float *>(field);
+        if (out_f64)
+            hipLaunchKernelGGL((preview_kernel<float, double>), grid, block, 0, c.stream, f, f + n, f + 2 * n, c.nx,
+                               c.ny, c.nz, pnx, pny, pnz, z0, zs, static_cast<double *>(dev_out));
+        else
+            hipLaunchKernelGGL((preview_kernel<float, float>), grid, block, 0, c.stream, f, f + n, f + 2 * n, c.nx,
+                               c.ny, c.nz, pnx, pny, pnz, z0, zs, static_cast<float *>(dev_out));
+    }
+    return hip_ok(hipGetLastError(), "preview kernel", c.last_error);
+}
+
+}  // namespace fm

@@ -1,0 +1,404 @@
+"""Host-side mirror of fullmag's native FDM backend wrapper.
+
+`NativeFdmBackend` follows crates/fullmag-runner/src/native_fdm.rs:44-800
+(create / step / copy_* / upload / refresh / device_info / snapshots) and
+drives libfullmag_fdm.so.0 through the C ABI only; `FdmPlan` carries the
+FdmPlanIR fields that the runner turns into a fullmag_fdm_plan_desc
+(crates/fullmag-ir/src/lib.rs:662-695, native_fdm.rs:90-348).
+
+Errors surface as `RunError` with the backend's last_error text, as the
+runner's `last_error_or` does (native_fdm.rs:696-706).
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi as A
+
+MU0 = 4.0 * np.pi * 1e-7
+
+_PRECISIONS = {"single": A.PRECISION_SINGLE, "double": A.PRECISION_DOUBLE, "mixed": A.PRECISION_MIXED}
+_INTEGRATORS = {"heun": A.INTEGRATOR_HEUN, "rk45": A.INTEGRATOR_DP45, "dp45": A.INTEGRATOR_DP45,
+                "abm3": A.INTEGRATOR_ABM3, "rk4": A.INTEGRATOR_RK4, "rk23": A.INTEGRATOR_RK23}
+OBSERVABLES = {"m": A.OBSERVABLE_M, "H_ex": A.OBSERVABLE_H_EX, "H_demag": A.OBSERVABLE_H_DEMAG,
+               "H_ext": A.OBSERVABLE_H_EXT, "H_eff": A.OBSERVABLE_H_EFF}
+
+
+class RunError(RuntimeError):
+    pass
+
+
+def is_available() -> bool:
+    """fullmag_fdm_is_available() == 1 (native_fdm.rs:31-40)."""
+    return A.lib().fullmag_fdm_is_available() == 1
+
+
+def generate_random_unit_vectors(seed: int, count: int) -> np.ndarray:
+    """The planner's seeded m0 (crates/fullmag-plan/src/lib.rs:1927-1957), (count, 3)."""
+    out = np.empty((count, 3), dtype=np.float64)
+    A.lib().fullmag_fdm_ext_random_unit_vectors(ctypes.c_uint64(seed), count,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def compute_newell_kernel_spectra(nx: int, ny: int, nz: int, dx: float, dy: float, dz: float,
+                                  thin_film_2d: Optional[bool] = None) -> np.ndarray:
+    """Newell spectra in the plan-descriptor layout, shape (6, 2*Npad).
+
+    Same contract as fullmag_engine::compute_newell_kernel_spectra and
+    compute_newell_kernel_spectra_thin_film_2d (crates/fullmag-engine/src/
+    lib.rs:511-610), chosen as the runner does for nz == 1
+    (native_fdm.rs:150-171); computed on the device.
+    """
+    if thin_film_2d is None:
+        thin_film_2d = nz == 1
+    npad = 4 * nx * ny if thin_film_2d else 8 * nx * ny * nz
+    out = np.empty((6, 2 * npad), dtype=np.float64)
+    rc = A.lib().fullmag_fdm_ext_newell_spectra(nx, ny, nz, dx, dy, dz, 1 if thin_film_2d else 0,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), out.size)
+    if rc != A.FULLMAG_FDM_OK:
+        raise RunError(f"Newell spectra computation failed (rc={rc})")
+    return out
+
+
+@dataclass
+class FdmPlan:
+    """FdmPlanIR subset consumed by the FDM backend."""
+
+    cells: Sequence[int]
+    cell_size: Sequence[float]
+    initial_magnetization: np.ndarray
+    saturation_magnetisation: float
+    exchange_stiffness: float
+    damping: float
+    gyromagnetic_ratio: float = 2.211e5
+    precision: str = "double"  # "single" | "double" | "mixed"
+    integrator: str = "heun"
+    enable_exchange: bool = True
+    enable_demag: bool = True
+    external_field: Optional[Sequence[float]] = None  # A/m
+    active_mask: Optional[np.ndarray] = None
+    region_mask: Optional[np.ndarray] = None
+    inter_region_exchange: list = field(default_factory=list)  # (ri, rj, A_ij)
+    disable_precession: bool = False  # LLG-overdamped relaxation
+    adaptive_max_error: float = 0.0
+    adaptive_dt_min: float = 0.0
+    adaptive_dt_max: float = 0.0
+    adaptive_headroom: float = 0.0
+    # "spectra": pass Newell spectra through the plan descriptor as the runner
+    # does; "device": let the library build the tensor; an explicit (6, 2*Npad)
+    # array is passed through as given.
+    demag_kernel: object = "device"
+    ms_cell: Optional[np.ndarray] = None  # extension: per-cell Ms [A/m]
+
+    @property
+    def cell_count(self) -> int:
+        return int(self.cells[0]) * int(self.cells[1]) * int(self.cells[2])
+
+
+@dataclass
+class StepStats:
+    """Per-step diagnostics (crates/fullmag-runner/src/native_fdm.rs:388-407)."""
+
+    step: int
+    time: float
+    dt: float
+    e_ex: float
+    e_demag: float
+    e_ext: float
+    e_total: float
+    max_h_eff: float
+    max_h_demag: float
+    max_dm_dt: float
+    wall_time_ns: int
+    dt_suggested: Optional[float]
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class NativeFdmBackend:
+    """RAII owner of one backend handle (native_fdm.rs:44-47, 708-715)."""
+
+    def __init__(self, plan: FdmPlan):
+        L = A.lib()
+        self.plan = plan
+        self.cell_count = plan.cell_count
+        n = self.cell_count
+        nx, ny, nz = (int(v) for v in plan.cells)
+        dx, dy, dz = (float(v) for v in plan.cell_size)
+        keep = []  # host arrays that must outlive the create() call
+
+        d = A.PlanDesc()
+        d.grid = A.GridDesc(nx, ny, nz, dx, dy, dz)
+        d.material = A.MaterialDesc(plan.saturation_magnetisation, plan.exchange_stiffness, plan.damping,
+                                    plan.gyromagnetic_ratio)
+        prec = _PRECISIONS[plan.precision]
+        d.precision = A.PRECISION_SINGLE if prec == A.PRECISION_MIXED else prec
+        d.integrator = _INTEGRATORS[plan.integrator]
+        d.disable_precession = 1 if plan.disable_precession else 0
+        d.enable_exchange = 1 if plan.enable_exchange else 0
+        d.enable_demag = 1 if plan.enable_demag else 0
+        d.has_external_field = 1 if plan.external_field is not None else 0
+        ext = plan.external_field if plan.external_field is not None else (0.0, 0.0, 0.0)
+        d.external_field_am = (ctypes.c_double * 3)(*[float(v) for v in ext])
+
+        m0 = np.ascontiguousarray(plan.initial_magnetization, dtype=np.float64).reshape(-1)
+        if m0.size != 3 * n:
+            raise RunError(f"initial magnetization has {m0.size} values, expected {3 * n}")
+        keep.append(m0)
+        d.initial_magnetization_xyz = _dp(m0)
+        d.initial_magnetization_len = m0.size
+
+        newell_on_device = 0
+        if plan.enable_demag:
+            spectra = None
+            if isinstance(plan.demag_kernel, np.ndarray):
+                spectra = np.ascontiguousarray(plan.demag_kernel, dtype=np.float64)
+            elif plan.demag_kernel == "spectra":
+                spectra = compute_newell_kernel_spectra(nx, ny, nz, dx, dy, dz)
+            elif plan.demag_kernel == "device":
+                newell_on_device = 1
+            elif plan.demag_kernel not in (None, "projection"):
+                raise RunError(f"unknown demag_kernel option {plan.demag_kernel!r}")
+            if spectra is not None:
+                keep.append(spectra)
+                ptrs = [spectra[c].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in range(6)]
+                (d.demag_kernel_xx_spectrum, d.demag_kernel_yy_spectrum, d.demag_kernel_zz_spectrum,
+                 d.demag_kernel_xy_spectrum, d.demag_kernel_xz_spectrum, d.demag_kernel_yz_spectrum) = ptrs
+                d.demag_kernel_spectrum_len = spectra.shape[1]
+
+        if plan.active_mask is not None:
+            mask = np.ascontiguousarray(np.asarray(plan.active_mask).reshape(-1).astype(np.uint8))
+            keep.append(mask)
+            d.active_mask = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            d.active_mask_len = mask.size
+        if plan.region_mask is not None:
+            reg = np.ascontiguousarray(np.asarray(plan.region_mask).reshape(-1).astype(np.uint32))
+            keep.append(reg)
+            d.region_mask = reg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            d.region_mask_len = reg.size
+            # native_fdm.rs:173-196: diagonal = material A, caller overrides symmetric
+            R = A.MAX_EXCHANGE_REGIONS
+            lut = np.zeros(R * R, dtype=np.float64)
+            lut[np.arange(R) * R + np.arange(R)] = plan.exchange_stiffness
+            for ri, rj, aij in plan.inter_region_exchange:
+                if ri < R and rj < R:
+                    lut[ri * R + rj] = aij
+                    lut[rj * R + ri] = aij
+            keep.append(lut)
+            d.exchange_lut = _dp(lut)
+            d.exchange_lut_len = lut.size
+        d.adaptive_max_error = plan.adaptive_max_error
+        d.adaptive_dt_min = plan.adaptive_dt_min
+        d.adaptive_dt_max = plan.adaptive_dt_max
+        d.adaptive_headroom = plan.adaptive_headroom
+
+        opt = A.ExtOptions()
+        opt.struct_size = ctypes.sizeof(A.ExtOptions)
+        opt.precision_override = prec
+        opt.newell_on_device = newell_on_device
+        if plan.ms_cell is not None:
+            ms = np.ascontiguousarray(plan.ms_cell, dtype=np.float64).reshape(-1)
+            keep.append(ms)
+            opt.ms_cell = _dp(ms)
+            opt.ms_cell_len = ms.size
+
+        self._handle = L.fullmag_fdm_ext_backend_create(ctypes.byref(d), ctypes.byref(opt))
+        if not self._handle:
+            raise RunError("fullmag_fdm_backend_create returned NULL")
+        err = L.fullmag_fdm_backend_last_error(self._handle)
+        if err is not None:
+            msg = err.decode()
+            L.fullmag_fdm_backend_destroy(self._handle)
+            self._handle = None
+            raise RunError(msg)
+        self.precision = plan.precision
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self) -> None:
+        h = getattr(self, "_handle", None)
+        if h:
+            A.lib().fullmag_fdm_backend_destroy(h)
+            self._handle = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._handle
+
+    def last_error_or(self, fallback: str) -> RunError:
+        err = A.lib().fullmag_fdm_backend_last_error(self._handle)
+        return RunError(err.decode() if err else fallback)
+
+    # -- stepping ----------------------------------------------------------
+    def step(self, dt: float) -> StepStats:
+        s = A.StepStats()
+        rc = A.lib().fullmag_fdm_backend_step(self._handle, dt, ctypes.byref(s))
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("step failed")
+        return StepStats(s.step, s.time_seconds, s.dt_seconds, s.exchange_energy_joules, s.demag_energy_joules,
+                         s.external_energy_joules, s.total_energy_joules, s.max_effective_field_amplitude,
+                         s.max_demag_field_amplitude, s.max_rhs_amplitude, s.wall_time_ns,
+                         s.suggested_next_dt if s.suggested_next_dt > 0.0 else None)
+
+    def step_many(self, dt: float, n_steps: int) -> StepStats:
+        s = A.StepStats()
+        rc = A.lib().fullmag_fdm_ext_step_many(self._handle, dt, n_steps, ctypes.byref(s))
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("step_many failed")
+        return StepStats(s.step, s.time_seconds, s.dt_seconds, s.exchange_energy_joules, s.demag_energy_joules,
+                         s.external_energy_joules, s.total_energy_joules, s.max_effective_field_amplitude,
+                         s.max_demag_field_amplitude, s.max_rhs_amplitude, s.wall_time_ns,
+                         s.suggested_next_dt if s.suggested_next_dt > 0.0 else None)
+
+    # -- fields ------------------------------------------------------------
+    def copy_field(self, observable: str) -> np.ndarray:
+        out = np.empty((self.cell_count, 3), dtype=np.float64)
+        rc = A.lib().fullmag_fdm_backend_copy_field_f64(self._handle, OBSERVABLES[observable], _dp(out), out.size)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or(f"copy_field({observable}) failed")
+        return out
+
+    def copy_field_f32(self, observable: str) -> np.ndarray:
+        out = np.empty((self.cell_count, 3), dtype=np.float32)
+        rc = A.lib().fullmag_fdm_backend_copy_field_f32(
+            self._handle, OBSERVABLES[observable], out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or(f"copy_field_f32({observable}) failed")
+        return out
+
+    def copy_m(self):
+        return self.copy_field("m")
+
+    def copy_h_ex(self):
+        return self.copy_field("H_ex")
+
+    def copy_h_demag(self):
+        return self.copy_field("H_demag")
+
+    def copy_h_ext(self):
+        return self.copy_field("H_ext")
+
+    def copy_h_eff(self):
+        return self.copy_field("H_eff")
+
+    def copy_live_preview_field(self, observable: str, preview_cells: Sequence[int], z_origin: int = 0,
+                                z_stride: int = 1, f32: bool = False) -> np.ndarray:
+        pnx, pny, pnz = (int(v) for v in preview_cells)
+        dt = np.float32 if f32 else np.float64
+        out = np.empty((pnx * pny * pnz, 3), dtype=dt)
+        L = A.lib()
+        if f32:
+            rc = L.fullmag_fdm_backend_copy_field_preview_f32(
+                self._handle, OBSERVABLES[observable], pnx, pny, pnz, z_origin, z_stride,
+                out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size)
+        else:
+            rc = L.fullmag_fdm_backend_copy_field_preview_f64(
+                self._handle, OBSERVABLES[observable], pnx, pny, pnz, z_origin, z_stride, _dp(out), out.size)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("preview copy failed")
+        return out
+
+    def begin_field_snapshot(self, observable: str) -> "FieldSnapshot":
+        h = A.lib().fullmag_fdm_backend_begin_field_snapshot(self._handle, OBSERVABLES[observable])
+        if not h:
+            raise self.last_error_or("snapshot failed")
+        return FieldSnapshot(h)
+
+    def upload_magnetization(self, m: np.ndarray) -> None:
+        a = np.ascontiguousarray(m, dtype=np.float64).reshape(-1)
+        rc = A.lib().fullmag_fdm_backend_upload_magnetization_f64(self._handle, _dp(a), a.size)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("upload failed")
+
+    def upload_magnetization_f32(self, m: np.ndarray) -> None:
+        a = np.ascontiguousarray(m, dtype=np.float32).reshape(-1)
+        rc = A.lib().fullmag_fdm_backend_upload_magnetization_f32(
+            self._handle, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("upload failed")
+
+    def refresh_observables(self) -> None:
+        rc = A.lib().fullmag_fdm_backend_refresh_observables(self._handle)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("refresh failed")
+
+    def set_external_field(self, h_am: Sequence[float]) -> None:
+        h = (ctypes.c_double * 3)(*[float(v) for v in h_am])
+        rc = A.lib().fullmag_fdm_ext_set_external_field(self._handle, h)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("set_external_field failed")
+
+    def device_info(self) -> dict:
+        info = A.DeviceInfo()
+        rc = A.lib().fullmag_fdm_backend_get_device_info(self._handle, ctypes.byref(info))
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("device info failed")
+        return {"name": info.name.decode(errors="replace"), "compute_capability": (
+            info.compute_capability_major, info.compute_capability_minor),
+            "driver_version": info.driver_version, "runtime_version": info.runtime_version}
+
+    def precision_mode(self) -> int:
+        return A.lib().fullmag_fdm_ext_precision(self._handle)
+
+    def device_bytes(self) -> int:
+        return int(A.lib().fullmag_fdm_ext_device_bytes(self._handle))
+
+    # -- kernel timing (extension) -------------------------------------------
+    def set_timing(self, enable: bool) -> None:
+        A.lib().fullmag_fdm_ext_set_timing(self._handle, 1 if enable else 0)
+
+    def timings(self) -> dict:
+        L = A.lib()
+        out = {}
+        i = 0
+        name = ctypes.c_char_p()
+        ms = ctypes.c_double()
+        cnt = ctypes.c_uint64()
+        while L.fullmag_fdm_ext_get_timing(self._handle, i, ctypes.byref(name), ctypes.byref(ms),
+                                           ctypes.byref(cnt)) == A.FULLMAG_FDM_OK:
+            out[name.value.decode()] = (ms.value, cnt.value)
+            i += 1
+        return out
+
+
+class FieldSnapshot:
+    """Async SoA snapshot (native_fdm.rs:718-800)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def wait(self) -> np.ndarray:
+        data = ctypes.c_void_p()
+        nbytes = ctypes.c_uint64()
+        desc = A.SnapshotDesc()
+        rc = A.lib().fullmag_fdm_field_snapshot_wait(self._h, ctypes.byref(data), ctypes.byref(nbytes),
+                                                    ctypes.byref(desc))
+        if rc != A.FULLMAG_FDM_OK:
+            raise RunError("snapshot wait failed")
+        dt = np.float32 if desc.scalar_type == A.SNAPSHOT_F32 else np.float64
+        buf = (ctypes.c_char * nbytes.value).from_address(data.value)
+        return np.frombuffer(buf, dtype=dt).reshape(3, desc.cell_count).copy()
+
+    def close(self):
+        if self._h:
+            A.lib().fullmag_fdm_field_snapshot_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

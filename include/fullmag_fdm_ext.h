@@ -1,0 +1,83 @@
+/*
+ * fullmag_fdm_ext.h -- additive extensions of the FDM C ABI.
+ *
+ * Nothing here changes the 16 entry points or the struct layouts of
+ * fullmag_fdm.h; callers that only know fullmag_fdm.h are unaffected.  None
+ * of these entry points has a counterpart in the reference ABI
+ * (native/include/fullmag_fdm.h); each comment names the caller it serves.
+ */
+
+#ifndef FULLMAG_FDM_EXT_H
+#define FULLMAG_FDM_EXT_H
+
+#include "fullmag_fdm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mixed precision: FP64 magnetisation state, renormalisation, Newell tensor
+ * and adaptive error norm; FP32 exchange, FFT demag and torque; compensated
+ * FP64 energy sums.  Selectable through fullmag_fdm_ext_options, or for the
+ * unchanged Rust runner by precision SINGLE plus FULLMAG_FDM_MIXED=1. */
+#define FULLMAG_FDM_PRECISION_MIXED 3
+
+typedef struct {
+    uint32_t struct_size;          /* sizeof(fullmag_fdm_ext_options) */
+    int32_t  precision_override;   /* 0: use plan->precision; else 1, 2 or 3 */
+    int32_t  newell_on_device;     /* 1: when the plan carries no spectra and
+                                      demag is enabled, build the Newell tensor
+                                      in the library instead of the k-space
+                                      projection fallback */
+    int32_t  reserved;
+    const double *ms_cell;         /* optional per-cell Ms [A/m], cell count */
+    uint64_t ms_cell_len;
+} fullmag_fdm_ext_options;
+
+/* create() with extension options.  For Python/C callers that need mixed
+ * precision, per-cell Ms or the in-library Newell build. */
+fullmag_fdm_backend *fullmag_fdm_ext_backend_create(const fullmag_fdm_plan_desc *plan,
+                                                    const fullmag_fdm_ext_options *options);
+
+/* Precision mode in use (1 single, 2 double, 3 mixed), or ERR_INVALID. */
+int fullmag_fdm_ext_precision(fullmag_fdm_backend *handle);
+
+/* Newell tensor spectra in the layout fullmag_fdm_plan_desc expects: six
+ * interleaved complex f64 arrays back to back (xx, yy, zz, xy, xz, yz), each
+ * 2 * (2nx)(2ny)(2nz) doubles, or 2 * (2nx)(2ny) when thin_film_2d (nz must
+ * be 1).  Computed on the device.  For callers that build a plan_desc
+ * themselves, as crates/fullmag-runner/src/native_fdm.rs:150-171 does. */
+int fullmag_fdm_ext_newell_spectra(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double dz,
+                                   int thin_film_2d, double *out, uint64_t out_len);
+
+/* Replace the uniform external field [A/m] (field sweeps, finite-difference
+ * checks) without re-creating the backend. */
+int fullmag_fdm_ext_set_external_field(fullmag_fdm_backend *handle, const double h_am[3]);
+
+/* n_steps fixed-dt steps with one host synchronisation at the end; stats of
+ * the final state.  For throughput callers that do not need per-step stats. */
+int fullmag_fdm_ext_step_many(fullmag_fdm_backend *handle, double dt_seconds, uint64_t n_steps,
+                              fullmag_fdm_step_stats *out_stats);
+
+/* Per-kernel device timing with HIP events on the backend's stream.  When
+ * enabled, every kernel launch is bracketed by events; get_timing returns the
+ * accumulated milliseconds and launch count of entry `index` (ERR_INVALID
+ * past the last entry).  For benchmarks and roofline reports. */
+int fullmag_fdm_ext_set_timing(fullmag_fdm_backend *handle, int enable);
+int fullmag_fdm_ext_get_timing(fullmag_fdm_backend *handle, int index, const char **name,
+                               double *total_ms, uint64_t *launches);
+
+/* Seeded random unit vectors, AoS, exactly as the planner's m0 generator
+ * (crates/fullmag-plan/src/lib.rs:1927-1957: xorshift64 <<13 >>7 <<17 per
+ * component, rejection below norm 1e-10).  Host-side; for callers building
+ * the initial magnetisation of a plan. */
+void fullmag_fdm_ext_random_unit_vectors(uint64_t seed, uint64_t count, double *out_xyz);
+
+/* Bytes of device memory the backend holds. */
+uint64_t fullmag_fdm_ext_device_bytes(fullmag_fdm_backend *handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FULLMAG_FDM_EXT_H */

@@ -90,6 +90,7 @@ struct FftPlan {
     int n = 0;
     std::vector<int> radices;
     std::vector<cd> tw;          // tw[t] = exp(-2 pi i t / n), long-double accurate
+    std::vector<std::vector<cd>> pass_tw;  // per pass: [k][r-1] = tw[r k n/(ns R)]
     bool bluestein = false;
     int m = 0;                   // Bluestein convolution length
     std::vector<cd> chirp;       // exp(-pi i j^2 / n)
@@ -127,6 +128,17 @@ std::shared_ptr<FftPlan> make_plan(int n) {
     bool large = false;
     p->radices = factorize(n, large);
     p->tw = roots(n);
+    {
+        int ns = 1;
+        for (int R : p->radices) {
+            std::vector<cd> t((size_t)ns * (R - 1));
+            const int stride = n / (ns * R);
+            for (int k = 0; k < ns; ++k)
+                for (int r = 1; r < R; ++r) t[(size_t)k * (R - 1) + (r - 1)] = p->tw[(size_t)r * k * stride];
+            p->pass_tw.push_back(std::move(t));
+            ns *= R;
+        }
+    }
     if (large) {
         p->bluestein = true;
         int m = 1;
@@ -192,43 +204,66 @@ void fft_exec(const FftPlan &p, cd *data, bool inverse, cd *scratch) {
     }
     cd *x = data, *y = scratch;
     int ns = 1;
-    for (int R : p.radices) {
+    for (size_t pi = 0; pi < p.radices.size(); ++pi) {
+        const int R = p.radices[pi];
         const int nr = n / R;
-        const int stride_tw = n / (ns * R);
-        for (int j = 0; j < nr; ++j) {
-            const int k = j % ns;
-            cd v[64];
-            cd vv[64];
-            for (int r = 0; r < R; ++r) {
-                cd a = x[j + r * nr];
-                int t = (int)(((long long)r * k * stride_tw) % n);
-                cd w = inverse ? std::conj(p.tw[t]) : p.tw[t];
-                v[r] = (r == 0 || k == 0) ? a : a * w;
-            }
-            // R-point DFT with exact table roots
-            if (R == 2) {
-                vv[0] = v[0] + v[1];
-                vv[1] = v[0] - v[1];
-            } else if (R == 4) {
-                cd a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = v[1] - v[3];
-                // -i (forward) or +i (inverse) times a3
-                cd ja3 = inverse ? cd(-a3.imag(), a3.real()) : cd(a3.imag(), -a3.real());
-                vv[0] = a0 + a2;
-                vv[1] = a1 + ja3;
-                vv[2] = a0 - a2;
-                vv[3] = a1 - ja3;
-            } else
-            for (int q = 0; q < R; ++q) {
-                cd acc = v[0];
-                for (int r = 1; r < R; ++r) {
-                    int t = (int)(((long long)r * q % R) * (n / R));
-                    cd w = inverse ? std::conj(p.tw[t]) : p.tw[t];
-                    acc += v[r] * w;
+        const cd *ptw = p.pass_tw[pi].data();
+        const int nblk = nr / ns;
+        if (R == 2) {
+            for (int b = 0; b < nblk; ++b)
+                for (int k = 0; k < ns; ++k) {
+                    const int j = b * ns + k;
+                    cd a = x[j], c = x[j + nr];
+                    if (k != 0) c *= inverse ? std::conj(ptw[k]) : ptw[k];
+                    const int base = b * ns * 2 + k;
+                    y[base] = a + c;
+                    y[base + ns] = a - c;
                 }
-                vv[q] = acc;
+        } else if (R == 4) {
+            for (int b = 0; b < nblk; ++b)
+                for (int k = 0; k < ns; ++k) {
+                    const int j = b * ns + k;
+                    cd v0 = x[j], v1 = x[j + nr], v2 = x[j + 2 * nr], v3 = x[j + 3 * nr];
+                    if (k != 0) {
+                        const cd *w = ptw + (size_t)k * 3;
+                        if (inverse) {
+                            v1 *= std::conj(w[0]); v2 *= std::conj(w[1]); v3 *= std::conj(w[2]);
+                        } else {
+                            v1 *= w[0]; v2 *= w[1]; v3 *= w[2];
+                        }
+                    }
+                    cd a0 = v0 + v2, a1 = v0 - v2, a2 = v1 + v3, a3 = v1 - v3;
+                    cd ja3 = inverse ? cd(-a3.imag(), a3.real()) : cd(a3.imag(), -a3.real());
+                    const int base = b * ns * 4 + k;
+                    y[base] = a0 + a2;
+                    y[base + ns] = a1 + ja3;
+                    y[base + 2 * ns] = a0 - a2;
+                    y[base + 3 * ns] = a1 - ja3;
+                }
+        } else {
+            for (int j = 0; j < nr; ++j) {
+                const int k = j % ns;
+                cd v[64], vv[64];
+                for (int r = 0; r < R; ++r) {
+                    cd a = x[j + r * nr];
+                    if (r != 0 && k != 0) {
+                        cd w = ptw[(size_t)k * (R - 1) + (r - 1)];
+                        a *= inverse ? std::conj(w) : w;
+                    }
+                    v[r] = a;
+                }
+                for (int q = 0; q < R; ++q) {
+                    cd acc = v[0];
+                    for (int r = 1; r < R; ++r) {
+                        int t = (int)(((long long)r * q % R) * (n / R));
+                        cd w = inverse ? std::conj(p.tw[t]) : p.tw[t];
+                        acc += v[r] * w;
+                    }
+                    vv[q] = acc;
+                }
+                const int base = (j / ns) * ns * R + k;
+                for (int q = 0; q < R; ++q) y[base + q * ns] = vv[q];
             }
-            const int base = (j / ns) * ns * R + k;
-            for (int q = 0; q < R; ++q) y[base + q * ns] = vv[q];
         }
         std::swap(x, y);
         ns *= R;
