@@ -7,10 +7,10 @@
 //   hd       Real    demag field at the most recently evaluated point
 //   mt[2]    Real    stage points of multi-stage integrators
 //   k[i]     Real    stage slopes dm/dt
-//   spec     cplx<Real>  half spectrum of the padded convolution, per
-//                    component [nz][rows][nx+1] (rows = 2ny in 3-D, ny in 2-D)
+//   spec     cplx<Real>  half spectrum of the padded convolution,
+//                    [c][z][kx][ky], kx <= nx, ky < 2ny (3-D) or ny (2-D)
 //   kern     Real    Newell tensor spectrum on the non-negative octant
-//                    [comp][nz+1 or 1][ny+1][nx+1] (real by parity)
+//                    [comp][kz <= nz (3-D) or 0][kx <= nx][ky <= ny]
 #pragma once
 
 #include "common.hpp"
@@ -25,7 +25,17 @@ namespace fm {
 
 enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 
+// Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
+// per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
+// of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds).
+struct DemagTuning {
+    int x_radix = 16, x_lanes = 0, x_rows = 0;
+    int y_radix = 16, y_lanes = 0, y_lpw = 0;
+    int z_lds = 0;  // 1: LDS z kernel even when the register path applies
+};
+
 struct DemagPlan {
+    DemagTuning tune;
     bool enabled = false;
     bool three_d = false;     // false: nz == 1 thin film, 2-D padded transform
     bool projection = false;  // no tensor: k-space projection fallback
@@ -38,8 +48,10 @@ struct DemagPlan {
     double inv_npad = 0.0;
     LinePlan lx, ly, lz;
     // launch geometry
-    int x_lines_per_wg = 1;
-    int y_tile = 1, yconv_tile = 1, z_tile = 1;
+    int xr = 2, x_ls = 0;        // x passes: real rows per workgroup, LDS line stride
+    int y_lines_per_wg = 1;      // 3-D y passes
+    int yconv_kpw = 1;           // thin-film y convolution: kx columns per workgroup
+    int z_tile = 1;              // long z lines: columns per workgroup
     // device buffers (element type depends on Real)
     void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
     void *spec = nullptr;

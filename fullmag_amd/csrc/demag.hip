@@ -3,33 +3,44 @@
 // Replaces the cuFFT pipeline of native/backends/fdm/src/demag_fp64.cu:264-381
 // (pack, 3 forward Z2Z transforms over the fully padded grid, 6-component
 // complex tensor multiply, 3 inverse transforms, unpack).  The mathematics is
-// the same as crates/fullmag-engine/src/lib.rs:2306-2400:
-//     H = IFFT( -K . FFT(Ms m) ) / Npad   on a (2nx, 2ny, 2nz) grid
-// but the transform is organised around what is actually non-zero:
-//   x  rows   real-to-complex: two real rows share one complex transform and
-//             only nx of the 2nx inputs are loaded; nx+1 outputs are kept.
-//   y  cols   forward transform reads only ny of the 2ny rows.
-//   z  cols   forward transform of nz inputs, 3x3 tensor multiply, inverse
-//             transform, only nz outputs written (3-D); for thin films the
-//             tensor multiply sits between the forward and inverse y
-//             transforms of one kernel and no z transform exists.
-//   y  cols   inverse transform writes only ny rows (3-D).
-//   x  rows   complex-to-real: two rows from one complex transform, only nx
-//             outputs, scaled by 1/Npad and masked.
+// that of crates/fullmag-engine/src/lib.rs:2306-2400,
+//     H = IFFT( -K . FFT(Ms m) ) / Npad   on a (2nx, 2ny, 2nz) grid,
+// organised around what is non-zero and around coalesced HBM access:
+//
+//   x fwd  R2C of 16 real rows per workgroup (two rows per complex transform,
+//          only nx of 2nx inputs loaded); the nx+1 outputs of each row are
+//          written transposed, so ky runs are contiguous in the spectrum.
+//   y fwd  contiguous lines, ny of 2ny inputs loaded (3-D only).
+//   z conv per (kx, ky): 3 components x (nz inputs -> 2nz), 3x3 tensor,
+//          inverse, nz outputs; in registers for 2nz <= 16 (3-D only).
+//   y inv  contiguous lines, 2ny inputs, ny outputs written (3-D only).
+//   x inv  C2R of 16 rows, reading transposed runs, nx outputs per row,
+//          scaled by 1/Npad and masked.
+// Thin films (nz = 1, 2-D spectra): x fwd, y conv (forward y, tensor at
+// kz = 0, inverse y in one kernel), x inv.
+//
+// Spectrum layout: spec[c][z][kx][ky] with z < nz (3-D) or a single plane
+// (2-D), kx <= nx, ky < pitch (2ny in 3-D, ny in 2-D).
 // The Newell spectra are real (the padded kernel is even or odd along every
-// axis), so the tensor is stored as six real values on the non-negative
-// octant (kx <= nx, ky <= ny, kz <= nz) and mirrored with parity signs.
+// axis); the tensor is stored on the non-negative octant as
+// kern[comp][kz][kx][ky] (kz <= nz, ky <= ny) and mirrored with parity signs.
 #include "context.hpp"
 
 #include <algorithm>
 #include <cmath>
-#include <thread>
+#include <cstdlib>
 
 namespace fm {
 namespace {
 
 constexpr int kPad = 1;         // LDS line padding (complex slots)
 constexpr int kMaxBlock = 512;  // lanes per workgroup for FFT kernels
+// largest padded z length done in registers (a lane pair per column holds
+// 3 x pz complex values)
+template <typename Real>
+constexpr int z_reg_max() {
+    return 16;
+}
 
 // ---------------------------------------------------------------------------
 // Tensor access
@@ -56,9 +67,9 @@ __device__ inline void tensor_at(const KernelView<Real> &K, int kx, int ky, int 
             oz = zlo ? kz : K.pz - kz;
             sz = zlo ? Real(1) : Real(-1);
         }
-        const int64_t plane = (int64_t)(K.ny + 1) * K.hx;
-        const int64_t off = (int64_t)oz * plane + (int64_t)oy * K.hx + kx;
-        const int64_t cs = (int64_t)K.kz_n * plane;
+        const int64_t ly = K.ny + 1;
+        const int64_t off = ((int64_t)oz * K.hx + kx) * ly + oy;
+        const int64_t cs = (int64_t)K.kz_n * K.hx * ly;
         t[0] = K.k[off];
         t[1] = K.k[cs + off];
         t[2] = K.k[2 * cs + off];
@@ -86,64 +97,67 @@ __device__ inline void tensor_at(const KernelView<Real> &K, int kx, int ky, int 
     }
 }
 
+template <typename Real>
+__device__ inline void apply_tensor(const Real t[6], cplx<Real> &mx, cplx<Real> &my, cplx<Real> &mz) {
+    const cplx<Real> X = mx, Y = my, Z = mz;
+    mx = {-(t[0] * X.x + t[3] * Y.x + t[4] * Z.x), -(t[0] * X.y + t[3] * Y.y + t[4] * Z.y)};
+    my = {-(t[3] * X.x + t[1] * Y.x + t[5] * Z.x), -(t[3] * X.y + t[1] * Y.y + t[5] * Z.y)};
+    mz = {-(t[4] * X.x + t[5] * Y.x + t[2] * Z.x), -(t[4] * X.y + t[5] * Y.y + t[2] * Z.y)};
+}
+
 // ---------------------------------------------------------------------------
-// Row (x) passes
+// Spectrum addressing: spec[c][z][kx][ky], `planes` z-planes, ky pitch.
+// ---------------------------------------------------------------------------
+struct SpecGeom {
+    int hx;       // half-spectrum width nx + 1
+    int planes;   // z planes stored
+    int pitch;    // ky line length stored
+    __device__ __host__ int64_t at(int c, int z, int kx, int ky) const {
+        return (((int64_t)c * planes + z) * hx + kx) * pitch + ky;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// x passes
 // ---------------------------------------------------------------------------
 template <typename Real>
 struct XArgs {
     const cplx<Real> *tw;
-    LinePlan lp;
-    int64_t lines;  // real lines
-    int lpw;        // complex lines (= pairs of real lines) per workgroup
-    int n_in;       // loaded inputs per real line (forward)
-    int n_out;      // stored outputs per real line (inverse)
-    int hx;         // half-spectrum length kept = px/2 + 1
-    Real scale;     // inverse: 1/Npad
+    LinePlan lp;      // length px
+    int xr, xr_log2;  // real rows per workgroup (power of two) and its log2
+    int ls;           // LDS line stride (complex slots)
+    int rows;         // valid real rows per (c, z) plane
+    int n_in;         // forward: inputs loaded per row
+    int n_out;        // inverse: outputs stored per row
+    Real scale;       // inverse: 1/Npad
+    SpecGeom g;
 };
 
-// Rows of the half spectrum: line l -> (component, plane, row).
-template <typename Real>
-struct SpecRows {
-    cplx<Real> *base;
-    int64_t lines_per_comp, comp_stride;
-    int lines_per_plane, rows, hx;
-    __device__ cplx<Real> *row(int64_t l) const {
-        const int64_t c = l / lines_per_comp, r = l - c * lines_per_comp;
-        const int64_t z = r / lines_per_plane, y = r - z * lines_per_plane;
-        return base + c * comp_stride + (z * rows + y) * hx;
-    }
-};
-
-// Ms m on line l = (c, z, y); zero on inactive cells.
+// Ms m at (c, z, y, x) of the state; zero on inactive cells.
 template <typename Real, typename S>
 struct MagSource {
     const S *m;
     const uint8_t *mask;
     const double *ms_cell;
     double ms;
-    int nx;
-    int64_t per_comp, n;
-    __device__ Real operator()(int64_t l, int x) const {
-        const int64_t c = l / per_comp;
-        const int64_t i = (l - c * per_comp) * nx + x;
+    int nx, ny, nz;
+    int64_t n;
+    __device__ Real operator()(int c, int z, int y, int x) const {
+        const int64_t i = ((int64_t)z * ny + y) * nx + x;
         if (mask && !mask[i]) return Real(0);
         const double w = ms_cell ? ms_cell[i] : ms;
-        return Real(double(m[c * n + i]) * w);
+        return Real(double(m[(int64_t)c * n + i]) * w);
     }
 };
 
-// Padded real-space Newell kernel on line l = (c, z, y) of the (px, py, pz)
-// grid, mirrored from the octant with the placement signs of
+// Padded real-space Newell kernel at (c, z, y, x) of the (px, py, pz) grid,
+// mirrored from the octant with the placement signs of
 // crates/fullmag-fdm-demag/src/newell.rs:397-426 (index n of each padded
 // axis is never written there and stays zero).
 struct NewellSource {
     const double *oct;  // [6][nz][ny][nx]
     int nx, ny, nz, px, py, pz;
-    __device__ double operator()(int64_t l, int x) const {
-        const int64_t per_comp = (int64_t)py * pz;
-        const int c = (int)(l / per_comp);
-        const int64_t r = l - (int64_t)c * per_comp;
-        const int zz = (int)(r / py), yy = (int)(r - (int64_t)zz * py);
+    __device__ double operator()(int c, int zz, int yy, int x) const {
         if (x == nx || yy == ny || (pz > 1 && zz == nz)) return 0.0;
         const int xi = x < nx ? x : px - x;
         const int yi = yy < ny ? yy : py - yy;
@@ -162,244 +176,376 @@ template <typename Real>
 struct FieldSink {
     Real *out;
     const uint8_t *mask;
-    int nx;
-    int64_t per_comp, n;
-    __device__ void operator()(int64_t l, int x, Real v) const {
-        const int64_t c = l / per_comp;
-        const int64_t i = (l - c * per_comp) * nx + x;
-        out[c * n + i] = (mask && !mask[i]) ? Real(0) : v;
+    int nx, ny;
+    int64_t n;
+    __device__ void operator()(int c, int z, int y, int x, Real v) const {
+        const int64_t i = ((int64_t)z * ny + y) * nx + x;
+        out[(int64_t)c * n + i] = (mask && !mask[i]) ? Real(0) : v;
     }
 };
 
-// Forward real-to-complex along x.  Real lines 2p and 2p+1 are packed as
-// z = a + i b; with Z = FFT(z):  A_k = (Z_k + conj Z_{N-k}) / 2,
-// B_k = (Z_k - conj Z_{N-k}) / (2i), for k = 0..N/2.
+// Forward real-to-complex along x for xr rows (y0 .. y0+xr-1) of plane
+// (c = blockIdx.z, z = blockIdx.y).  Rows 2j, 2j+1 are packed as a + i b;
+// with Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k}) / 2,
+// B_k = (Z_k - conj Z_{N-k}) / (2i), k = 0..N/2, stored at spec[c][z][k][y].
 template <typename Real, class Src>
-__global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src, SpecRows<Real> dst) {
+__global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
+    const int LINES = a.xr >> 1;
+    const int n = a.lp.n, ls = a.ls, T = a.lp.threads;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int64_t pairs = (a.lines + 1) / 2;
-    const int64_t p0 = (int64_t)blockIdx.x * a.lpw;
-    for (int idx = tid; idx < a.lpw * n; idx += nthr) {
-        const int l = idx / n, x = idx - l * n;
-        const int64_t p = p0 + l;
+    const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
+    for (int idx = tid; idx < LINES * n; idx += nthr) {
+        const int j = idx / n, x = idx - j * n;
+        const int ya = y0 + 2 * j;
         cplx<Real> v = {Real(0), Real(0)};
-        if (x < a.n_in && p < pairs) {
-            v.x = src(2 * p, x);
-            if (2 * p + 1 < a.lines) v.y = src(2 * p + 1, x);
+        if (x < a.n_in) {
+            if (ya < a.rows) v.x = src(c, z, ya, x);
+            if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
         }
-        lds[l * ls + x] = v;
+        lds[j * ls + lslot(a.lp, x)] = v;
     }
     __syncthreads();
     line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
-    for (int idx = tid; idx < a.lpw * a.hx; idx += nthr) {
-        const int l = idx / a.hx, k = idx - l * a.hx;
-        const int64_t p = p0 + l;
-        if (p >= pairs) continue;
-        const cplx<Real> zk = lds[l * ls + k];
-        const cplx<Real> zn = lds[l * ls + (k == 0 ? 0 : n - k)];
+    const int hx = a.g.hx;
+    for (int idx = tid; idx < hx * a.xr; idx += nthr) {
+        const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
+        const int y = y0 + yl;
+        if (y >= a.rows) continue;
+        const int j = yl >> 1;
+        const cplx<Real> zk = lds[j * ls + lslot(a.lp, k)];
+        const cplx<Real> zn = lds[j * ls + lslot(a.lp, k == 0 ? 0 : n - k)];
         const Real h = Real(0.5);
-        dst.row(2 * p)[k] = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
-        if (2 * p + 1 < a.lines) dst.row(2 * p + 1)[k] = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
+        cplx<Real> v;
+        if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
+        else v = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
+        spec[a.g.at(c, z, k, y)] = v;
     }
 }
 
-// Inverse complex-to-real along x: Z_k = A_k + i B_k with the Hermitian
-// continuation A_{N-k} = conj A_k; z = IFFT(Z); a = Re z, b = Im z.
+// Inverse complex-to-real along x: Z_k = A_k + i B_k for k <= N/2 and the
+// Hermitian continuation A_{N-k} = conj A_k; z = IFFT(Z); rows a = Re z,
+// b = Im z.  Reads spec[c][z][k][y0..y0+xr-1].
 template <typename Real, class Sink>
-__global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, SpecRows<Real> srcr, Sink sink) {
+__global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
+    const int LINES = a.xr >> 1;
+    const int n = a.lp.n, ls = a.ls, T = a.lp.threads;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int64_t pairs = (a.lines + 1) / 2;
-    const int64_t p0 = (int64_t)blockIdx.x * a.lpw;
-    for (int idx = tid; idx < a.lpw * n; idx += nthr) {
-        const int l = idx / n, k = idx - l * n;
-        const int64_t p = p0 + l;
-        cplx<Real> z = {Real(0), Real(0)};
-        if (p < pairs) {
-            const bool lo = k < a.hx;
-            const int kk = lo ? k : n - k;
-            cplx<Real> A = srcr.row(2 * p)[kk];
-            cplx<Real> B = {Real(0), Real(0)};
-            if (2 * p + 1 < a.lines) B = srcr.row(2 * p + 1)[kk];
-            if (!lo) {
-                A.y = -A.y;
-                B.y = -B.y;
-            }
-            z = {A.x - B.y, A.y + B.x};
-        }
-        lds[l * ls + k] = z;
+    const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
+    const int hx = a.g.hx;
+    for (int idx = tid; idx < hx * LINES; idx += nthr) {
+        const int j = idx % LINES, k = idx / LINES;
+        const int ya = y0 + 2 * j;
+        cplx<Real> A = {Real(0), Real(0)}, B = {Real(0), Real(0)};
+        if (ya < a.rows) A = spec[a.g.at(c, z, k, ya)];
+        if (ya + 1 < a.rows) B = spec[a.g.at(c, z, k, ya + 1)];
+        lds[j * ls + lslot(a.lp, k)] = {A.x - B.y, A.y + B.x};
+        if (k > 0 && k < n - k) lds[j * ls + lslot(a.lp, n - k)] = {A.x + B.y, B.x - A.y};
     }
     __syncthreads();
     line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
-    for (int idx = tid; idx < a.lpw * a.n_out; idx += nthr) {
-        const int l = idx / a.n_out, x = idx - l * a.n_out;
-        const int64_t p = p0 + l;
-        if (p >= pairs) continue;
-        const cplx<Real> z = lds[l * ls + x];
-        sink(2 * p, x, z.x * a.scale);
-        if (2 * p + 1 < a.lines) sink(2 * p + 1, x, z.y * a.scale);
+    for (int idx = tid; idx < a.xr * a.n_out; idx += nthr) {
+        const int yl = idx / a.n_out, x = idx - yl * a.n_out;
+        const int y = y0 + yl;
+        if (y >= a.rows) continue;
+        const cplx<Real> v = lds[(yl >> 1) * ls + lslot(a.lp, x)];
+        sink(c, z, y, x, ((yl & 1) ? v.y : v.x) * a.scale);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Column (y or z) passes, in place on the half spectrum
+// Contiguous line passes (y; and the Newell setup's z when long)
 // ---------------------------------------------------------------------------
 template <typename Real>
-struct ColArgs {
+struct LineArgs {
     cplx<Real> *data;
     const cplx<Real> *tw;
     LinePlan lp;
+    int64_t lines;      // number of lines
+    int64_t line_pitch; // distance between consecutive lines
+    int lpw;            // lines per workgroup
     int n_in, n_out;
-    int64_t estride;  // element stride along the transformed axis
-    int ncols;        // columns along the contiguous axis (= half-spectrum width)
-    int tile;         // columns per workgroup
-    int q_div;        // blockIdx.y = q -> offset (q / q_div) s_hi + (q % q_div) s_lo
-    int64_t s_hi, s_lo;
     int inverse;
-    int64_t comp_stride;  // CONV: distance between components
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(kMaxBlock) line_kernel(LineArgs<Real> a) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
+    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int64_t l0 = (int64_t)blockIdx.x * a.lpw;
+    for (int idx = tid; idx < a.lpw * n; idx += nthr) {
+        const int j = idx / n, e = idx - j * n;
+        const int64_t l = l0 + j;
+        cplx<Real> v = {Real(0), Real(0)};
+        if (e < a.n_in && l < a.lines) v = a.data[l * a.line_pitch + e];
+        lds[j * ls + lslot(a.lp, e)] = v;
+    }
+    __syncthreads();
+    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, a.inverse != 0);
+    for (int idx = tid; idx < a.lpw * a.n_out; idx += nthr) {
+        const int j = idx / a.n_out, e = idx - j * a.n_out;
+        const int64_t l = l0 + j;
+        if (l < a.lines) a.data[l * a.line_pitch + e] = lds[j * ls + lslot(a.lp, e)];
+    }
+}
+
+// Thin film: per kx, the three components' ky lines (ny of 2ny inputs) are
+// transformed, multiplied by the tensor at kz = 0, transformed back and the
+// first ny outputs stored.
+template <typename Real>
+struct YConvArgs {
+    cplx<Real> *data;
+    const cplx<Real> *tw;
+    LinePlan lp;  // length py
+    int ny, hx, kpw;  // kx columns per workgroup
+    SpecGeom g;
     KernelView<Real> K;
 };
 
-// CONV = 0: plain transform; 2: forward y, tensor (ky = e, kz = 0), inverse y;
-// 3: forward z, tensor (ky = blockIdx.y, kz = e), inverse z.
-template <typename Real, int CONV>
-__global__ void __launch_bounds__(kMaxBlock) col_kernel(ColArgs<Real> a) {
+template <typename Real>
+__global__ void __launch_bounds__(kMaxBlock) yconv_kernel(YConvArgs<Real> a) {
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    constexpr int NC = CONV ? 3 : 1;
     const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
-    const int nslots = NC * a.tile;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int kx0 = blockIdx.x * a.tile;
-    const int q = blockIdx.y;
-    int64_t qoff;
-    if (CONV) qoff = (int64_t)q * a.s_lo;
-    else qoff = (int64_t)(q / a.q_div) * a.s_hi + (int64_t)(q % a.q_div) * a.s_lo;
-
-    for (int idx = tid; idx < nslots * n; idx += nthr) {
-        const int s = idx % nslots, e = idx / nslots;
-        const int c = s / a.tile, j = s - c * a.tile;
-        const int kx = kx0 + j;
+    const int kx0 = blockIdx.x * a.kpw;
+    const int nl = 3 * a.kpw;  // slot s = c * kpw + j
+    for (int idx = tid; idx < nl * n; idx += nthr) {
+        const int s = idx / n, e = idx - s * n;
+        const int c = s / a.kpw, kx = kx0 + (s - c * a.kpw);
         cplx<Real> v = {Real(0), Real(0)};
-        if (e < a.n_in && kx < a.ncols) v = a.data[c * a.comp_stride + qoff + (int64_t)e * a.estride + kx];
-        lds[s * ls + e] = v;
+        if (e < a.ny && kx < a.hx) v = a.data[a.g.at(c, 0, kx, e)];
+        lds[s * ls + lslot(a.lp, e)] = v;
     }
     __syncthreads();
-    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, CONV ? false : (a.inverse != 0));
-    if (CONV) {
+    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    for (int idx = tid; idx < a.kpw * n; idx += nthr) {
+        const int j = idx / n, e = idx - j * n;
+        const int kx = kx0 + j;
+        if (kx >= a.hx) continue;
+        Real t[6];
+        tensor_at(a.K, kx, e, 0, t);
+        const int se = lslot(a.lp, e);
+        apply_tensor(t, lds[j * ls + se], lds[(a.kpw + j) * ls + se], lds[(2 * a.kpw + j) * ls + se]);
+    }
+    __syncthreads();
+    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
+    for (int idx = tid; idx < nl * a.ny; idx += nthr) {
+        const int s = idx / a.ny, e = idx - s * a.ny;
+        const int c = s / a.kpw, kx = kx0 + (s - c * a.kpw);
+        if (kx < a.hx) a.data[a.g.at(c, 0, kx, e)] = lds[s * ls + lslot(a.lp, e)];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// z passes: one lane per (kx, ky) column
+// ---------------------------------------------------------------------------
+template <typename Real>
+struct ZArgs {
+    cplx<Real> *data;
+    const cplx<Real> *tw;  // roots of length pz
+    int nz, hx, py;
+    int64_t columns;       // hx * py
+    int64_t zstride;       // hx * py (3-D demag layout)
+    int64_t comp_stride;   // nz * hx * py
+    KernelView<Real> K;
+};
+
+// DFT of length N on registers with roots w_N^t = w_P^(2t) taken from the
+// length-P = 2N table (power-of-two N uses exact constants).
+template <typename Real, int N>
+__device__ inline void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, bool inv) {
+    if constexpr (N == 1) {
+        (void)v; (void)tw_p; (void)inv;
+    } else if constexpr (N == 2 || N == 4 || N == 8 || N == 16) {
+        dft_any<Real, N>(v, tw_p, N, inv);
+    } else {
+        cplx<Real> out[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            cplx<Real> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < N; ++r) {
+                cplx<Real> w = tw_p[2 * ((r * q) % N)];
+                if (inv) w.y = -w.y;
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = out[q];
+    }
+}
+
+// z convolution with two lanes per (kx, ky) column (pz = 2 NZ): lane parity p
+// owns kz = 2k + p.  Forward: X[2k+p] = DFT_NZ(x_z w_pz^(p z))[k] (only the
+// NZ non-zero inputs exist); tensor at its own kz; inverse:
+// y_z = IDFT_NZ(Y_even)[z] + w_pz^(-z) IDFT_NZ(Y_odd)[z], z < NZ, summed
+// across the lane pair with a shuffle.  All three components in registers.
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t col = gid >> 1;
+    const int p = (int)(gid & 1);
+    const bool valid = col < a.columns;  // both lanes of a pair stay for the shuffle
+    const int kx = valid ? (int)(col / a.py) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+    cplx<Real> v[3][NZ];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cplx<Real> x = {Real(0), Real(0)};
+            if (valid) x = a.data[c * a.comp_stride + z * a.zstride + col];
+            if (p && z) x = cmul(x, a.tw[z]);
+            v[c][z] = x;
+        }
+        half_dft<Real, NZ>(v[c], a.tw, false);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        Real t[6];
+        tensor_at(a.K, kx, ky, 2 * k + p, t);
+        apply_tensor(t, v[0][k], v[1][k], v[2][k]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        half_dft<Real, NZ>(v[c], a.tw, true);
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cplx<Real> y = v[c][z];
+            if (p && z) y = cmul(y, cconj(a.tw[z]));
+            const cplx<Real> o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+            v[c][z] = cadd(y, o);
+        }
+    }
+    if (!valid) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+            if ((z & 1) == p) a.data[c * a.comp_stride + z * a.zstride + col] = v[c][z];
+}
+
+// Long z lines: a workgroup holds `tile` consecutive columns (contiguous in
+// ky) of all three components in LDS.  conv = 0: plain forward transform of
+// one component (blockIdx.y = component), used by the Newell setup.
+template <typename Real>
+struct ZLdsArgs {
+    cplx<Real> *data;
+    const cplx<Real> *tw;
+    LinePlan lp;      // length pz
+    int n_in, n_out;
+    int tile, conv;
+    int64_t columns, zstride, comp_stride;
+    int py;
+    KernelView<Real> K;
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
+    extern __shared__ unsigned char smem_raw[];
+    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
+    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int ncomp = a.conv ? 3 : 1;
+    const int cbase = a.conv ? 0 : blockIdx.y;
+    const int nslots = ncomp * a.tile;
+    const int64_t col0 = (int64_t)blockIdx.x * a.tile;
+    for (int idx = tid; idx < nslots * n; idx += nthr) {
+        const int s = idx % nslots, e = idx / nslots;
+        const int c = s / a.tile;
+        const int64_t col = col0 + (s - c * a.tile);
+        cplx<Real> v = {Real(0), Real(0)};
+        if (e < a.n_in && col < a.columns) v = a.data[(cbase + c) * a.comp_stride + e * a.zstride + col];
+        lds[s * ls + lslot(a.lp, e)] = v;
+    }
+    __syncthreads();
+    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    if (a.conv) {
         for (int idx = tid; idx < a.tile * n; idx += nthr) {
             const int j = idx % a.tile, e = idx / a.tile;
-            const int kx = kx0 + j;
-            if (kx >= a.ncols) continue;
-            const int ky = CONV == 2 ? e : q;
-            const int kz = CONV == 2 ? 0 : e;
+            const int64_t col = col0 + j;
+            if (col >= a.columns) continue;
+            const int kx = (int)(col / a.py), ky = (int)(col - (int64_t)kx * a.py);
             Real t[6];
-            tensor_at(a.K, kx, ky, kz, t);
-            cplx<Real> &mx = lds[j * ls + e];
-            cplx<Real> &my = lds[(a.tile + j) * ls + e];
-            cplx<Real> &mz = lds[(2 * a.tile + j) * ls + e];
-            const cplx<Real> X = mx, Y = my, Z = mz;
-            mx = {-(t[0] * X.x + t[3] * Y.x + t[4] * Z.x), -(t[0] * X.y + t[3] * Y.y + t[4] * Z.y)};
-            my = {-(t[3] * X.x + t[1] * Y.x + t[5] * Z.x), -(t[3] * X.y + t[1] * Y.y + t[5] * Z.y)};
-            mz = {-(t[4] * X.x + t[5] * Y.x + t[2] * Z.x), -(t[4] * X.y + t[5] * Y.y + t[2] * Z.y)};
+            tensor_at(a.K, kx, ky, e, t);
+            const int se = lslot(a.lp, e);
+            apply_tensor(t, lds[j * ls + se], lds[(a.tile + j) * ls + se], lds[(2 * a.tile + j) * ls + se]);
         }
         __syncthreads();
         line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
     }
     for (int idx = tid; idx < nslots * a.n_out; idx += nthr) {
         const int s = idx % nslots, e = idx / nslots;
-        const int c = s / a.tile, j = s - c * a.tile;
-        const int kx = kx0 + j;
-        if (kx < a.ncols) a.data[c * a.comp_stride + qoff + (int64_t)e * a.estride + kx] = lds[s * ls + e];
+        const int c = s / a.tile;
+        const int64_t col = col0 + (s - c * a.tile);
+        if (col < a.columns) a.data[(cbase + c) * a.comp_stride + e * a.zstride + col] = lds[s * ls + lslot(a.lp, e)];
     }
 }
 
-// Octant extraction after the setup transform: kern[c][oz][oy][kx] =
-// Re spec[c][oz][oy][kx] for oy <= ny, oz <= nz (or oz = 0 in 2-D).
+// Octant extraction after the setup transform of layout spec[c][z][kx][ky]
+// (z < pz, ky < py): kern[c][oz][kx][oy] = Re spec[c][oz][kx][oy].
 template <typename Real>
-__global__ void extract_octant_kernel(const cplx<double> *spec, Real *kern, int kcomp, int kz_n, int ny, int hx,
-                                      int py, int64_t spec_comp, int64_t total) {
+__global__ void extract_octant_kernel(const cplx<double> *spec, Real *kern, int kz_n, int hx, int ny, int pz,
+                                      int py, int64_t total) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    const int64_t plane = (int64_t)(ny + 1) * hx;
-    const int64_t per_comp = (int64_t)kz_n * plane;
+    const int64_t ly = ny + 1;
+    const int64_t per_comp = (int64_t)kz_n * hx * ly;
     const int c = (int)(i / per_comp);
-    const int64_t r = i - (int64_t)c * per_comp;
-    const int oz = (int)(r / plane);
-    const int64_t r2 = r - (int64_t)oz * plane;
-    const int oy = (int)(r2 / hx), kx = (int)(r2 - (int64_t)oy * hx);
-    (void)kcomp;  // 2-D keeps only the first four components
-    kern[i] = Real(spec[c * spec_comp + ((int64_t)oz * py + oy) * hx + kx].x);
+    int64_t r = i - (int64_t)c * per_comp;
+    const int oz = (int)(r / ((int64_t)hx * ly));
+    r -= (int64_t)oz * hx * ly;
+    const int kx = (int)(r / ly), oy = (int)(r - (int64_t)kx * ly);
+    kern[i] = Real(spec[(((int64_t)c * pz + oz) * hx + kx) * py + oy].x);
 }
 
 // ---------------------------------------------------------------------------
 // Launch geometry
 // ---------------------------------------------------------------------------
-size_t lds_budget() {
-    static size_t b = 0;
-    if (b == 0) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        hipDeviceProp_t p;
-        size_t lim = 65536;
-        if (hipGetDeviceProperties(&p, dev) == hipSuccess && p.sharedMemPerBlock > 0)
-            lim = p.sharedMemPerBlock;
-        b = std::min<size_t>(lim, 98304);
+size_t lds_budget() { return 65536; }
+
+// Rows per x workgroup: the largest power of two <= 16 whose packed lines fit
+// in 64 KB of LDS (padded stride when that still fits) and 512 lanes.
+bool pick_x(const LinePlan &lp, size_t ce, int &xr, int &ls, int max_rows = 16) {
+    for (xr = max_rows; xr >= 2; xr >>= 1) {
+        const int lines = xr / 2;
+        if (lines * lp.threads > kMaxBlock) continue;
+        for (int pad = kPad; pad >= 0; --pad)
+            if ((size_t)lines * (lp.n + pad) * ce <= 65536) {
+                ls = lp.n + pad;
+                return true;
+            }
     }
-    return b;
+    return false;
 }
 
-int pick_tile(const LinePlan &lp, int ncomp, size_t elem, int ncols) {
-    int tile = 64;
-    const size_t budget = lds_budget();
-    while (tile > 1 && ((size_t)ncomp * tile * (lp.n + kPad) * elem > budget ||
-                        ncomp * tile * lp.threads > kMaxBlock))
-        tile >>= 1;
-    while (tile > 1 && tile / 2 >= ncols) tile >>= 1;
-    return tile;
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
 }
 
-int pick_lpw(const LinePlan &lp, size_t elem) {
-    int lpw = std::max(1, 256 / std::max(1, lp.threads));
-    const size_t budget = lds_budget();
-    while (lpw > 1 && ((size_t)lpw * (lp.n + kPad) * elem > budget || lpw * lp.threads > kMaxBlock)) --lpw;
+int pick_lpw(const LinePlan &lp, size_t ce, int nlines_per_unit = 1) {
+    int lpw = std::max(1, 256 / std::max(1, lp.threads * nlines_per_unit));
+    while (lpw > 1 && ((size_t)lpw * nlines_per_unit * (lp.n + kPad) * ce > lds_budget() ||
+                       lpw * nlines_per_unit * lp.threads > kMaxBlock))
+        --lpw;
     return lpw;
 }
 
-template <typename Real>
-bool geometry_ok(const DemagPlan &dp, std::string &err) {
-    const size_t e = sizeof(cplx<Real>);
-    auto fits = [&](const LinePlan &lp, int ncomp, int tile) {
-        return (size_t)ncomp * tile * (lp.n + kPad) * e <= lds_budget() && ncomp * tile * lp.threads <= kMaxBlock;
-    };
-    if (!fits(dp.lx, 1, dp.x_lines_per_wg)) { err = "x line FFT does not fit in LDS"; return false; }
-    if (dp.three_d) {
-        if (!fits(dp.ly, 1, dp.y_tile)) { err = "y line FFT does not fit in LDS"; return false; }
-        if (!fits(dp.lz, 3, dp.z_tile)) { err = "z line FFT does not fit in LDS"; return false; }
-    } else if (!fits(dp.ly, 3, dp.yconv_tile)) {
-        err = "thin-film y convolution does not fit in LDS";
-        return false;
-    }
-    return true;
-}
-
-template <typename Real>
-XArgs<Real> xargs(const DemagPlan &dp, int64_t lines, int lpw, int n_in, int n_out, Real scale) {
-    XArgs<Real> a;
-    a.tw = static_cast<const cplx<Real> *>(dp.tw_x);
-    a.lp = dp.lx;
-    a.lines = lines;
-    a.lpw = lpw;
-    a.n_in = n_in;
-    a.n_out = n_out;
-    a.hx = dp.hx;
-    a.scale = scale;
-    return a;
+int pick_ztile(const LinePlan &lp, int ncomp, size_t ce) {
+    int tile = 64;
+    while (tile > 1 && ((size_t)ncomp * tile * (lp.n + kPad) * ce > lds_budget() ||
+                        ncomp * tile * lp.threads > kMaxBlock))
+        tile >>= 1;
+    return tile;
 }
 
 template <typename Real>
@@ -420,116 +566,133 @@ KernelView<Real> kview(const Context &c) {
     return K;
 }
 
-template <typename Real>
-void launch_col(Context &c, const char *name, int mode, const ColArgs<Real> &a, dim3 grid) {
-    const int ncomp = mode ? 3 : 1;
-    const int threads = ncomp * a.tile * a.lp.threads;
-    const size_t lds = (size_t)ncomp * a.tile * (a.lp.n + kPad) * sizeof(cplx<Real>);
+struct Timed {
+    Context &c;
+    int entry = -1;
     hipEvent_t ev{};
-    if (c.timing) timing_begin(c, timing_entry(c, name), &ev);
-    if (mode == 0) hipLaunchKernelGGL((col_kernel<Real, 0>), grid, dim3(threads), lds, c.stream, a);
-    else if (mode == 2) hipLaunchKernelGGL((col_kernel<Real, 2>), grid, dim3(threads), lds, c.stream, a);
-    else hipLaunchKernelGGL((col_kernel<Real, 3>), grid, dim3(threads), lds, c.stream, a);
-    if (c.timing) timing_end(c, timing_entry(c, name), ev);
-}
-
-template <typename Real>
-SpecRows<Real> spec_rows(cplx<Real> *base, int64_t lines_per_comp, int64_t comp_stride, int lines_per_plane, int rows,
-                         int hx) {
-    SpecRows<Real> r;
-    r.base = base;
-    r.lines_per_comp = lines_per_comp;
-    r.comp_stride = comp_stride;
-    r.lines_per_plane = lines_per_plane;
-    r.rows = rows;
-    r.hx = hx;
-    return r;
-}
+    Timed(Context &cc, const char *name) : c(cc) {
+        if (c.timing) {
+            entry = timing_entry(c, name);
+            timing_begin(c, entry, &ev);
+        }
+    }
+    ~Timed() {
+        if (entry >= 0) timing_end(c, entry, ev);
+    }
+};
 
 template <typename Real, typename S>
 void demag_eval_t(Context &c, const S *src, Real *out) {
     DemagPlan &dp = c.dp;
     cplx<Real> *spec = static_cast<cplx<Real> *>(dp.spec);
-    const int64_t per_comp_lines = (int64_t)dp.ny * dp.nz;
-    const int64_t lines = 3 * per_comp_lines;
-    const int64_t comp_stride = (int64_t)dp.nz * dp.rows * dp.hx;
-    SpecRows<Real> rows = spec_rows<Real>(spec, per_comp_lines, comp_stride, dp.ny, dp.rows, dp.hx);
-
-    // x forward
+    const SpecGeom g{dp.hx, dp.three_d ? dp.nz : 1, dp.rows};
+    const size_t ce = sizeof(cplx<Real>);
+    XArgs<Real> xa{};
+    xa.tw = static_cast<const cplx<Real> *>(dp.tw_x);
+    xa.lp = dp.lx;
+    xa.rows = dp.ny;
+    xa.n_in = dp.nx;
+    xa.n_out = dp.nx;
+    xa.scale = Real(dp.inv_npad);
+    xa.g = g;
+    xa.xr = dp.xr;
+    xa.xr_log2 = ilog2(dp.xr);
+    xa.ls = dp.x_ls;
+    const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
+    const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
+    const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
     {
-        MagSource<Real, S> ms{src, c.d_mask, c.d_ms_cell, c.ms, dp.nx, per_comp_lines, (int64_t)c.n};
-        XArgs<Real> a = xargs<Real>(dp, lines, dp.x_lines_per_wg, dp.nx, 0, Real(1));
-        const int64_t pairs = (lines + 1) / 2;
-        dim3 grid((unsigned)cdiv(pairs, a.lpw));
-        const size_t lds = (size_t)a.lpw * (dp.lx.n + kPad) * sizeof(cplx<Real>);
-        hipEvent_t ev{};
-        if (c.timing) timing_begin(c, timing_entry(c, "demag_x_r2c"), &ev);
-        hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>>), grid, dim3(a.lpw * dp.lx.threads), lds, c.stream,
-                           a, ms, rows);
-        if (c.timing) timing_end(c, timing_entry(c, "demag_x_r2c"), ev);
+        MagSource<Real, S> ms{src, c.d_mask, c.d_ms_cell, c.ms, dp.nx, dp.ny, dp.nz, (int64_t)c.n};
+        Timed t(c, "demag_x_r2c");
+        hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>>), xgrid, xblock, xlds, c.stream, xa, ms, spec);
     }
-    ColArgs<Real> ca{};
-    ca.data = spec;
-    ca.ncols = dp.hx;
-    ca.K = kview<Real>(c);
+    const KernelView<Real> K = kview<Real>(c);
     if (dp.three_d) {
-        // y forward: planes q = (component, z)
-        ca.tw = static_cast<const cplx<Real> *>(dp.tw_y);
-        ca.lp = dp.ly;
-        ca.n_in = dp.ny;
-        ca.n_out = dp.py;
-        ca.estride = dp.hx;
-        ca.tile = dp.y_tile;
-        ca.q_div = dp.nz;
-        ca.s_hi = comp_stride;
-        ca.s_lo = (int64_t)dp.rows * dp.hx;
-        ca.inverse = 0;
-        ca.comp_stride = 0;
-        launch_col(c, "demag_y_fwd", 0, ca, dim3((unsigned)cdiv(dp.hx, ca.tile), 3 * dp.nz));
-        // z forward, tensor, z inverse: lines (ky, kx), three components
-        ColArgs<Real> cz = ca;
-        cz.tw = static_cast<const cplx<Real> *>(dp.tw_z);
-        cz.lp = dp.lz;
-        cz.n_in = dp.nz;
-        cz.n_out = dp.nz;
-        cz.estride = (int64_t)dp.rows * dp.hx;
-        cz.tile = dp.z_tile;
-        cz.q_div = 1;
-        cz.s_hi = 0;
-        cz.s_lo = dp.hx;
-        cz.comp_stride = comp_stride;
-        launch_col(c, "demag_z_conv", 3, cz, dim3((unsigned)cdiv(dp.hx, cz.tile), dp.py));
-        // y inverse
-        ca.n_in = dp.py;
-        ca.n_out = dp.ny;
-        ca.inverse = 1;
-        launch_col(c, "demag_y_inv", 0, ca, dim3((unsigned)cdiv(dp.hx, ca.tile), 3 * dp.nz));
+        LineArgs<Real> la{};
+        la.data = spec;
+        la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
+        la.lp = dp.ly;
+        la.lines = 3ll * dp.nz * dp.hx;
+        la.line_pitch = dp.py;
+        la.lpw = dp.y_lines_per_wg;
+        const dim3 ygrid((unsigned)cdiv(la.lines, la.lpw)), yblock((unsigned)(la.lpw * dp.ly.threads));
+        const size_t ylds = (size_t)la.lpw * (dp.ly.n + kPad) * ce;
+        {
+            la.n_in = dp.ny;
+            la.n_out = dp.py;
+            la.inverse = 0;
+            Timed t(c, "demag_y_fwd");
+            hipLaunchKernelGGL((line_kernel<Real>), ygrid, yblock, ylds, c.stream, la);
+        }
+        {
+            Timed t(c, "demag_z_conv");
+            if (dp.pz <= z_reg_max<Real>() && !dp.tune.z_lds) {
+                ZArgs<Real> za{};
+                za.data = spec;
+                za.tw = static_cast<const cplx<Real> *>(dp.tw_z);
+                za.nz = dp.nz;
+                za.hx = dp.hx;
+                za.py = dp.py;
+                za.columns = (int64_t)dp.hx * dp.py;
+                za.zstride = (int64_t)dp.hx * dp.py;
+                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
+                za.K = K;
+                const dim3 zg((unsigned)cdiv(2 * za.columns, 256)), zb(256);
+                switch (dp.nz) {
+                    case 1: hipLaunchKernelGGL((zconv_pair_kernel<Real, 1>), zg, zb, 0, c.stream, za); break;
+                    case 2: hipLaunchKernelGGL((zconv_pair_kernel<Real, 2>), zg, zb, 0, c.stream, za); break;
+                    case 3: hipLaunchKernelGGL((zconv_pair_kernel<Real, 3>), zg, zb, 0, c.stream, za); break;
+                    case 4: hipLaunchKernelGGL((zconv_pair_kernel<Real, 4>), zg, zb, 0, c.stream, za); break;
+                    case 5: hipLaunchKernelGGL((zconv_pair_kernel<Real, 5>), zg, zb, 0, c.stream, za); break;
+                    case 6: hipLaunchKernelGGL((zconv_pair_kernel<Real, 6>), zg, zb, 0, c.stream, za); break;
+                    case 7: hipLaunchKernelGGL((zconv_pair_kernel<Real, 7>), zg, zb, 0, c.stream, za); break;
+                    default: hipLaunchKernelGGL((zconv_pair_kernel<Real, 8>), zg, zb, 0, c.stream, za); break;
+                }
+            } else {
+                ZLdsArgs<Real> za{};
+                za.data = spec;
+                za.tw = static_cast<const cplx<Real> *>(dp.tw_z);
+                za.lp = dp.lz;
+                za.n_in = dp.nz;
+                za.n_out = dp.nz;
+                za.tile = dp.z_tile;
+                za.conv = 1;
+                za.columns = (int64_t)dp.hx * dp.py;
+                za.zstride = (int64_t)dp.hx * dp.py;
+                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
+                za.py = dp.py;
+                za.K = K;
+                hipLaunchKernelGGL((zlds_kernel<Real>), dim3((unsigned)cdiv(za.columns, za.tile)),
+                                   dim3((unsigned)(3 * za.tile * dp.lz.threads)),
+                                   (size_t)3 * za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+            }
+        }
+        {
+            la.n_in = dp.py;
+            la.n_out = dp.ny;
+            la.inverse = 1;
+            Timed t(c, "demag_y_inv");
+            hipLaunchKernelGGL((line_kernel<Real>), ygrid, yblock, ylds, c.stream, la);
+        }
     } else {
-        ca.tw = static_cast<const cplx<Real> *>(dp.tw_y);
-        ca.lp = dp.ly;
-        ca.n_in = dp.ny;
-        ca.n_out = dp.ny;
-        ca.estride = dp.hx;
-        ca.tile = dp.yconv_tile;
-        ca.q_div = 1;
-        ca.s_hi = 0;
-        ca.s_lo = 0;
-        ca.inverse = 0;
-        ca.comp_stride = comp_stride;
-        launch_col(c, "demag_y_conv", 2, ca, dim3((unsigned)cdiv(dp.hx, ca.tile), 1));
+        YConvArgs<Real> ya{};
+        ya.data = spec;
+        ya.tw = static_cast<const cplx<Real> *>(dp.tw_y);
+        ya.lp = dp.ly;
+        ya.ny = dp.ny;
+        ya.hx = dp.hx;
+        ya.kpw = dp.yconv_kpw;
+        ya.g = g;
+        ya.K = K;
+        Timed t(c, "demag_y_conv");
+        hipLaunchKernelGGL((yconv_kernel<Real>), dim3((unsigned)cdiv(dp.hx, ya.kpw)),
+                           dim3((unsigned)(3 * ya.kpw * dp.ly.threads)),
+                           (size_t)3 * ya.kpw * (dp.ly.n + kPad) * ce, c.stream, ya);
     }
-    // x inverse
     {
-        FieldSink<Real> sink{out, c.d_mask, dp.nx, per_comp_lines, (int64_t)c.n};
-        XArgs<Real> a = xargs<Real>(dp, lines, dp.x_lines_per_wg, 0, dp.nx, Real(dp.inv_npad));
-        const int64_t pairs = (lines + 1) / 2;
-        dim3 grid((unsigned)cdiv(pairs, a.lpw));
-        const size_t lds = (size_t)a.lpw * (dp.lx.n + kPad) * sizeof(cplx<Real>);
-        hipEvent_t ev{};
-        if (c.timing) timing_begin(c, timing_entry(c, "demag_x_c2r"), &ev);
-        hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>>), grid, dim3(a.lpw * dp.lx.threads), lds, c.stream,
-                           a, rows, sink);
-        if (c.timing) timing_end(c, timing_entry(c, "demag_x_c2r"), ev);
+        FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
+        Timed t(c, "demag_x_c2r");
+        hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>>), xgrid, xblock, xlds, c.stream, xa, spec, sink);
     }
 }
 
@@ -546,23 +709,48 @@ bool upload_twiddles(Context &c, int n, void **dst) {
     return hip_ok(hipMemcpy(*dst, wf.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)", c.last_error);
 }
 
-// Shared plan construction for the demag convolution and the Newell setup.
 bool plan_lines(DemagPlan &dp, std::string &err) {
-    // column passes prefer radices >= 8 (fewer lanes per line, wider tiles)
-    if (!plan_line(dp.px, dp.lx, err, false)) return false;
-    if (!plan_line(dp.py, dp.ly, err, true)) return false;
-    if (dp.three_d && !plan_line(dp.pz, dp.lz, err, true)) return false;
+    const DemagTuning &t = dp.tune;
+    if (!plan_line(dp.px, dp.lx, err, std::max(1, t.x_lanes), t.x_radix)) return false;
+    if (!plan_line(dp.py, dp.ly, err, std::max(1, t.y_lanes), t.y_radix)) return false;
+    if (dp.three_d && !plan_line(dp.pz, dp.lz, err)) return false;
     return true;
 }
 
-// Runs the padded forward transform of the Newell kernel (FP64 throughout)
-// into a temporary half spectrum [6][pz][py][hx].
+void parse_tuning(DemagTuning &t) {
+    const char *env = std::getenv("FULLMAG_FDM_TUNE");
+    if (!env) return;
+    std::string s(env);
+    size_t pos = 0;
+    while (pos < s.size()) {
+        size_t end = s.find(',', pos);
+        if (end == std::string::npos) end = s.size();
+        const std::string kv = s.substr(pos, end - pos);
+        const size_t eq = kv.find('=');
+        if (eq != std::string::npos) {
+            const std::string k = kv.substr(0, eq);
+            const int v = std::atoi(kv.c_str() + eq + 1);
+            if (k == "x_radix") t.x_radix = v;
+            else if (k == "x_lanes") t.x_lanes = v;
+            else if (k == "x_rows") t.x_rows = v;
+            else if (k == "y_radix") t.y_radix = v;
+            else if (k == "y_lanes") t.y_lanes = v;
+            else if (k == "y_lpw") t.y_lpw = v;
+            else if (k == "z_lds") t.z_lds = v;
+        }
+        pos = end + 1;
+    }
+}
+
+// Padded forward transform of the Newell kernel, FP64, into a temporary
+// spectrum spec[6][pz][hx][py].
 bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &oct, cplx<double> **spec_out,
                     std::string &err) {
     void *d_oct = nullptr, *d_spec = nullptr, *twx = nullptr, *twy = nullptr, *twz = nullptr;
     const int pz = dp.three_d ? dp.pz : 1;
-    const int64_t spec_comp = (int64_t)pz * dp.py * dp.hx;
-    const size_t spec_bytes = 6 * (size_t)spec_comp * sizeof(cplx<double>);
+    const SpecGeom g{dp.hx, pz, dp.py};
+    const size_t spec_bytes = 6 * (size_t)pz * dp.hx * dp.py * sizeof(cplx<double>);
+    const size_t ce = sizeof(cplx<double>);
     bool ok = hip_ok(hipMalloc(&d_oct, oct.size() * sizeof(double)), "hipMalloc(newell octant)", err) &&
               hip_ok(hipMalloc(&d_spec, spec_bytes), "hipMalloc(newell spectrum)", err) &&
               hip_ok(hipMemcpy(d_oct, oct.data(), oct.size() * sizeof(double), hipMemcpyHostToDevice),
@@ -574,70 +762,77 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
                       "hipMemcpy(twiddles)", err);
     };
     ok = ok && up(dp.px, &twx) && up(dp.py, &twy) && (!dp.three_d || up(dp.pz, &twz));
+    int xr = 2, xls = dp.px;
+    if (ok && !pick_x(dp.lx, ce, xr, xls)) {
+        err = "Newell setup: x line does not fit in LDS";
+        ok = false;
+    }
     if (ok) {
         cplx<double> *spec = static_cast<cplx<double> *>(d_spec);
-        const int kz_src = dp.three_d ? dp.nz : 1;
-        NewellSource src{static_cast<const double *>(d_oct), dp.nx, dp.ny, kz_src, dp.px, dp.py, pz};
-        const int64_t lines = 6 * (int64_t)dp.py * pz;
-        SpecRows<double> rows = spec_rows<double>(spec, (int64_t)dp.py * pz, spec_comp, dp.py, dp.py, dp.hx);
-        XArgs<double> a{};
-        a.tw = static_cast<const cplx<double> *>(twx);
-        a.lp = dp.lx;
-        a.lines = lines;
-        a.lpw = pick_lpw(dp.lx, sizeof(cplx<double>));
-        a.n_in = dp.px;
-        a.n_out = 0;
-        a.hx = dp.hx;
-        a.scale = 1.0;
-        const size_t lds = (size_t)a.lpw * (dp.lx.n + kPad) * sizeof(cplx<double>);
-        hipLaunchKernelGGL((xfwd_kernel<double, NewellSource>), dim3((unsigned)cdiv((lines + 1) / 2, a.lpw)),
-                           dim3(a.lpw * dp.lx.threads), lds, c.stream, a, src, rows);
-        ColArgs<double> ca{};
-        ca.data = spec;
-        ca.ncols = dp.hx;
-        ca.tw = static_cast<const cplx<double> *>(twy);
-        ca.lp = dp.ly;
-        ca.n_in = dp.py;
-        ca.n_out = dp.py;
-        ca.estride = dp.hx;
-        ca.tile = pick_tile(dp.ly, 1, sizeof(cplx<double>), dp.hx);
-        ca.q_div = pz;
-        ca.s_hi = spec_comp;
-        ca.s_lo = (int64_t)dp.py * dp.hx;
-        ca.inverse = 0;
-        if ((size_t)ca.tile * (dp.ly.n + kPad) * sizeof(cplx<double>) > lds_budget()) {
+        NewellSource src{static_cast<const double *>(d_oct), dp.nx, dp.ny, dp.three_d ? dp.nz : 1, dp.px, dp.py, pz};
+        XArgs<double> xa{};
+        xa.tw = static_cast<const cplx<double> *>(twx);
+        xa.lp = dp.lx;
+        xa.rows = dp.py;
+        xa.n_in = dp.px;
+        xa.n_out = 0;
+        xa.scale = 1.0;
+        xa.g = g;
+        xa.xr = xr;
+        xa.xr_log2 = ilog2(xr);
+        xa.ls = xls;
+        hipLaunchKernelGGL((xfwd_kernel<double, NewellSource>), dim3((unsigned)cdiv(dp.py, xr), pz, 6),
+                           dim3((unsigned)((xr / 2) * dp.lx.threads)), (size_t)(xr / 2) * xls * ce, c.stream, xa, src,
+                           spec);
+        LineArgs<double> la{};
+        la.data = spec;
+        la.tw = static_cast<const cplx<double> *>(twy);
+        la.lp = dp.ly;
+        la.lines = 6ll * pz * dp.hx;
+        la.line_pitch = dp.py;
+        la.lpw = pick_lpw(dp.ly, ce);
+        la.n_in = dp.py;
+        la.n_out = dp.py;
+        la.inverse = 0;
+        if ((size_t)la.lpw * (dp.ly.n + kPad) * ce > lds_budget()) {
             err = "Newell setup: y line does not fit in LDS";
             ok = false;
         }
-        if (ok) {
-            hipLaunchKernelGGL((col_kernel<double, 0>), dim3((unsigned)cdiv(dp.hx, ca.tile), 6 * pz),
-                               dim3(ca.tile * dp.ly.threads), (size_t)ca.tile * (dp.ly.n + kPad) * sizeof(cplx<double>),
-                               c.stream, ca);
-        }
+        if (ok)
+            hipLaunchKernelGGL((line_kernel<double>), dim3((unsigned)cdiv(la.lines, la.lpw)),
+                               dim3((unsigned)(la.lpw * dp.ly.threads)), (size_t)la.lpw * (dp.ly.n + kPad) * ce,
+                               c.stream, la);
         if (ok && dp.three_d) {
-            ColArgs<double> cz = ca;
-            cz.tw = static_cast<const cplx<double> *>(twz);
-            cz.lp = dp.lz;
-            cz.n_in = dp.pz;
-            cz.n_out = dp.pz;
-            cz.estride = (int64_t)dp.py * dp.hx;
-            cz.tile = pick_tile(dp.lz, 1, sizeof(cplx<double>), dp.hx);
-            cz.q_div = dp.py;
-            cz.s_hi = spec_comp;
-            cz.s_lo = dp.hx;
-            hipLaunchKernelGGL((col_kernel<double, 0>), dim3((unsigned)cdiv(dp.hx, cz.tile), 6 * dp.py),
-                               dim3(cz.tile * dp.lz.threads),
-                               (size_t)cz.tile * (dp.lz.n + kPad) * sizeof(cplx<double>), c.stream, cz);
+            ZLdsArgs<double> za{};
+            za.data = spec;
+            za.tw = static_cast<const cplx<double> *>(twz);
+            za.lp = dp.lz;
+            za.n_in = dp.pz;
+            za.n_out = dp.pz;
+            za.tile = pick_ztile(dp.lz, 1, ce);
+            za.conv = 0;
+            za.columns = (int64_t)dp.hx * dp.py;
+            za.zstride = (int64_t)dp.hx * dp.py;
+            za.comp_stride = (int64_t)dp.pz * dp.hx * dp.py;
+            za.py = dp.py;
+            if ((size_t)za.tile * (dp.lz.n + kPad) * ce > lds_budget()) {
+                err = "Newell setup: z line does not fit in LDS";
+                ok = false;
+            } else {
+                hipLaunchKernelGGL((zlds_kernel<double>), dim3((unsigned)cdiv(za.columns, za.tile), 6),
+                                   dim3((unsigned)(za.tile * dp.lz.threads)),
+                                   (size_t)za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+            }
         }
         ok = ok && hip_ok(hipStreamSynchronize(c.stream), "Newell setup transform", err);
         ok = ok && hip_ok(hipGetLastError(), "Newell setup launch", err);
     }
-    if (d_oct) hipFree(d_oct);
-    if (twx) hipFree(twx);
-    if (twy) hipFree(twy);
-    if (twz) hipFree(twz);
+    if (d_oct) (void)hipFree(d_oct);
+    if (twx) (void)hipFree(twx);
+    if (twy) (void)hipFree(twy);
+    if (twz) (void)hipFree(twz);
     if (!ok) {
-        if (d_spec) hipFree(d_spec);
+        if (d_spec) (void)hipFree(d_spec);
         return false;
     }
     *spec_out = static_cast<cplx<double> *>(d_spec);
@@ -651,9 +846,9 @@ bool host_octant_from_spectra(const DemagPlan &dp, const fullmag_fdm_plan_desc &
                             plan.demag_kernel_zz_spectrum, plan.demag_kernel_xy_spectrum,
                             plan.demag_kernel_xz_spectrum, plan.demag_kernel_yz_spectrum};
     const int pz = dp.three_d ? dp.pz : 1;
-    const int64_t plane = (int64_t)(dp.ny + 1) * dp.hx;
-    kern.assign((size_t)dp.kcomp * dp.kz_n * plane, 0.0);
-    // parity signs of the six components along (x, y, z)
+    const int64_t ly = dp.ny + 1;
+    kern.assign((size_t)dp.kcomp * dp.kz_n * dp.hx * ly, 0.0);
+    // parity of the six components along (x, y, z)
     const int odd[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
     double kmax = 0.0, bad = 0.0;
     for (int c = 0; c < 6; ++c) {
@@ -665,7 +860,6 @@ bool host_octant_from_spectra(const DemagPlan &dp, const fullmag_fdm_plan_desc &
                     const double re = s[2 * i], im = s[2 * i + 1];
                     kmax = std::max(kmax, std::fabs(re));
                     bad = std::max(bad, std::fabs(im));
-                    // parity against the mirrored octant entry
                     const int ox = x <= dp.nx ? x : dp.px - x;
                     const int oy = y <= dp.ny ? y : dp.py - y;
                     const int oz = (pz == 1 || z <= dp.nz) ? z : pz - z;
@@ -676,7 +870,7 @@ bool host_octant_from_spectra(const DemagPlan &dp, const fullmag_fdm_plan_desc &
                     const int64_t j = ((int64_t)oz * dp.py + oy) * dp.px + ox;
                     bad = std::max(bad, std::fabs(re - sg * s[2 * j]));
                     if (x <= dp.nx && y <= dp.ny && (pz == 1 || z <= dp.nz) && c < dp.kcomp)
-                        kern[((size_t)c * dp.kz_n + z) * plane + (int64_t)y * dp.hx + x] = re;
+                        kern[(((size_t)c * dp.kz_n + z) * dp.hx + x) * ly + y] = re;
                 }
     }
     if (bad > 1e-8 * std::max(kmax, 1e-300)) {
@@ -712,20 +906,38 @@ bool upload_kern(Context &c, const std::vector<double> &kern) {
 }
 
 template <typename Real>
-bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_device) {
+bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
     DemagPlan &dp = c.dp;
     const size_t ce = sizeof(cplx<Real>);
-    dp.x_lines_per_wg = pick_lpw(dp.lx, ce);
-    if (dp.three_d) {
-        dp.y_tile = pick_tile(dp.ly, 1, ce, dp.hx);
-        dp.z_tile = pick_tile(dp.lz, 3, ce, dp.hx);
-    } else {
-        dp.yconv_tile = pick_tile(dp.ly, 3, ce, dp.hx);
+    if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, dp.tune.x_rows > 0 ? dp.tune.x_rows : 16)) {
+        c.last_error = "x line FFT does not fit in one workgroup";
+        return false;
     }
-    if (!geometry_ok<Real>(dp, c.last_error)) return false;
+    if (dp.three_d) {
+        dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
+        if (dp.tune.y_lpw > 0) dp.y_lines_per_wg = std::min(dp.y_lines_per_wg, dp.tune.y_lpw);
+        if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > lds_budget()) {
+            c.last_error = "y line FFT does not fit in LDS";
+            return false;
+        }
+        if (dp.pz > z_reg_max<Real>() || dp.tune.z_lds) {
+            dp.z_tile = pick_ztile(dp.lz, 3, ce);
+            if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > lds_budget()) {
+                c.last_error = "z line FFT does not fit in LDS";
+                return false;
+            }
+        }
+    } else {
+        dp.yconv_kpw = pick_lpw(dp.ly, ce, 3);
+        if ((size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > lds_budget() ||
+            3 * dp.yconv_kpw * dp.ly.threads > kMaxBlock) {
+            c.last_error = "thin-film y convolution does not fit in one workgroup";
+            return false;
+        }
+    }
     if (!upload_twiddles<Real>(c, dp.px, &dp.tw_x) || !upload_twiddles<Real>(c, dp.py, &dp.tw_y)) return false;
     if (dp.three_d && !upload_twiddles<Real>(c, dp.pz, &dp.tw_z)) return false;
-    dp.spec_elems = 3ull * dp.nz * dp.rows * dp.hx;
+    dp.spec_elems = 3ull * (dp.three_d ? dp.nz : 1) * dp.hx * dp.rows;
     if (!ctx_alloc(c, &dp.spec, dp.spec_elems * ce, "demag spectrum")) return false;
     if (dp.projection) return true;
     if (plan.demag_kernel_spectrum_len != 0) {
@@ -733,23 +945,21 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_devic
         if (!host_octant_from_spectra(dp, plan, kern, c.last_error)) return false;
         return upload_kern<Real>(c, kern);
     }
-    (void)newell_on_device;
     // In-library Newell build: octant on the host, padded transform on the device.
     std::vector<double> oct;
     newell_octant(dp.nx, dp.ny, dp.three_d ? dp.nz : 1, c.dx, c.dy, c.dz, oct);
     cplx<double> *spec = nullptr;
     if (!newell_forward(c, dp, oct, &spec, c.last_error)) return false;
-    dp.kern_elems = (uint64_t)dp.kcomp * dp.kz_n * (dp.ny + 1) * dp.hx;
+    dp.kern_elems = (uint64_t)dp.kcomp * dp.kz_n * dp.hx * (dp.ny + 1);
     bool ok = ctx_alloc(c, &dp.kern, dp.kern_elems * sizeof(Real), "demag tensor");
     if (ok) {
         const int pz = dp.three_d ? dp.pz : 1;
-        const int64_t spec_comp = (int64_t)pz * dp.py * dp.hx;
         hipLaunchKernelGGL((extract_octant_kernel<Real>), dim3((unsigned)cdiv(dp.kern_elems, 256)), dim3(256), 0,
-                           c.stream, spec, static_cast<Real *>(dp.kern), dp.kcomp, dp.kz_n, dp.ny, dp.hx, dp.py,
-                           spec_comp, (int64_t)dp.kern_elems);
+                           c.stream, spec, static_cast<Real *>(dp.kern), dp.kz_n, dp.hx, dp.ny, pz, dp.py,
+                           (int64_t)dp.kern_elems);
         ok = hip_ok(hipStreamSynchronize(c.stream), "extract octant", c.last_error);
     }
-    hipFree(spec);
+    (void)hipFree(spec);
     return ok;
 }
 
@@ -776,9 +986,10 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     dp.kcomp = dp.three_d ? 6 : 4;
     dp.kz_n = dp.three_d ? dp.nz + 1 : 1;
     dp.inv_npad = 1.0 / ((double)dp.px * dp.py * dp.pz);
+    parse_tuning(dp.tune);
     if (!plan_lines(dp, c.last_error)) return false;
-    if (c.mode == MODE_DOUBLE) return setup_t<double>(c, plan, newell_on_device);
-    return setup_t<float>(c, plan, newell_on_device);
+    if (c.mode == MODE_DOUBLE) return setup_t<double>(c, plan);
+    return setup_t<float>(c, plan);
 }
 
 void demag_eval(Context &c, const void *src, bool src_f64, void *out) {
@@ -811,11 +1022,11 @@ bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, doubl
     }
     if (ok) {
         const int pz = dp.pz;
-        const int64_t spec_comp = (int64_t)pz * dp.py * dp.hx;
-        std::vector<cplx<double>> h((size_t)6 * spec_comp);
+        const SpecGeom g{dp.hx, pz, dp.py};
+        std::vector<cplx<double>> h((size_t)6 * pz * dp.hx * dp.py);
         ok = hip_ok(hipMemcpy(h.data(), spec, h.size() * sizeof(cplx<double>), hipMemcpyDeviceToHost),
                     "hipMemcpy(newell spectrum)", err);
-        hipFree(spec);
+        (void)hipFree(spec);
         if (ok) {
             const int64_t npad = (int64_t)dp.px * dp.py * pz;
             for (int comp = 0; comp < 6; ++comp)
@@ -824,11 +1035,10 @@ bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, doubl
                         for (int x = 0; x < dp.px; ++x) {
                             cplx<double> v;
                             if (x < dp.hx) {
-                                v = h[comp * spec_comp + ((int64_t)z * dp.py + y) * dp.hx + x];
+                                v = h[g.at(comp, z, x, y)];
                             } else {
                                 // Hermitian continuation of a real kernel
-                                const int zz = (pz - z) % pz, yy = (dp.py - y) % dp.py, xx = dp.px - x;
-                                v = h[comp * spec_comp + ((int64_t)zz * dp.py + yy) * dp.hx + xx];
+                                v = h[g.at(comp, (pz - z) % pz, dp.px - x, (dp.py - y) % dp.py)];
                                 v.y = -v.y;
                             }
                             const int64_t i = ((int64_t)z * dp.py + y) * dp.px + x;
@@ -837,7 +1047,7 @@ bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, doubl
                         }
         }
     }
-    hipStreamDestroy(c.stream);
+    (void)hipStreamDestroy(c.stream);
     return ok;
 }
 

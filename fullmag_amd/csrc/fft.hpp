@@ -26,14 +26,24 @@ constexpr int kMaxPass = 16;
 struct LinePlan {
     int n = 0;        // line length
     int npass = 0;    // number of Stockham passes
-    int threads = 0;  // lanes per line = n / min radix
+    int threads = 0;  // lanes per line (each holds up to 16 complex values)
+    int swz = 0;      // 1: LDS slots XOR-swizzled within 16-element blocks
     int radix[kMaxPass] = {};
 };
 
-// Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13}.  prefer_wide first
-// tries power-of-two radices of 8 and 16 only.  Returns false (with `err`)
-// for lengths with a prime factor above 13 or needing more than 1024 lanes.
-bool plan_line(int n, LinePlan &lp, std::string &err, bool prefer_wide = false);
+// LDS slot of line element i.  With swz set (n a multiple of 16) element
+// 16b + c lives at 16b + (c ^ (b & 15)): the stride-16 writes of the first
+// Stockham pass then spread over 32 banks instead of 2.  Bijective within
+// each 16-element block.
+__host__ __device__ inline int lslot(const LinePlan &lp, int i) { return lp.swz ? (i ^ ((i >> 4) & 15)) : i; }
+
+// Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13} and the lanes per line:
+// the fewest lanes such that no lane holds more than 16 complex values in any
+// pass, or `min_lanes` if larger (capped at the largest pass's butterfly
+// count); power-of-two radices are capped at max_radix (16, 8 or 4).  Returns
+// false (with `err`) for lengths with a prime factor above 13 or needing more
+// than 1024 lanes.
+bool plan_line(int n, LinePlan &lp, std::string &err, int min_lanes = 1, int max_radix = 16);
 
 // w[t] = exp(-2 pi i t / n), t in [0, n), correctly rounded from long double.
 std::vector<cplx<double>> twiddle_table(int n);
@@ -155,35 +165,76 @@ __device__ inline void dft_any(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
     }
 }
 
-// One Stockham pass over a line in LDS.  Every lane of the workgroup must
-// call it (it contains barriers); `t` is the lane's index within its line.
+// v[r] *= w^(r st), r = 1..R-1, where w^t = tw[t] (r st < n).  For
+// power-of-two R only the roots w^(2^b st) are read from the table; each
+// other power is one product of an earlier power and a table root, so no
+// twiddle carries more than three extra roundings.
 template <typename T, int R>
-__device__ inline void stockham_pass(cplx<T> *line, int n, int ns, int t, const cplx<T> *tw, bool inv) {
-    const int nr = n / R;
-    const bool act = t < nr;
-    cplx<T> v[R];
-    int base = 0;
-    if (act) {
-        const int k = t % ns;
+__device__ inline void twiddle(cplx<T> *v, const cplx<T> *tw, int st, bool inv) {
+    if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+        constexpr int LB = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+        cplx<T> pw[LB];
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = line[t + r * nr];
-        if (k != 0) {
-            // r k n/(ns R) < n for r < R, k < ns: no modular reduction needed
-            const int stride = n / (ns * R);
-#pragma unroll
-            for (int r = 1; r < R; ++r) {
-                cplx<T> w = tw[r * k * stride];
-                if (inv) w.y = -w.y;
-                v[r] = cmul(v[r], w);
-            }
+        for (int b = 0; b < LB; ++b) {
+            pw[b] = tw[st << b];
+            if (inv) pw[b].y = -pw[b].y;
         }
-        dft_any<T, R>(v, tw, n, inv);
-        base = (t / ns) * ns * R + k;
+        cplx<T> w[R];
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            const int low = r & -r;
+            const int lb = low == 1 ? 0 : low == 2 ? 1 : low == 4 ? 2 : 3;
+            w[r] = (r == low) ? pw[lb] : cmul(w[r - low], pw[lb]);
+            v[r] = cmul(v[r], w[r]);
+        }
+    } else {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+            cplx<T> w = tw[r * st];
+            if (inv) w.y = -w.y;
+            v[r] = cmul(v[r], w);
+        }
+    }
+}
+
+// Butterflies one lane may hold in a radix-R pass (16 complex values).
+template <int R>
+constexpr int lane_butterflies() {
+    return R >= 16 ? 1 : 16 / R;
+}
+
+// One Stockham pass over a line in LDS.  Every lane of the workgroup must
+// call it (it contains barriers); `t` is the lane's index within its line
+// and `nt` the number of lanes per line.  Lane t handles butterflies t,
+// t + nt, ... (at most lane_butterflies<R>()); the planner guarantees that
+// covers all n/R of them.
+template <typename T, int R>
+__device__ inline void stockham_pass(cplx<T> *line, const LinePlan &lp, int ns, int t, const cplx<T> *tw, bool inv) {
+    constexpr int MB = lane_butterflies<R>();
+    const int n = lp.n, nt = lp.threads;
+    const int nr = n / R;
+    cplx<T> v[MB][R];
+    int base[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = t + b * nt;
+        base[b] = -1;
+        if (j < nr) {
+            const int k = j % ns;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = line[lslot(lp, j + r * nr)];
+            if (k != 0) twiddle<T, R>(v[b], tw, k * (n / (ns * R)), inv);
+            dft_any<T, R>(v[b], tw, n, inv);
+            base[b] = (j / ns) * ns * R + k;
+        }
     }
     __syncthreads();
-    if (act) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) line[base + r * ns] = v[r];
+    for (int b = 0; b < MB; ++b) {
+        if (base[b] >= 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[lslot(lp, base[b] + r * ns)] = v[b][r];
+        }
     }
     __syncthreads();
 }
@@ -195,18 +246,45 @@ __device__ inline void line_fft(cplx<T> *line, const LinePlan &lp, int t, const 
     for (int p = 0; p < lp.npass; ++p) {
         const int R = lp.radix[p];
         switch (R) {
-            case 2: stockham_pass<T, 2>(line, lp.n, ns, t, tw, inv); break;
-            case 3: stockham_pass<T, 3>(line, lp.n, ns, t, tw, inv); break;
-            case 4: stockham_pass<T, 4>(line, lp.n, ns, t, tw, inv); break;
-            case 5: stockham_pass<T, 5>(line, lp.n, ns, t, tw, inv); break;
-            case 7: stockham_pass<T, 7>(line, lp.n, ns, t, tw, inv); break;
-            case 8: stockham_pass<T, 8>(line, lp.n, ns, t, tw, inv); break;
-            case 11: stockham_pass<T, 11>(line, lp.n, ns, t, tw, inv); break;
-            case 13: stockham_pass<T, 13>(line, lp.n, ns, t, tw, inv); break;
-            case 16: stockham_pass<T, 16>(line, lp.n, ns, t, tw, inv); break;
+            case 2: stockham_pass<T, 2>(line, lp, ns, t, tw, inv); break;
+            case 3: stockham_pass<T, 3>(line, lp, ns, t, tw, inv); break;
+            case 4: stockham_pass<T, 4>(line, lp, ns, t, tw, inv); break;
+            case 5: stockham_pass<T, 5>(line, lp, ns, t, tw, inv); break;
+            case 7: stockham_pass<T, 7>(line, lp, ns, t, tw, inv); break;
+            case 8: stockham_pass<T, 8>(line, lp, ns, t, tw, inv); break;
+            case 11: stockham_pass<T, 11>(line, lp, ns, t, tw, inv); break;
+            case 13: stockham_pass<T, 13>(line, lp, ns, t, tw, inv); break;
+            case 16: stockham_pass<T, 16>(line, lp, ns, t, tw, inv); break;
             default: break;
         }
         ns *= R;
+    }
+}
+
+// Register-resident DFT of a short line (all n values in one lane), used for
+// the z transforms of thin films.  Direct O(n^2) sum with table roots for n
+// not a power of two <= 16.
+template <typename T, int N>
+__device__ inline void reg_dft(cplx<T> *v, const cplx<T> *tw, bool inv) {
+    if constexpr (N == 1) {
+        (void)v; (void)tw; (void)inv;
+    } else if constexpr (N == 2 || N == 4 || N == 8 || N == 16) {
+        dft_any<T, N>(v, tw, N, inv);
+    } else {
+        cplx<T> out[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            cplx<T> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < N; ++r) {
+                cplx<T> w = tw[(r * q) % N];
+                if (inv) w.y = -w.y;
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = out[q];
     }
 }
 

@@ -1,18 +1,26 @@
 // fft_plan.cpp -- host planning for the LDS line FFTs (see fft.hpp).
 #include "fft.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace fm {
 
 namespace {
 
-void pow2_radices(int k, bool wide, std::vector<int> &out) {
+void pow2_radices(int k, bool wide, std::vector<int> &out, int max_radix) {
     // k = log2 of the power-of-two factor.  Default: as many radix-16 passes
     // as possible with a radix-8/4 tail (few LDS round trips, >= 4 lanes of
     // work per pass).  `wide` restricts to radices 8 and 16 so that long
     // lines fit in 1024 lanes.
     if (k == 0) return;
+    if (max_radix < 16) {
+        // radix <= max_radix (8 or 4), largest first, remainder last
+        const int lr = max_radix >= 8 ? 3 : 2;
+        while (k >= lr) { out.push_back(1 << lr); k -= lr; }
+        if (k > 0) out.push_back(1 << k);
+        return;
+    }
     if (wide) {
         int y = k / 4;
         while (y >= 0 && (k - 4 * y) % 3 != 0) --y;
@@ -40,7 +48,7 @@ void pow2_radices(int k, bool wide, std::vector<int> &out) {
 
 }  // namespace
 
-bool plan_line(int n, LinePlan &lp, std::string &err, bool prefer_wide) {
+bool plan_line(int n, LinePlan &lp, std::string &err, int min_lanes, int max_radix) {
     lp = LinePlan{};
     if (n < 1) {
         err = "FFT line length must be >= 1";
@@ -62,25 +70,31 @@ bool plan_line(int n, LinePlan &lp, std::string &err, bool prefer_wide) {
               " has a prime factor above 13, which the line FFT does not support";
         return false;
     }
-    const int order[2][2] = {{0, 1}, {1, 0}};
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        const int wide = order[prefer_wide ? 1 : 0][attempt];
-        std::vector<int> rad;
-        pow2_radices(k, wide != 0, rad);
-        if (!rad.empty() && rad.back() < 0) continue;
-        for (int p : odd) rad.push_back(p);
-        if ((int)rad.size() > kMaxPass) continue;
-        int rmin = 1 << 30;
-        for (int r : rad) rmin = r < rmin ? r : rmin;
-        int threads = n / rmin;
-        if (threads > 1024) continue;
-        lp.npass = (int)rad.size();
-        for (int i = 0; i < lp.npass; ++i) lp.radix[i] = rad[i];
-        lp.threads = threads;
-        return true;
+    std::vector<int> rad;
+    pow2_radices(k, false, rad, max_radix);
+    for (int p : odd) rad.push_back(p);
+    if ((int)rad.size() > kMaxPass) {
+        err = "padded FFT length " + std::to_string(n) + " needs too many passes";
+        return false;
     }
-    err = "padded FFT length " + std::to_string(n) + " needs more than 1024 lanes per line";
-    return false;
+    // lanes: every pass's n/R butterflies must fit in lanes * (16 / R)
+    int lanes = 1, most = 1;
+    for (int r : rad) {
+        const int mb = r >= 16 ? 1 : 16 / r;
+        const int nr = n / r;
+        lanes = std::max(lanes, (nr + mb - 1) / mb);
+        most = std::max(most, nr);
+    }
+    lanes = std::max(lanes, std::min(min_lanes, most));
+    if (lanes > 1024) {
+        err = "padded FFT length " + std::to_string(n) + " needs more than 1024 lanes per line";
+        return false;
+    }
+    lp.npass = (int)rad.size();
+    for (int i = 0; i < lp.npass; ++i) lp.radix[i] = rad[i];
+    lp.threads = lanes;
+    lp.swz = (n % 16 == 0) ? 1 : 0;
+    return true;
 }
 
 std::vector<cplx<double>> twiddle_table(int n) {

@@ -119,6 +119,49 @@ __device__ inline void neu_add(double &s, double &c, double v) {
 
 enum { STAGE_PLAIN = 0, STAGE_STATS = 1, STAGE_ERROR = 2 };
 
+// Combines two compensated partial sums (s, c) += (s2, c2).
+__device__ inline void neu_merge(double &s, double &c, double s2, double c2) {
+    neu_add(s, c, s2);
+    c += c2;
+}
+
+// Workgroup reduction of the per-lane accumulators: wave shuffles, then the
+// wavefront partials through LDS; lane 0 writes the result to `out`.
+template <int MODE>
+__device__ inline void block_reduce(double *acc, double *out) {
+    constexpr int NQ = MODE == STAGE_STATS ? kNStat : 1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        if (MODE == STAGE_STATS) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const double s2 = __shfl_down(acc[2 * e], off), c2 = __shfl_down(acc[2 * e + 1], off);
+                neu_merge(acc[2 * e], acc[2 * e + 1], s2, c2);
+            }
+#pragma unroll
+            for (int q = 6; q < 9; ++q) acc[q] = fmax(acc[q], __shfl_down(acc[q], off));
+        } else {
+            acc[0] = fmax(acc[0], __shfl_down(acc[0], off));
+        }
+    }
+    __shared__ double red[kBlock / 64][NQ];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int q = 0; q < NQ; ++q) red[wave][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            if (MODE == STAGE_STATS) {
+                for (int e = 0; e < 3; ++e) neu_merge(red[0][2 * e], red[0][2 * e + 1], red[w][2 * e], red[w][2 * e + 1]);
+                for (int q = 6; q < 9; ++q) red[0][q] = fmax(red[0][q], red[w][q]);
+            } else {
+                red[0][0] = fmax(red[0][0], red[w][0]);
+            }
+        }
+        for (int q = 0; q < NQ; ++q) out[q] = red[0][q];
+    }
+}
+
 template <typename State, typename Real, typename Src, typename Out>
 struct StageArgs {
     FieldArgs f;
@@ -213,71 +256,25 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
         }
     }
     if (MODE == STAGE_PLAIN) return;
-    __shared__ double red[kNStat][kBlock];
-    const int tid = threadIdx.x;
-    const int nq = MODE == STAGE_STATS ? kNStat : 1;
-    for (int q = 0; q < nq; ++q) red[q][tid] = acc[q];
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if (tid < off) {
-            if (MODE == STAGE_STATS) {
-                for (int e = 0; e < 3; ++e) {
-                    double s = red[2 * e][tid], c = red[2 * e + 1][tid];
-                    neu_add(s, c, red[2 * e][tid + off]);
-                    red[2 * e][tid] = s;
-                    red[2 * e + 1][tid] = c + red[2 * e + 1][tid + off];
-                }
-                for (int q = 6; q < 9; ++q) red[q][tid] = fmax(red[q][tid], red[q][tid + off]);
-            } else {
-                red[0][tid] = fmax(red[0][tid], red[0][tid + off]);
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0)
-        for (int q = 0; q < nq; ++q) a.partials[(int64_t)blockIdx.x * nq + q] = red[q][0];
+    block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * (MODE == STAGE_STATS ? kNStat : 1));
 }
 
 // Reduces per-block partials into out[0..nq).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) finalize_kernel(const double *partials, int nblocks, double *out) {
     constexpr int nq = MODE == STAGE_STATS ? kNStat : 1;
-    __shared__ double red[nq][kBlock];
-    const int tid = threadIdx.x;
-    double acc[nq];
-    for (int q = 0; q < nq; ++q) acc[q] = 0.0;
-    for (int b = tid; b < nblocks; b += kBlock) {
+    double acc[kNStat];
+    for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
         const double *p = partials + (int64_t)b * nq;
         if (MODE == STAGE_STATS) {
-            for (int e = 0; e < 3; ++e) {
-                neu_add(acc[2 * e], acc[2 * e + 1], p[2 * e]);
-                acc[2 * e + 1] += p[2 * e + 1];
-            }
+            for (int e = 0; e < 3; ++e) neu_merge(acc[2 * e], acc[2 * e + 1], p[2 * e], p[2 * e + 1]);
             for (int q = 6; q < 9; ++q) acc[q] = fmax(acc[q], p[q]);
         } else {
             acc[0] = fmax(acc[0], p[0]);
         }
     }
-    for (int q = 0; q < nq; ++q) red[q][tid] = acc[q];
-    __syncthreads();
-    for (int off = kBlock / 2; off > 0; off >>= 1) {
-        if (tid < off) {
-            if (MODE == STAGE_STATS) {
-                for (int e = 0; e < 3; ++e) {
-                    double s = red[2 * e][tid], c = red[2 * e + 1][tid];
-                    neu_add(s, c, red[2 * e][tid + off]);
-                    red[2 * e][tid] = s;
-                    red[2 * e + 1][tid] = c + red[2 * e + 1][tid + off];
-                }
-                for (int q = 6; q < 9; ++q) red[q][tid] = fmax(red[q][tid], red[q][tid + off]);
-            } else {
-                red[0][tid] = fmax(red[0][tid], red[0][tid + off]);
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0)
-        for (int q = 0; q < nq; ++q) out[q] = red[q][0];
+    block_reduce<MODE>(acc, out);
 }
 
 // out = normalise(m0 + dt sum_j a_j k_j), no field evaluation (DP45 FSAL
@@ -417,9 +414,7 @@ struct Driver {
     static constexpr bool kRealF64 = sizeof(Real) == 8;
 
     unsigned grid_plain() const { return (unsigned)std::max<uint64_t>(1, cdiv(c.n, kBlock)); }
-    unsigned grid_reduce() const {
-        return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(cdiv(c.n, kBlock), (uint64_t)c.max_partial_blocks));
-    }
+    unsigned grid_reduce() const { return grid_plain(); }
 
     template <typename Src, typename Out>
     StageArgs<State, Real, Src, Out> args(const Src *src, const Real *hdp) {
@@ -769,7 +764,7 @@ bool alloc_integrator_buffers(Context &c) {
     for (int i = 0; i < nmt; ++i)
         if (!ctx_alloc(c, &c.mt[i], rb, "stage point")) return false;
     if (need_mst && !ctx_alloc(c, &c.mst, sb, "candidate state")) return false;
-    c.max_partial_blocks = 2048;
+    c.max_partial_blocks = (int)std::max<uint64_t>(1, cdiv(c.n, kBlock));
     if (!ctx_alloc(c, (void **)&c.d_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double), "partials"))
         return false;
     if (!ctx_alloc(c, (void **)&c.d_result, 16 * sizeof(double), "result")) return false;
