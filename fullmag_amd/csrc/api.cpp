@@ -119,6 +119,11 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
             return c;
         }
         const uint64_t n = c->n;
+        if (n >= (uint64_t(1) << 32)) {
+            // cell coordinates are derived with 32-bit index arithmetic
+            c->last_error = "grid has " + std::to_string(n) + " cells; at most 2^32 - 1 are supported";
+            return c;
+        }
         if (p.initial_magnetization_len != 3 * n || !p.initial_magnetization_xyz) {
             c->last_error = "initial_magnetization_len mismatch: expected " + std::to_string(3 * n) + ", got " +
                             std::to_string(p.initial_magnetization_len);

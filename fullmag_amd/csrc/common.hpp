@@ -31,6 +31,29 @@ __host__ __device__ inline cplx<T> cconj(cplx<T> a) { return {a.x, -a.y}; }
 // Integer ceil-div for launch geometry.
 __host__ __device__ inline uint64_t cdiv(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
 
+// Division of 32-bit unsigned integers by a fixed divisor d >= 1 with one
+// multiply-high (the round-up method of Granlund & Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994): with l = ceil(log2 d)
+// and m = floor(2^32 (2^l - d) / d) + 1, x / d = (mulhi(m, x) + x) >> l for
+// every x < 2^32 (the sum taken in 64 bits).
+struct FastDiv {
+    uint32_t d = 1, m = 1, l = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t div) : d(div) {
+        l = 0;
+        while ((uint64_t(1) << l) < div) ++l;
+        m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << l) - div)) / div + 1);
+    }
+    __host__ __device__ inline uint32_t div(uint32_t x) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t t = __umulhi(m, x);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)m * x) >> 32);
+#endif
+        return (uint32_t)(((uint64_t)t + x) >> l);
+    }
+};
+
 // Records a HIP failure into `err` and returns false.
 inline bool hip_ok(hipError_t e, const char *what, std::string &err) {
     if (e == hipSuccess) return true;

@@ -27,11 +27,16 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
-// of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds).
+// of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
+// fixed, z_quad, xcd, z_mirror).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
-    int z_lds = 0;  // 1: LDS z kernel even when the register path applies
+    int z_lds = 0;   // 1: LDS z kernel even when the register path applies
+    int fixed = 1;   // 0: runtime-planned line kernels even for fixed schedules
+    int z_quad = 0;  // 1: four lanes per z column instead of two (even nz)
+    int xcd = 1;     // 0: no per-XCD renumbering of workgroups
+    int z_mirror = 1;  // 0: two-lane z kernel without mirrored-column pairing
 };
 
 struct DemagPlan {

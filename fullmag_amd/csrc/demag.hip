@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 namespace fm {
 namespace {
@@ -40,6 +41,14 @@ constexpr int kMaxBlock = 512;  // lanes per workgroup for FFT kernels
 template <typename Real>
 constexpr int z_reg_max() {
     return 16;
+}
+
+// Workgroup b runs on XCD b mod 8 (round-robin dispatch over the 8 XCDs of
+// an MI355X).  Renumbered so that each XCD receives one contiguous range of
+// logical workgroups.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t x = b & 7u, q = g >> 3, r = g & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
 }
 
 // ---------------------------------------------------------------------------
@@ -55,7 +64,7 @@ struct KernelView {
 
 // t = (xx, yy, zz, xy, xz, yz) at frequency (kx, ky, kz); kx <= nx.
 template <typename Real>
-__device__ inline void tensor_at(const KernelView<Real> &K, int kx, int ky, int kz, Real t[6]) {
+__device__ __forceinline__ void tensor_at(const KernelView<Real> &K, int kx, int ky, int kz, Real t[6]) {
     if (K.k) {
         const bool ylo = ky <= K.ny;
         const int oy = ylo ? ky : K.py - ky;
@@ -98,7 +107,7 @@ __device__ inline void tensor_at(const KernelView<Real> &K, int kx, int ky, int 
 }
 
 template <typename Real>
-__device__ inline void apply_tensor(const Real t[6], cplx<Real> &mx, cplx<Real> &my, cplx<Real> &mz) {
+__device__ __forceinline__ void apply_tensor(const Real t[6], cplx<Real> &mx, cplx<Real> &my, cplx<Real> &mz) {
     const cplx<Real> X = mx, Y = my, Z = mz;
     mx = {-(t[0] * X.x + t[3] * Y.x + t[4] * Z.x), -(t[0] * X.y + t[3] * Y.y + t[4] * Z.y)};
     my = {-(t[3] * X.x + t[1] * Y.x + t[5] * Z.x), -(t[3] * X.y + t[1] * Y.y + t[5] * Z.y)};
@@ -188,34 +197,41 @@ struct FieldSink {
 // (c = blockIdx.z, z = blockIdx.y).  Rows 2j, 2j+1 are packed as a + i b;
 // with Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k}) / 2,
 // B_k = (Z_k - conj Z_{N-k}) / (2i), k = 0..N/2, stored at spec[c][z][k][y].
-template <typename Real, class Src>
+template <typename Real, class Src, int N = 0>
 __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
+    using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int LINES = a.xr >> 1;
-    const int n = a.lp.n, ls = a.ls, T = a.lp.threads;
+    const int n = L::n(a.lp), ls = a.ls, T = L::threads(a.lp);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
-    for (int idx = tid; idx < LINES * n; idx += nthr) {
-        const int j = idx / n, x = idx - j * n;
-        const int ya = y0 + 2 * j;
-        cplx<Real> v = {Real(0), Real(0)};
-        if (x < a.n_in) {
-            if (ya < a.rows) v.x = src(c, z, ya, x);
-            if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
+    // the T lanes of packed line j (= tid / T, blockDim.x = LINES * T) load it
+    const int jl = tid / T, t = tid - jl * T;
+    cplx<Real> *line = lds + jl * ls;
+    const int lx = L::line_xor(jl);
+    {
+        const int ya = y0 + 2 * jl;
+        for (int x = t; x < n; x += T) {
+            cplx<Real> v = {Real(0), Real(0)};
+            if (x < a.n_in) {
+                if (ya < a.rows) v.x = src(c, z, ya, x);
+                if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
+            }
+            line[L::slot(a.lp, x, lx)] = v;
         }
-        lds[j * ls + lslot(a.lp, x)] = v;
     }
     __syncthreads();
-    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
+    (void)LINES;
     const int hx = a.g.hx;
     for (int idx = tid; idx < hx * a.xr; idx += nthr) {
         const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
         const int y = y0 + yl;
         if (y >= a.rows) continue;
-        const int j = yl >> 1;
-        const cplx<Real> zk = lds[j * ls + lslot(a.lp, k)];
-        const cplx<Real> zn = lds[j * ls + lslot(a.lp, k == 0 ? 0 : n - k)];
+        const int j = yl >> 1, jx = L::line_xor(j);
+        const cplx<Real> zk = lds[j * ls + L::slot(a.lp, k, jx)];
+        const cplx<Real> zn = lds[j * ls + L::slot(a.lp, k == 0 ? 0 : n - k, jx)];
         const Real h = Real(0.5);
         cplx<Real> v;
         if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
@@ -227,32 +243,37 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
 // Inverse complex-to-real along x: Z_k = A_k + i B_k for k <= N/2 and the
 // Hermitian continuation A_{N-k} = conj A_k; z = IFFT(Z); rows a = Re z,
 // b = Im z.  Reads spec[c][z][k][y0..y0+xr-1].
-template <typename Real, class Sink>
+template <typename Real, class Sink, int N = 0>
 __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
+    using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int LINES = a.xr >> 1;
-    const int n = a.lp.n, ls = a.ls, T = a.lp.threads;
+    const int n = L::n(a.lp), ls = a.ls, T = L::threads(a.lp);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
     const int hx = a.g.hx;
+    // consecutive lanes read consecutive rows of one kx column
     for (int idx = tid; idx < hx * LINES; idx += nthr) {
-        const int j = idx % LINES, k = idx / LINES;
-        const int ya = y0 + 2 * j;
+        const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+        const int ya = y0 + 2 * j, jx = L::line_xor(j);
         cplx<Real> A = {Real(0), Real(0)}, B = {Real(0), Real(0)};
         if (ya < a.rows) A = spec[a.g.at(c, z, k, ya)];
         if (ya + 1 < a.rows) B = spec[a.g.at(c, z, k, ya + 1)];
-        lds[j * ls + lslot(a.lp, k)] = {A.x - B.y, A.y + B.x};
-        if (k > 0 && k < n - k) lds[j * ls + lslot(a.lp, n - k)] = {A.x + B.y, B.x - A.y};
+        lds[j * ls + L::slot(a.lp, k, jx)] = {A.x - B.y, A.y + B.x};
+        if (k > 0 && k < n - k) lds[j * ls + L::slot(a.lp, n - k, jx)] = {A.x + B.y, B.x - A.y};
     }
     __syncthreads();
-    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
-    for (int idx = tid; idx < a.xr * a.n_out; idx += nthr) {
-        const int yl = idx / a.n_out, x = idx - yl * a.n_out;
-        const int y = y0 + yl;
-        if (y >= a.rows) continue;
-        const cplx<Real> v = lds[(yl >> 1) * ls + lslot(a.lp, x)];
-        sink(c, z, y, x, ((yl & 1) ? v.y : v.x) * a.scale);
+    const int jl = tid / T, t = tid - jl * T;
+    const cplx<Real> *line = lds + jl * ls;
+    const int lx = L::line_xor(jl);
+    L::template fft<Real>(lds + jl * ls, a.lp, t, a.tw, true, lx);
+    // the lanes of line jl store its rows 2 jl (real part) and 2 jl + 1
+    const int ya = y0 + 2 * jl;
+    for (int x = t; x < a.n_out; x += T) {
+        const cplx<Real> v = line[L::slot(a.lp, x, lx)];
+        if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
+        if (ya + 1 < a.rows) sink(c, z, ya + 1, x, v.y * a.scale);
     }
 }
 
@@ -271,27 +292,27 @@ struct LineArgs {
     int inverse;
 };
 
-template <typename Real>
+template <typename Real, int N = 0>
 __global__ void __launch_bounds__(kMaxBlock) line_kernel(LineArgs<Real> a) {
+    using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int64_t l0 = (int64_t)blockIdx.x * a.lpw;
-    for (int idx = tid; idx < a.lpw * n; idx += nthr) {
-        const int j = idx / n, e = idx - j * n;
-        const int64_t l = l0 + j;
+    const int n = L::n(a.lp), ls = n + kPad, T = L::threads(a.lp);
+    // line j = tid / T of the workgroup belongs to its T lanes (blockDim.x = lpw T)
+    const int j = threadIdx.x / T, t = threadIdx.x - j * T;
+    const int64_t l = (int64_t)blockIdx.x * a.lpw + j;
+    cplx<Real> *line = lds + j * ls;
+    cplx<Real> *g = a.data + l * a.line_pitch;
+    const bool live = l < a.lines;
+    for (int e = t; e < n; e += T) {
         cplx<Real> v = {Real(0), Real(0)};
-        if (e < a.n_in && l < a.lines) v = a.data[l * a.line_pitch + e];
-        lds[j * ls + lslot(a.lp, e)] = v;
+        if (e < a.n_in && live) v = g[e];
+        line[L::slot(a.lp, e)] = v;
     }
     __syncthreads();
-    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, a.inverse != 0);
-    for (int idx = tid; idx < a.lpw * a.n_out; idx += nthr) {
-        const int j = idx / a.n_out, e = idx - j * a.n_out;
-        const int64_t l = l0 + j;
-        if (l < a.lines) a.data[l * a.line_pitch + e] = lds[j * ls + lslot(a.lp, e)];
-    }
+    L::template fft<Real>(line, a.lp, t, a.tw, a.inverse != 0);
+    if (live)
+        for (int e = t; e < a.n_out; e += T) g[e] = line[L::slot(a.lp, e)];
 }
 
 // Thin film: per kx, the three components' ky lines (ny of 2ny inputs) are
@@ -353,13 +374,39 @@ struct ZArgs {
     int64_t columns;       // hx * py
     int64_t zstride;       // hx * py (3-D demag layout)
     int64_t comp_stride;   // nz * hx * py
+    int xcd;               // renumber workgroups per XCD
+    int half, groups;      // py / 2; groups of 16 direct ky (0..half) per kx
     KernelView<Real> K;
 };
 
-// DFT of length N on registers with roots w_N^t = w_P^(2t) taken from the
-// length-P = 2N table (power-of-two N uses exact constants).
+// DFT of length N on registers with roots w_N^t = w_P^(S t) taken from the
+// length-P = S N table (power-of-two N uses exact constants).
+template <typename Real, int N, int S>
+__device__ __forceinline__ void strided_dft(cplx<Real> *v, const cplx<Real> *tw_p, bool inv) {
+    if constexpr (N == 1) {
+        (void)v; (void)tw_p; (void)inv;
+    } else if constexpr (N == 2 || N == 4 || N == 8 || N == 16) {
+        dft_any<Real, N>(v, tw_p, N, inv);
+    } else {
+        cplx<Real> out[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            cplx<Real> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < N; ++r) {
+                cplx<Real> w = tw_p[S * ((r * q) % N)];
+                if (inv) w.y = -w.y;
+                acc = cadd(acc, cmul(v[r], w));
+            }
+            out[q] = acc;
+        }
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = out[q];
+    }
+}
+
 template <typename Real, int N>
-__device__ inline void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, bool inv) {
+__device__ __forceinline__ void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, bool inv) {
     if constexpr (N == 1) {
         (void)v; (void)tw_p; (void)inv;
     } else if constexpr (N == 2 || N == 4 || N == 8 || N == 16) {
@@ -387,14 +434,33 @@ __device__ inline void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, bool inv)
 // NZ non-zero inputs exist); tensor at its own kz; inverse:
 // y_z = IDFT_NZ(Y_even)[z] + w_pz^(-z) IDFT_NZ(Y_odd)[z], z < NZ, summed
 // across the lane pair with a shuffle.  All three components in registers.
+// Lane layout: wave w of kx row w / groups holds the 16 direct columns
+// ky = 16 g + i (g = w mod groups, ky <= py/2) and their mirrors py - ky, so
+// a direct column and its mirror read the same tensor octant entries in one
+// instruction; lane = 4 i + 2 mirror + p.
 template <typename Real, int NZ>
 __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t col = gid >> 1;
     const int p = (int)(gid & 1);
-    const bool valid = col < a.columns;  // both lanes of a pair stay for the shuffle
-    const int kx = valid ? (int)(col / a.py) : 0;
-    const int ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+    bool valid;  // both lanes of a pair stay for the shuffle
+    int kx, ky;
+    int64_t col;
+    if (a.groups > 0) {
+        const int64_t wave = gid >> 6;
+        const int lane = (int)(threadIdx.x & 63);
+        const int mir = (lane >> 1) & 1, i = lane >> 2;
+        kx = (int)(wave / a.groups);
+        const int kd = 16 * (int)(wave - (int64_t)kx * a.groups) + i;
+        valid = kx < a.hx && kd <= a.half && (!mir || (kd >= 1 && kd < a.half));
+        ky = valid ? (mir ? a.py - kd : kd) : 0;
+        col = valid ? (int64_t)kx * a.py + ky : 0;
+    } else {
+        // consecutive lane pairs take consecutive columns
+        col = gid >> 1;
+        valid = col < a.columns;
+        kx = valid ? (int)(col / a.py) : 0;
+        ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+    }
     cplx<Real> v[3][NZ];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -430,6 +496,68 @@ __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
 #pragma unroll
         for (int z = 0; z < NZ; ++z)
             if ((z & 1) == p) a.data[c * a.comp_stride + z * a.zstride + col] = v[c][z];
+}
+
+// z convolution with four lanes per (kx, ky) column (pz = 2 NZ = 4 M, NZ
+// even): lane q owns kz = 4k + q.  Forward: X[4k+q] = DFT_M(u)[k] with the
+// folded input u_z = x_z w^(q z) + x_(z+M) w^(q (z+M)), z < M; tensor at its
+// own kz; inverse: y_z = sum_q w^(-q z) IDFT_M(Y_q)[z mod M], summed over the
+// quad with two shuffles; lane q stores the z = q (mod 4).  Workgroups are
+// renumbered per XCD so the columns ky and py - ky of one kx row, which read
+// the same tensor octant entries, share an L2.
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_quad_kernel(ZArgs<Real> a) {
+    static_assert(NZ % 2 == 0, "four lanes per column need an even nz");
+    constexpr int M = NZ / 2, PZ = 2 * NZ;
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int64_t gid = (int64_t)blk * blockDim.x + threadIdx.x;
+    const int64_t col = gid >> 2;
+    const int q = (int)(gid & 3);
+    const bool valid = col < a.columns;  // all four lanes of a quad stay for the shuffles
+    const int kx = valid ? (int)(col / a.py) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+    // w^(q z), z < NZ: one gather per z, reused by all components and the inverse
+    cplx<Real> wq[NZ];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) wq[z] = a.tw[(q * z) % PZ];
+    cplx<Real> v[3][M];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int z = 0; z < M; ++z) {
+            cplx<Real> lo = {Real(0), Real(0)}, hi = {Real(0), Real(0)};
+            if (valid) {
+                lo = a.data[c * a.comp_stride + z * a.zstride + col];
+                hi = a.data[c * a.comp_stride + (z + M) * a.zstride + col];
+            }
+            v[c][z] = cadd(cmul(lo, wq[z]), cmul(hi, wq[z + M]));
+        }
+        strided_dft<Real, M, 4>(v[c], a.tw, false);
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        Real t[6];
+        tensor_at(a.K, kx, ky, 4 * k + q, t);
+        apply_tensor(t, v[0][k], v[1][k], v[2][k]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        strided_dft<Real, M, 4>(v[c], a.tw, true);
+        cplx<Real> y[NZ];
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            cplx<Real> s = cmul(v[c][z % M], cconj(wq[z]));
+            s.x += __shfl_xor(s.x, 1);
+            s.y += __shfl_xor(s.y, 1);
+            s.x += __shfl_xor(s.x, 2);
+            s.y += __shfl_xor(s.y, 2);
+            y[z] = s;
+        }
+        if (valid)
+#pragma unroll
+            for (int z = 0; z < NZ; ++z)
+                if ((z & 3) == q) a.data[c * a.comp_stride + z * a.zstride + col] = y[z];
+    }
 }
 
 // Long z lines: a workgroup holds `tile` consecutive columns (contiguous in
@@ -581,6 +709,27 @@ struct Timed {
     }
 };
 
+// Calls f(std::integral_constant<int, N>) with N the line length when `lp`
+// has a compile-time schedule (power of two, 32..4096, default radices),
+// else with N = 0 (runtime plan).
+template <class F>
+void with_line(const LinePlan &lp, bool allow_fixed, F &&f) {
+    if (allow_fixed && p2_matches(lp, lp.n)) {
+        switch (lp.n) {
+            case 32: f(std::integral_constant<int, 32>{}); return;
+            case 64: f(std::integral_constant<int, 64>{}); return;
+            case 128: f(std::integral_constant<int, 128>{}); return;
+            case 256: f(std::integral_constant<int, 256>{}); return;
+            case 512: f(std::integral_constant<int, 512>{}); return;
+            case 1024: f(std::integral_constant<int, 1024>{}); return;
+            case 2048: f(std::integral_constant<int, 2048>{}); return;
+            case 4096: f(std::integral_constant<int, 4096>{}); return;
+            default: break;
+        }
+    }
+    f(std::integral_constant<int, 0>{});
+}
+
 template <typename Real, typename S>
 void demag_eval_t(Context &c, const S *src, Real *out) {
     DemagPlan &dp = c.dp;
@@ -604,7 +753,10 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
     {
         MagSource<Real, S> ms{src, c.d_mask, c.d_ms_cell, c.ms, dp.nx, dp.ny, dp.nz, (int64_t)c.n};
         Timed t(c, "demag_x_r2c");
-        hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>>), xgrid, xblock, xlds, c.stream, xa, ms, spec);
+        with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
+            hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, decltype(NC)::value>), xgrid, xblock, xlds,
+                               c.stream, xa, ms, spec);
+        });
     }
     const KernelView<Real> K = kview<Real>(c);
     if (dp.three_d) {
@@ -622,7 +774,9 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
             la.n_out = dp.py;
             la.inverse = 0;
             Timed t(c, "demag_y_fwd");
-            hipLaunchKernelGGL((line_kernel<Real>), ygrid, yblock, ylds, c.stream, la);
+            with_line(dp.ly, dp.tune.fixed != 0, [&](auto NC) {
+                hipLaunchKernelGGL((line_kernel<Real, decltype(NC)::value>), ygrid, yblock, ylds, c.stream, la);
+            });
         }
         {
             Timed t(c, "demag_z_conv");
@@ -637,14 +791,27 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
                 za.zstride = (int64_t)dp.hx * dp.py;
                 za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
                 za.K = K;
-                const dim3 zg((unsigned)cdiv(2 * za.columns, 256)), zb(256);
-                switch (dp.nz) {
+                za.xcd = dp.tune.xcd;
+                za.half = dp.py / 2;
+                za.groups = dp.tune.z_mirror ? (int)cdiv(za.half + 1, 16) : 0;
+                const dim3 zg((unsigned)(za.groups ? cdiv((uint64_t)dp.hx * za.groups * 64, 256)
+                                                   : cdiv(2 * za.columns, 256))),
+                    zb(256);
+                const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
+                const bool quad = dp.tune.z_quad && (dp.nz == 4 || dp.nz == 6 || dp.nz == 8);
+                if (quad) {
+                    switch (dp.nz) {
+                        case 4: hipLaunchKernelGGL((zconv_quad_kernel<Real, 4>), zq, zb, 0, c.stream, za); break;
+                        case 6: hipLaunchKernelGGL((zconv_quad_kernel<Real, 6>), zq, zb, 0, c.stream, za); break;
+                        default: hipLaunchKernelGGL((zconv_quad_kernel<Real, 8>), zq, zb, 0, c.stream, za); break;
+                    }
+                } else switch (dp.nz) {
+                    case 4: hipLaunchKernelGGL((zconv_pair_kernel<Real, 4>), zg, zb, 0, c.stream, za); break;
+                    case 6: hipLaunchKernelGGL((zconv_pair_kernel<Real, 6>), zg, zb, 0, c.stream, za); break;
                     case 1: hipLaunchKernelGGL((zconv_pair_kernel<Real, 1>), zg, zb, 0, c.stream, za); break;
                     case 2: hipLaunchKernelGGL((zconv_pair_kernel<Real, 2>), zg, zb, 0, c.stream, za); break;
                     case 3: hipLaunchKernelGGL((zconv_pair_kernel<Real, 3>), zg, zb, 0, c.stream, za); break;
-                    case 4: hipLaunchKernelGGL((zconv_pair_kernel<Real, 4>), zg, zb, 0, c.stream, za); break;
                     case 5: hipLaunchKernelGGL((zconv_pair_kernel<Real, 5>), zg, zb, 0, c.stream, za); break;
-                    case 6: hipLaunchKernelGGL((zconv_pair_kernel<Real, 6>), zg, zb, 0, c.stream, za); break;
                     case 7: hipLaunchKernelGGL((zconv_pair_kernel<Real, 7>), zg, zb, 0, c.stream, za); break;
                     default: hipLaunchKernelGGL((zconv_pair_kernel<Real, 8>), zg, zb, 0, c.stream, za); break;
                 }
@@ -672,7 +839,9 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
             la.n_out = dp.ny;
             la.inverse = 1;
             Timed t(c, "demag_y_inv");
-            hipLaunchKernelGGL((line_kernel<Real>), ygrid, yblock, ylds, c.stream, la);
+            with_line(dp.ly, dp.tune.fixed != 0, [&](auto NC) {
+                hipLaunchKernelGGL((line_kernel<Real, decltype(NC)::value>), ygrid, yblock, ylds, c.stream, la);
+            });
         }
     } else {
         YConvArgs<Real> ya{};
@@ -692,7 +861,10 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
     {
         FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
         Timed t(c, "demag_x_c2r");
-        hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>>), xgrid, xblock, xlds, c.stream, xa, spec, sink);
+        with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
+            hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, decltype(NC)::value>), xgrid, xblock, xlds,
+                               c.stream, xa, spec, sink);
+        });
     }
 }
 
@@ -737,6 +909,10 @@ void parse_tuning(DemagTuning &t) {
             else if (k == "y_lanes") t.y_lanes = v;
             else if (k == "y_lpw") t.y_lpw = v;
             else if (k == "z_lds") t.z_lds = v;
+            else if (k == "fixed") t.fixed = v;
+            else if (k == "z_quad") t.z_quad = v;
+            else if (k == "xcd") t.xcd = v;
+            else if (k == "z_mirror") t.z_mirror = v;
         }
         pos = end + 1;
     }

@@ -35,7 +35,7 @@ struct LinePlan {
 // 16b + c lives at 16b + (c ^ (b & 15)): the stride-16 writes of the first
 // Stockham pass then spread over 32 banks instead of 2.  Bijective within
 // each 16-element block.
-__host__ __device__ inline int lslot(const LinePlan &lp, int i) { return lp.swz ? (i ^ ((i >> 4) & 15)) : i; }
+__host__ __device__ __forceinline__ int lslot(const LinePlan &lp, int i) { return lp.swz ? (i ^ ((i >> 4) & 15)) : i; }
 
 // Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13} and the lanes per line:
 // the fewest lanes such that no lane holds more than 16 complex values in any
@@ -58,7 +58,7 @@ std::vector<cplx<double>> twiddle_table(int n);
 #define FM_H 0.70710678118654752440
 
 template <typename T>
-__device__ inline cplx<T> w16(int m, bool inv) {
+__device__ __forceinline__ cplx<T> w16(int m, bool inv) {
     // returns exp(-+ 2 pi i m / 16) (forward: minus sign)
     const T c[16] = {T(1), T(FM_C1), T(FM_H), T(FM_S1), T(0), T(-FM_S1), T(-FM_H), T(-FM_C1),
                      T(-1), T(-FM_C1), T(-FM_H), T(-FM_S1), T(0), T(FM_S1), T(FM_H), T(FM_C1)};
@@ -69,7 +69,7 @@ __device__ inline cplx<T> w16(int m, bool inv) {
 }
 
 template <typename T>
-__device__ inline void dft2(cplx<T> &a, cplx<T> &b) {
+__device__ __forceinline__ void dft2(cplx<T> &a, cplx<T> &b) {
     cplx<T> t = a;
     a = cadd(t, b);
     b = csub(t, b);
@@ -77,7 +77,7 @@ __device__ inline void dft2(cplx<T> &a, cplx<T> &b) {
 
 // In-place 4-point DFT of (v[0], v[s], v[2s], v[3s]).
 template <typename T>
-__device__ inline void dft4(cplx<T> &a0, cplx<T> &a1, cplx<T> &a2, cplx<T> &a3, bool inv) {
+__device__ __forceinline__ void dft4(cplx<T> &a0, cplx<T> &a1, cplx<T> &a2, cplx<T> &a3, bool inv) {
     cplx<T> s0 = cadd(a0, a2), d0 = csub(a0, a2);
     cplx<T> s1 = cadd(a1, a3), d1 = csub(a1, a3);
     // forward: -i * d1 ; inverse: +i * d1
@@ -91,7 +91,7 @@ __device__ inline void dft4(cplx<T> &a0, cplx<T> &a1, cplx<T> &a2, cplx<T> &a3, 
 // 8-point DFT: inner 4-point DFTs over even/odd samples, twiddle w8^(r2 q1),
 // outer 2-point DFTs: X[q1 + 4 q2].
 template <typename T>
-__device__ inline void dft8(cplx<T> *v, bool inv) {
+__device__ __forceinline__ void dft8(cplx<T> *v, bool inv) {
     cplx<T> u0[4] = {v[0], v[2], v[4], v[6]};
     cplx<T> u1[4] = {v[1], v[3], v[5], v[7]};
     dft4(u0[0], u0[1], u0[2], u0[3], inv);
@@ -110,7 +110,7 @@ __device__ inline void dft8(cplx<T> *v, bool inv) {
 // 16-point DFT as 4 x 4: inner DFTs over r1 (stride 4), twiddle w16^(r2 q1),
 // outer DFTs over r2: X[q1 + 4 q2].
 template <typename T>
-__device__ inline void dft16(cplx<T> *v, bool inv) {
+__device__ __forceinline__ void dft16(cplx<T> *v, bool inv) {
     cplx<T> u[4][4];
 #pragma unroll
     for (int r2 = 0; r2 < 4; ++r2) {
@@ -132,7 +132,7 @@ __device__ inline void dft16(cplx<T> *v, bool inv) {
 
 // Odd prime R: direct DFT with roots from the line's twiddle table.
 template <typename T, int R>
-__device__ inline void dft_prime(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
+__device__ __forceinline__ void dft_prime(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
     cplx<T> out[R];
     const int step = n / R;
 #pragma unroll
@@ -151,7 +151,7 @@ __device__ inline void dft_prime(cplx<T> *v, const cplx<T> *tw, int n, bool inv)
 }
 
 template <typename T, int R>
-__device__ inline void dft_any(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
+__device__ __forceinline__ void dft_any(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
     if constexpr (R == 2) {
         dft2(v[0], v[1]);
     } else if constexpr (R == 4) {
@@ -170,7 +170,7 @@ __device__ inline void dft_any(cplx<T> *v, const cplx<T> *tw, int n, bool inv) {
 // other power is one product of an earlier power and a table root, so no
 // twiddle carries more than three extra roundings.
 template <typename T, int R>
-__device__ inline void twiddle(cplx<T> *v, const cplx<T> *tw, int st, bool inv) {
+__device__ __forceinline__ void twiddle(cplx<T> *v, const cplx<T> *tw, int st, bool inv) {
     if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
         constexpr int LB = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
         cplx<T> pw[LB];
@@ -209,7 +209,7 @@ constexpr int lane_butterflies() {
 // t + nt, ... (at most lane_butterflies<R>()); the planner guarantees that
 // covers all n/R of them.
 template <typename T, int R>
-__device__ inline void stockham_pass(cplx<T> *line, const LinePlan &lp, int ns, int t, const cplx<T> *tw, bool inv) {
+__device__ __forceinline__ void stockham_pass(cplx<T> *line, const LinePlan &lp, int ns, int t, const cplx<T> *tw, bool inv) {
     constexpr int MB = lane_butterflies<R>();
     const int n = lp.n, nt = lp.threads;
     const int nr = n / R;
@@ -241,7 +241,7 @@ __device__ inline void stockham_pass(cplx<T> *line, const LinePlan &lp, int ns, 
 
 // Full transform of one line.  Uniform across the workgroup.
 template <typename T>
-__device__ inline void line_fft(cplx<T> *line, const LinePlan &lp, int t, const cplx<T> *tw, bool inv) {
+__device__ __forceinline__ void line_fft(cplx<T> *line, const LinePlan &lp, int t, const cplx<T> *tw, bool inv) {
     int ns = 1;
     for (int p = 0; p < lp.npass; ++p) {
         const int R = lp.radix[p];
@@ -261,11 +261,106 @@ __device__ inline void line_fft(cplx<T> *line, const LinePlan &lp, int t, const 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time schedules for power-of-two lines, 16 <= n <= 4096
+// ---------------------------------------------------------------------------
+// The same radices plan_line() picks for these lengths with default knobs
+// (radix-16 passes, then an 8/4 tail; 2^(4a+1) uses 16^(a-1) 8 4), so a line
+// transformed by either path carries the same roundings.  With n, the radix
+// and the span known at compile time every index is a shift or mask, the
+// swizzle is fixed and each kernel only holds one pass's registers.
+constexpr int clog2(int n) { return n <= 1 ? 0 : 1 + clog2(n / 2); }
+
+constexpr int p2_npass(int n) {
+    return (clog2(n) % 4 == 0) ? clog2(n) / 4 : (clog2(n) % 4 == 1 && clog2(n) < 4) ? 1 : clog2(n) / 4 + 1;
+}
+
+constexpr int p2_radix(int n, int p) {
+    const int a = clog2(n) / 4, r = clog2(n) % 4;
+    if (r == 0) return 16;
+    if (r == 1) return a == 0 ? 2 : p < a - 1 ? 16 : p == a - 1 ? 8 : 4;
+    if (r == 2) return p < a ? 16 : 4;
+    return p < a ? 16 : 8;
+}
+
+// Host check: `lp` is the default plan of a length with a fixed schedule.
+inline bool p2_matches(const LinePlan &lp, int n) {
+    if (lp.n != n || lp.threads != n / 16 || lp.npass != p2_npass(n) || !lp.swz) return false;
+    for (int p = 0; p < lp.npass; ++p)
+        if (lp.radix[p] != p2_radix(n, p)) return false;
+    return true;
+}
+
+__host__ __device__ constexpr int p2slot(int i) { return i ^ ((i >> 4) & 15); }
+
+// `lx` XORs every slot of the line (a per-line bank rotation for workgroups
+// that read several lines at one element index; 0 or a multiple of 4 < 32).
+template <typename T, int N, int R, int NS>
+__device__ __forceinline__ void stockham_pass_p2(cplx<T> *line, int t, const cplx<T> *tw, bool inv, int lx) {
+    constexpr int MB = lane_butterflies<R>();
+    constexpr int NT = N / 16, NR = N / R, ST = N / (NS * R);
+    static_assert(MB * NT == NR, "every butterfly has a lane");
+    cplx<T> v[MB][R];
+    int base[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const int j = t + b * NT;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = line[p2slot(j + r * NR) ^ lx];
+        if (NS > 1 && k != 0) twiddle<T, R>(v[b], tw, k * ST, inv);
+        dft_any<T, R>(v[b], tw, N, inv);
+        base[b] = (j / NS) * NS * R + k;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int r = 0; r < R; ++r) line[p2slot(base[b] + r * NS) ^ lx] = v[b][r];
+    __syncthreads();
+}
+
+template <typename T, int N, int P, int NS>
+__device__ __forceinline__ void p2_passes(cplx<T> *line, int t, const cplx<T> *tw, bool inv, int lx) {
+    if constexpr (P < p2_npass(N)) {
+        constexpr int R = p2_radix(N, P);
+        stockham_pass_p2<T, N, R, NS>(line, t, tw, inv, lx);
+        p2_passes<T, N, P + 1, NS * R>(line, t, tw, inv, lx);
+    }
+}
+
+// Line access policy: N > 0 a fixed power-of-two schedule (N / 16 lanes per
+// line), N == 0 the runtime plan.
+template <int N>
+struct LineOps {
+    __device__ __forceinline__ static int n(const LinePlan &lp) { return N ? N : lp.n; }
+    __device__ __forceinline__ static int threads(const LinePlan &lp) { return N ? N / 16 : lp.threads; }
+    __device__ __forceinline__ static int slot(const LinePlan &lp, int i) { return N ? p2slot(i) : lslot(lp, i); }
+    // Bank rotation of line j of a multi-line workgroup: fixed schedules of
+    // n >= 32 XOR the slots with (j mod 8) * 4, so eight lines read at one
+    // element index hit eight different bank groups without padding.
+    __device__ __forceinline__ static int line_xor(int j) { return N >= 32 ? ((j & 7) << 2) : 0; }
+    __device__ __forceinline__ static int slot(const LinePlan &lp, int i, int lx) {
+        return N ? (p2slot(i) ^ lx) : lslot(lp, i);
+    }
+    template <typename T>
+    __device__ __forceinline__ static void fft(cplx<T> *line, const LinePlan &lp, int t, const cplx<T> *tw, bool inv,
+                                               int lx = 0) {
+        if constexpr (N > 0) {
+            (void)lp;
+            p2_passes<T, N, 0, 1>(line, t, tw, inv, lx);
+        } else {
+            (void)lx;
+            line_fft(line, lp, t, tw, inv);
+        }
+    }
+};
+
 // Register-resident DFT of a short line (all n values in one lane), used for
 // the z transforms of thin films.  Direct O(n^2) sum with table roots for n
 // not a power of two <= 16.
 template <typename T, int N>
-__device__ inline void reg_dft(cplx<T> *v, const cplx<T> *tw, bool inv) {
+__device__ __forceinline__ void reg_dft(cplx<T> *v, const cplx<T> *tw, bool inv) {
     if constexpr (N == 1) {
         (void)v; (void)tw; (void)inv;
     } else if constexpr (N == 2 || N == 4 || N == 8 || N == 16) {
