@@ -17,4 +17,6 @@ from .native_fdm import (  # noqa: F401
     is_available,
 )
 
+from .adjoint import heun_rollout  # noqa: F401,E402
+
 MU0 = 4.0 * 3.141592653589793 * 1e-7

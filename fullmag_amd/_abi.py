@@ -167,6 +167,10 @@ SIGNATURES = {
                                           ctypes.POINTER(_u64)]),
     "fullmag_fdm_ext_random_unit_vectors": (None, [_u64, _u64, _pd]),
     "fullmag_fdm_ext_device_bytes": (_u64, [ctypes.c_void_p]),
+    "fullmag_fdm_ext_heun_adjoint": (_i32, [ctypes.c_void_p, _d, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, _pd]),
+    "fullmag_fdm_ext_copy_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
+    "fullmag_fdm_ext_set_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
 }
 
 _lib = None

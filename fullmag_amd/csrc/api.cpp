@@ -650,6 +650,55 @@ int fullmag_fdm_ext_set_external_field(fullmag_fdm_backend *handle, const double
     c.has_ext = true;
     for (int k = 0; k < 3; ++k) c.hext[k] = h_am[k];
     c.obs_valid = false;
+    // stored slopes were evaluated with the old field
+    c.fsal_valid = false;
+    c.abm_count = 0;
+    c.abm_startup = 0;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_ext_heun_adjoint(fullmag_fdm_backend *handle, double dt_seconds, const double *m0_dev,
+                                 const double *lam1_dev, double *lam0_dev, double *grad_ms_dev, double *grad_hext) {
+    if (!handle) return FULLMAG_FDM_ERR_INVALID;
+    Context &c = *as_ctx(handle);
+    if (!c.ready || !(dt_seconds > 0.0) || !std::isfinite(dt_seconds) || !m0_dev || !lam1_dev || !lam0_dev) {
+        c.last_error = "heun_adjoint needs an initialised backend, dt > 0 and device arrays m0, lam1, lam0";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    return heun_adjoint(c, dt_seconds, m0_dev, lam1_dev, lam0_dev, grad_ms_dev, grad_hext);
+}
+
+int fullmag_fdm_ext_copy_state_device(fullmag_fdm_backend *handle, double *dst_dev, uint64_t len) {
+    if (!handle) return FULLMAG_FDM_ERR_INVALID;
+    Context &c = *as_ctx(handle);
+    if (!c.ready || !dst_dev || len != 3 * c.n || c.state_bytes() != 8) {
+        c.last_error = "copy_state_device needs an FP64 state (double or mixed) and len = 3 * cells";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    if (!hip_ok(hipMemcpyAsync(dst_dev, c.m, len * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+                "copy_state_device", c.last_error) ||
+        !hip_ok(hipStreamSynchronize(c.stream), "copy_state_device", c.last_error))
+        return FULLMAG_FDM_ERR_CUDA;
+    return FULLMAG_FDM_OK;
+}
+
+int fullmag_fdm_ext_set_state_device(fullmag_fdm_backend *handle, const double *src_dev, uint64_t len) {
+    if (!handle) return FULLMAG_FDM_ERR_INVALID;
+    Context &c = *as_ctx(handle);
+    if (!c.ready || !src_dev || len != 3 * c.n || c.state_bytes() != 8) {
+        c.last_error = "set_state_device needs an FP64 state (double or mixed) and len = 3 * cells";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    if (!hip_ok(hipMemcpyAsync(c.m, src_dev, len * sizeof(double), hipMemcpyDeviceToDevice, c.stream),
+                "set_state_device", c.last_error) ||
+        !hip_ok(hipStreamSynchronize(c.stream), "set_state_device", c.last_error))
+        return FULLMAG_FDM_ERR_CUDA;
+    c.hd_at_m = false;
+    c.obs_valid = false;
+    c.obs_frozen = false;
+    c.fsal_valid = false;
+    c.abm_count = 0;
+    c.abm_startup = 0;
     return FULLMAG_FDM_OK;
 }
 

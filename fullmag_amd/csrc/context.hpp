@@ -28,7 +28,7 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror).
+// fixed, z_quad, xcd, z_mirror, y_pipe).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
@@ -37,6 +37,7 @@ struct DemagTuning {
     int z_quad = 0;  // 1: four lanes per z column instead of two (even nz)
     int xcd = 1;     // 0: no per-XCD renumbering of workgroups
     int z_mirror = 1;  // 0: two-lane z kernel without mirrored-column pairing
+    int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
 };
 
 struct DemagPlan {
@@ -125,6 +126,12 @@ struct Context {
     double *d_result = nullptr;
     double *h_result = nullptr;  // pinned
 
+    // adjoint work buffers (allocated by the first adjoint call)
+    bool adj_ready = false;
+    void *adj_hd0 = nullptr, *adj_k1 = nullptr, *adj_mt = nullptr, *adj_hdt = nullptr;  // Real
+    void *adj_g = nullptr, *adj_q = nullptr, *adj_w = nullptr;                          // Real
+    double *adj_lu = nullptr, *adj_lmt = nullptr;
+
     // scratch for previews
     void *d_preview = nullptr;
     uint64_t preview_bytes = 0;
@@ -159,8 +166,9 @@ void timing_resolve(Context &c);
 // demag.hip
 bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_device);
 // Evaluates H_demag of the field `src` (3n elements, FP64 when src_f64) into
-// `out` (Real).  Inactive cells get zero.
-void demag_eval(Context &c, const void *src, bool src_f64, void *out);
+// `out` (Real): -N * (Ms src), or the bare convolution -N * src when
+// weight_ms is false.  Inactive cells get zero.
+void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true);
 bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double dz, bool thin_film_2d,
                          double *out, std::string &err);
 
@@ -175,5 +183,12 @@ bool materialize_observables(Context &c);  // fills h_ex_obs / h_eff_obs at m
 bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
                    uint32_t z0, uint32_t zs, void *dev_out, bool out_f64);
 bool alloc_integrator_buffers(Context &c);
+
+// adjoint.hip: discrete adjoint of one Heun step from the state m0 (device,
+// FP64 SoA 3n): lam0 = (dm1/dm0)^T lam1; grad_ms (device, n, may be null)
+// and grad_hext (host, 3, may be null) accumulate the gradients of
+// <lam1, m1> with respect to per-cell Ms and H_ext.  State must be FP64.
+int heun_adjoint(Context &c, double dt, const double *m0, const double *lam1, double *lam0, double *grad_ms,
+                 double *grad_hext);
 
 }  // namespace fm

@@ -290,6 +290,7 @@ struct LineArgs {
     int lpw;            // lines per workgroup
     int n_in, n_out;
     int inverse;
+    int iters;          // line_pipe_kernel: line groups per workgroup
 };
 
 template <typename Real, int N = 0>
@@ -313,6 +314,50 @@ __global__ void __launch_bounds__(kMaxBlock) line_kernel(LineArgs<Real> a) {
     L::template fft<Real>(line, a.lp, t, a.tw, a.inverse != 0);
     if (live)
         for (int e = t; e < a.n_out; e += T) g[e] = line[L::slot(a.lp, e)];
+}
+
+// Software-pipelined contiguous line pass for a fixed power-of-two length N:
+// a workgroup holds G = max(1, 64 / T) lines at a time (T = N / 16 lanes per
+// line) and walks `iters` groups of lines; the global loads of the next group
+// are issued into registers before the current group's LDS transform, so
+// memory traffic overlaps the FFT.  Inputs: N / IN_DIV per line (the rest
+// zero); outputs: a.n_out per line.
+template <typename Real, int N, int IN_DIV>
+__global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
+    constexpr int T = N / 16;
+    constexpr int G = T >= 64 ? 1 : 64 / T;
+    constexpr int NIN = N / IN_DIV;
+    constexpr int NI = NIN / T;
+    static_assert(NIN % T == 0, "whole loads per lane");
+    extern __shared__ unsigned char smem_raw[];
+    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
+    const int j = (int)threadIdx.x / T, t = (int)threadIdx.x - j * T;
+    cplx<Real> *line = lds + j * (N + kPad);
+    const int64_t first = (int64_t)blockIdx.x * a.iters * G + j;
+    cplx<Real> buf[NI];
+    auto fetch = [&](int64_t l) {
+        const bool live = l < a.lines;
+        const cplx<Real> *g = a.data + (live ? l : 0) * a.line_pitch;
+#pragma unroll
+        for (int e = 0; e < NI; ++e) buf[e] = live ? g[t + e * T] : cplx<Real>{Real(0), Real(0)};
+    };
+    fetch(first);
+    for (int it = 0; it < a.iters; ++it) {
+        const int64_t l = first + (int64_t)it * G;
+#pragma unroll
+        for (int e = 0; e < NI; ++e) line[p2slot(t + e * T)] = buf[e];
+        if (IN_DIV > 1)
+#pragma unroll
+            for (int e = NI; e < N / T; ++e) line[p2slot(t + e * T)] = cplx<Real>{Real(0), Real(0)};
+        __syncthreads();
+        if (it + 1 < a.iters) fetch(l + G);
+        p2_passes<Real, N, 0, 1>(line, t, a.tw, a.inverse != 0, 0);
+        if (l < a.lines) {
+            cplx<Real> *g = a.data + l * a.line_pitch;
+            for (int e = t; e < a.n_out; e += T) g[e] = line[p2slot(e)];
+        }
+        __syncthreads();
+    }
 }
 
 // Thin film: per kx, the three components' ky lines (ny of 2ny inputs) are
@@ -730,8 +775,35 @@ void with_line(const LinePlan &lp, bool allow_fixed, F &&f) {
     f(std::integral_constant<int, 0>{});
 }
 
+// One contiguous-line pass (the 3-D y transforms): the pipelined kernel for
+// fixed power-of-two schedules with n_in = N or N / 2, else line_kernel.
+template <typename Real>
+void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) {
+    const DemagPlan &dp = c.dp;
+    with_line(lp, dp.tune.fixed != 0, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        if constexpr (N >= 32) {
+            const int div = la.n_in == N ? 1 : (2 * la.n_in == N ? 2 : 0);
+            if (dp.tune.y_pipe > 0 && div > 0) {
+                constexpr int T = N / 16, G = T >= 64 ? 1 : 64 / T;
+                la.iters = dp.tune.y_pipe;
+                const dim3 grid((unsigned)cdiv(la.lines, (int64_t)G * la.iters)), block((unsigned)(G * T));
+                const size_t lds = (size_t)G * (N + kPad) * ce;
+                if (div == 1)
+                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1>), grid, block, lds, c.stream, la);
+                else
+                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2>), grid, block, lds, c.stream, la);
+                return;
+            }
+        }
+        const dim3 grid((unsigned)cdiv(la.lines, la.lpw)), block((unsigned)(la.lpw * lp.threads));
+        const size_t lds = (size_t)la.lpw * (lp.n + kPad) * ce;
+        hipLaunchKernelGGL((line_kernel<Real, N>), grid, block, lds, c.stream, la);
+    });
+}
+
 template <typename Real, typename S>
-void demag_eval_t(Context &c, const S *src, Real *out) {
+void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     DemagPlan &dp = c.dp;
     cplx<Real> *spec = static_cast<cplx<Real> *>(dp.spec);
     const SpecGeom g{dp.hx, dp.three_d ? dp.nz : 1, dp.rows};
@@ -751,7 +823,8 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
     const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
     const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
     {
-        MagSource<Real, S> ms{src, c.d_mask, c.d_ms_cell, c.ms, dp.nx, dp.ny, dp.nz, (int64_t)c.n};
+        MagSource<Real, S> ms{src, c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
+                              dp.nx, dp.ny, dp.nz, (int64_t)c.n};
         Timed t(c, "demag_x_r2c");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, decltype(NC)::value>), xgrid, xblock, xlds,
@@ -767,16 +840,12 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
         la.lines = 3ll * dp.nz * dp.hx;
         la.line_pitch = dp.py;
         la.lpw = dp.y_lines_per_wg;
-        const dim3 ygrid((unsigned)cdiv(la.lines, la.lpw)), yblock((unsigned)(la.lpw * dp.ly.threads));
-        const size_t ylds = (size_t)la.lpw * (dp.ly.n + kPad) * ce;
         {
             la.n_in = dp.ny;
             la.n_out = dp.py;
             la.inverse = 0;
             Timed t(c, "demag_y_fwd");
-            with_line(dp.ly, dp.tune.fixed != 0, [&](auto NC) {
-                hipLaunchKernelGGL((line_kernel<Real, decltype(NC)::value>), ygrid, yblock, ylds, c.stream, la);
-            });
+            launch_lines<Real>(c, dp.ly, la, ce);
         }
         {
             Timed t(c, "demag_z_conv");
@@ -839,9 +908,7 @@ void demag_eval_t(Context &c, const S *src, Real *out) {
             la.n_out = dp.ny;
             la.inverse = 1;
             Timed t(c, "demag_y_inv");
-            with_line(dp.ly, dp.tune.fixed != 0, [&](auto NC) {
-                hipLaunchKernelGGL((line_kernel<Real, decltype(NC)::value>), ygrid, yblock, ylds, c.stream, la);
-            });
+            launch_lines<Real>(c, dp.ly, la, ce);
         }
     } else {
         YConvArgs<Real> ya{};
@@ -913,6 +980,7 @@ void parse_tuning(DemagTuning &t) {
             else if (k == "z_quad") t.z_quad = v;
             else if (k == "xcd") t.xcd = v;
             else if (k == "z_mirror") t.z_mirror = v;
+            else if (k == "y_pipe") t.y_pipe = v;
         }
         pos = end + 1;
     }
@@ -1168,13 +1236,13 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     return setup_t<float>(c, plan);
 }
 
-void demag_eval(Context &c, const void *src, bool src_f64, void *out) {
+void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms) {
     if (c.mode == MODE_DOUBLE) {
-        demag_eval_t<double, double>(c, static_cast<const double *>(src), static_cast<double *>(out));
+        demag_eval_t<double, double>(c, static_cast<const double *>(src), static_cast<double *>(out), weight_ms);
     } else if (src_f64) {
-        demag_eval_t<float, double>(c, static_cast<const double *>(src), static_cast<float *>(out));
+        demag_eval_t<float, double>(c, static_cast<const double *>(src), static_cast<float *>(out), weight_ms);
     } else {
-        demag_eval_t<float, float>(c, static_cast<const float *>(src), static_cast<float *>(out));
+        demag_eval_t<float, float>(c, static_cast<const float *>(src), static_cast<float *>(out), weight_ms);
     }
 }
 

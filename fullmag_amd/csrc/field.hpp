@@ -1,0 +1,428 @@
+// field.hpp -- per-cell field and torque building blocks shared by the
+// integrator (llg.hip) and the adjoint (adjoint.hip): exchange stencils,
+// torque, compensated sums, workgroup reductions, XCD-aware workgroup
+// numbering, the fused stage kernel and its column tiling, kernel timing.
+// Semantics follow crates/fullmag-engine/src/lib.rs (exchange :2247-2299,
+// rhs :2584-2595).
+#pragma once
+
+#include "context.hpp"
+
+#include <algorithm>
+
+namespace fm {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kNStat = 9;  // s_ex, c_ex, s_d, c_d, s_ext, c_ext, max|Heff|^2, max|Hd|^2, max|k|^2
+
+struct FieldArgs {
+    int nx, ny, nz;
+    int64_t n;
+    FastDiv div_nx, div_nxy;  // cell index -> (x, y, z); n < 2^32
+    // column tiling of the stage kernels: a workgroup owns a TX x TY tile of
+    // (x, y) columns (TX TY = kBlock) and marches z through zc planes
+    int tx_log2, tiles_x, tiles_y, zc, nchunks;
+    int xcd;  // renumber workgroups per XCD
+    double idx2, idy2, idz2;  // 1 / d^2
+    double pref;              // 2 A / (mu0 Ms) for uniform A and Ms
+    double a_uniform;         // A when ms_cell is set and no table
+    double two_over_mu0;
+    double pre_lut;           // 2 / (mu0 Ms) for the per-pair table
+    const uint8_t *mask;
+    const uint32_t *region;  // non-null: per-pair A from `lut`
+    const double *lut;
+    const double *ms_cell;
+    int exchange, has_ext, precession;
+    double hx, hy, hz;
+    double gamma_bar, alpha;
+};
+
+// Exchange field at cell i in the arithmetic type C of the source, with
+// clamped (Neumann) neighbours and inactive neighbours replaced by the centre
+// (lib.rs:2270-2283).
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_at(const FieldArgs &f, const S *__restrict__ m, int64_t i, C h[3]) {
+    const int64_t nxy = (int64_t)f.nx * f.ny;
+    const uint32_t ui = (uint32_t)i;
+    const uint32_t z = f.div_nxy.div(ui);
+    const uint32_t r = ui - z * (uint32_t)nxy;
+    const uint32_t y = f.div_nx.div(r);
+    const uint32_t x = r - y * (uint32_t)f.nx;
+    int64_t xm = x > 0 ? i - 1 : i, xp = (int)x < f.nx - 1 ? i + 1 : i;
+    int64_t ym = y > 0 ? i - f.nx : i, yp = (int)y < f.ny - 1 ? i + f.nx : i;
+    int64_t zm = z > 0 ? i - nxy : i, zp = (int)z < f.nz - 1 ? i + nxy : i;
+    if (f.mask) {
+        if (!f.mask[xm]) xm = i;
+        if (!f.mask[xp]) xp = i;
+        if (!f.mask[ym]) ym = i;
+        if (!f.mask[yp]) yp = i;
+        if (!f.mask[zm]) zm = i;
+        if (!f.mask[zp]) zp = i;
+    }
+    const C ix = C(f.idx2), iy = C(f.idy2), iz = C(f.idz2);
+    if (!f.region) {
+        const C pref = f.ms_cell ? C(f.two_over_mu0 * f.a_uniform / f.ms_cell[i]) : C(f.pref);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(mc[i]);
+            const C lap = (C(mc[xp]) - C(2) * cc + C(mc[xm])) * ix + (C(mc[yp]) - C(2) * cc + C(mc[ym])) * iy +
+                          (C(mc[zp]) - C(2) * cc + C(mc[zm])) * iz;
+            h[c] = pref * lap;
+        }
+    } else {
+        // per-pair coupling (native/backends/fdm/src/exchange_fp64.cu:80-114)
+        const uint32_t ri = f.region[i];
+        const double *row = f.lut + (size_t)ri * FULLMAG_FDM_MAX_EXCHANGE_REGIONS;
+        const C axm = C(row[f.region[xm]]), axp = C(row[f.region[xp]]);
+        const C aym = C(row[f.region[ym]]), ayp = C(row[f.region[yp]]);
+        const C azm = C(row[f.region[zm]]), azp = C(row[f.region[zp]]);
+        const double msi = f.ms_cell ? f.ms_cell[i] : 0.0;
+        const C pre = f.ms_cell ? C(f.two_over_mu0 / msi) : C(f.pre_lut);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(mc[i]);
+            const C s = (axp * (C(mc[xp]) - cc) + axm * (C(mc[xm]) - cc)) * ix +
+                        (ayp * (C(mc[yp]) - cc) + aym * (C(mc[ym]) - cc)) * iy +
+                        (azp * (C(mc[zp]) - cc) + azm * (C(mc[zm]) - cc)) * iz;
+            h[c] = pre * s;
+        }
+    }
+}
+
+// exchange_at for a z-marching thread: the z neighbours' values arrive in
+// registers (vzm, vzp as stored at i -+ nx ny, ignored where the neighbour is
+// clamped or inactive); x and y neighbours are loaded.  Same operations in
+// the same order as exchange_at.
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_col(const FieldArgs &f, const S *__restrict__ m, int64_t i, int x, int y,
+                                             int z, const S cc_s[3], const S vzm[3], const S vzp[3], C h[3]) {
+    const int64_t nxy = (int64_t)f.nx * f.ny;
+    int64_t xm = x > 0 ? i - 1 : i, xp = x < f.nx - 1 ? i + 1 : i;
+    int64_t ym = y > 0 ? i - f.nx : i, yp = y < f.ny - 1 ? i + f.nx : i;
+    bool zm_in = z > 0, zp_in = z < f.nz - 1;
+    if (f.mask) {
+        if (!f.mask[xm]) xm = i;
+        if (!f.mask[xp]) xp = i;
+        if (!f.mask[ym]) ym = i;
+        if (!f.mask[yp]) yp = i;
+        if (zm_in && !f.mask[i - nxy]) zm_in = false;
+        if (zp_in && !f.mask[i + nxy]) zp_in = false;
+    }
+    const C ix = C(f.idx2), iy = C(f.idy2), iz = C(f.idz2);
+    if (!f.region) {
+        const C pref = f.ms_cell ? C(f.two_over_mu0 * f.a_uniform / f.ms_cell[i]) : C(f.pref);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(cc_s[c]);
+            const C zmv = zm_in ? C(vzm[c]) : cc, zpv = zp_in ? C(vzp[c]) : cc;
+            const C lap = (C(mc[xp]) - C(2) * cc + C(mc[xm])) * ix + (C(mc[yp]) - C(2) * cc + C(mc[ym])) * iy +
+                          (zpv - C(2) * cc + zmv) * iz;
+            h[c] = pref * lap;
+        }
+    } else {
+        const int64_t zm = zm_in ? i - nxy : i, zp = zp_in ? i + nxy : i;
+        const uint32_t ri = f.region[i];
+        const double *row = f.lut + (size_t)ri * FULLMAG_FDM_MAX_EXCHANGE_REGIONS;
+        const C axm = C(row[f.region[xm]]), axp = C(row[f.region[xp]]);
+        const C aym = C(row[f.region[ym]]), ayp = C(row[f.region[yp]]);
+        const C azm = C(row[f.region[zm]]), azp = C(row[f.region[zp]]);
+        const double msi = f.ms_cell ? f.ms_cell[i] : 0.0;
+        const C pre = f.ms_cell ? C(f.two_over_mu0 / msi) : C(f.pre_lut);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const S *mc = m + c * f.n;
+            const C cc = C(cc_s[c]);
+            const C zmv = zm_in ? C(vzm[c]) : cc, zpv = zp_in ? C(vzp[c]) : cc;
+            const C s = (axp * (C(mc[xp]) - cc) + axm * (C(mc[xm]) - cc)) * ix +
+                        (ayp * (C(mc[yp]) - cc) + aym * (C(mc[ym]) - cc)) * iy +
+                        (azp * (zpv - cc) + azm * (zmv - cc)) * iz;
+            h[c] = pre * s;
+        }
+    }
+}
+
+// Workgroup b runs on XCD b mod 8 (round-robin dispatch over the 8 XCDs of
+// an MI355X).  Renumbered so that each XCD receives one contiguous range of
+// logical workgroups: neighbouring tiles then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t g) {
+    const uint32_t x = b & 7u, q = g >> 3, r = g & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+template <typename R>
+__device__ __forceinline__ void torque(const FieldArgs &f, const R m[3], const R h[3], R k[3]) {
+    const R p0 = m[1] * h[2] - m[2] * h[1];
+    const R p1 = m[2] * h[0] - m[0] * h[2];
+    const R p2 = m[0] * h[1] - m[1] * h[0];
+    const R d0 = m[1] * p2 - m[2] * p1;
+    const R d1 = m[2] * p0 - m[0] * p2;
+    const R d2 = m[0] * p1 - m[1] * p0;
+    const R a = R(f.alpha), g = R(-f.gamma_bar);
+    const R pe = f.precession ? R(1) : R(0);
+    k[0] = (pe * p0 + d0 * a) * g;
+    k[1] = (pe * p1 + d1 * a) * g;
+    k[2] = (pe * p2 + d2 * a) * g;
+}
+
+__device__ __forceinline__ void neu_add(double &s, double &c, double v) {
+    const double t = s + v;
+    if (fabs(s) >= fabs(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+}
+
+enum { STAGE_PLAIN = 0, STAGE_STATS = 1, STAGE_ERROR = 2 };
+
+// Combines two compensated partial sums (s, c) += (s2, c2).
+__device__ __forceinline__ void neu_merge(double &s, double &c, double s2, double c2) {
+    neu_add(s, c, s2);
+    c += c2;
+}
+
+// Workgroup reduction of the per-lane accumulators: wave shuffles, then the
+// wavefront partials through LDS; lane 0 writes the result to `out`.
+template <int MODE>
+__device__ __forceinline__ void block_reduce(double *acc, double *out) {
+    constexpr int NQ = MODE == STAGE_STATS ? kNStat : 1;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        if (MODE == STAGE_STATS) {
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                const double s2 = __shfl_down(acc[2 * e], off), c2 = __shfl_down(acc[2 * e + 1], off);
+                neu_merge(acc[2 * e], acc[2 * e + 1], s2, c2);
+            }
+#pragma unroll
+            for (int q = 6; q < 9; ++q) acc[q] = fmax(acc[q], __shfl_down(acc[q], off));
+        } else {
+            acc[0] = fmax(acc[0], __shfl_down(acc[0], off));
+        }
+    }
+    __shared__ double red[kBlock / 64][NQ];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int q = 0; q < NQ; ++q) red[wave][q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+            if (MODE == STAGE_STATS) {
+                for (int e = 0; e < 3; ++e) neu_merge(red[0][2 * e], red[0][2 * e + 1], red[w][2 * e], red[w][2 * e + 1]);
+                for (int q = 6; q < 9; ++q) red[0][q] = fmax(red[0][q], red[w][q]);
+            } else {
+                red[0][0] = fmax(red[0][0], red[w][0]);
+            }
+        }
+        for (int q = 0; q < NQ; ++q) out[q] = red[0][q];
+    }
+}
+
+template <typename State, typename Real, typename Src, typename Out>
+struct StageArgs {
+    FieldArgs f;
+    const Src *src;
+    const Real *hd;    // demag field at src, or null
+    const State *m0;   // base point of the update (MODE_PLAIN with out)
+    const Real *kp[6];
+    double ap[6];
+    int nprev;
+    double anew, dt;
+    Real *kout;
+    Out *out;
+    double *partials;
+};
+
+// Per-cell body of the stage kernel; cc, vzm, vzp: src at i and at its z
+// neighbours (see exchange_col).
+template <typename State, typename Real, typename Src, typename Out, int MODE>
+__device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out> &a, int64_t i, int x, int y, int z,
+                                           const Src cc[3], const Src vzm[3], const Src vzp[3], double *acc) {
+    const FieldArgs &f = a.f;
+    const int64_t n = f.n;
+    const bool act = !f.mask || f.mask[i];
+    Real m[3], hex[3] = {Real(0), Real(0), Real(0)}, hdv[3] = {Real(0), Real(0), Real(0)}, h[3], k[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = Real(cc[c]);
+    if (act) {
+        if (f.exchange) {
+            Src he[3];
+            exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
+        }
+        if (a.hd) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) hdv[c] = a.hd[c * n + i];
+        }
+        const Real he0 = f.has_ext ? Real(f.hx) : Real(0), he1 = f.has_ext ? Real(f.hy) : Real(0),
+                   he2 = f.has_ext ? Real(f.hz) : Real(0);
+        h[0] = (hex[0] + hdv[0]) + he0;
+        h[1] = (hex[1] + hdv[1]) + he1;
+        h[2] = (hex[2] + hdv[2]) + he2;
+        torque(f, m, h, k);
+    } else {
+        h[0] = h[1] = h[2] = Real(0);
+        k[0] = k[1] = k[2] = Real(0);
+    }
+    if (a.kout) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.kout[c * n + i] = k[c];
+    }
+    if (MODE == STAGE_PLAIN && a.out) {
+        State v[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            State s = State(a.anew) * State(k[c]);
+            for (int j = 0; j < a.nprev; ++j) s += State(a.ap[j]) * State(a.kp[j][c * n + i]);
+            v[c] = a.m0[c * n + i] + State(a.dt) * s;
+        }
+        const State nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        const State inv = nrm > State(0) ? State(1) / nrm : State(0);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.out[c * n + i] = Out(v[c] * inv);
+    }
+    if (MODE == STAGE_STATS && act) {
+        // m . H per term in FP64 with the state's own m
+        double md[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) md[c] = double(cc[c]);
+        const double w = f.ms_cell ? f.ms_cell[i] : 1.0;
+        neu_add(acc[0], acc[1], w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2])));
+        neu_add(acc[2], acc[3], w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2])));
+        if (f.has_ext) neu_add(acc[4], acc[5], w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz));
+        const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
+        const double d2 = double(hdv[0]) * hdv[0] + double(hdv[1]) * hdv[1] + double(hdv[2]) * hdv[2];
+        const double k2 = double(k[0]) * k[0] + double(k[1]) * k[1] + double(k[2]) * k[2];
+        acc[6] = fmax(acc[6], h2);
+        acc[7] = fmax(acc[7], d2);
+        acc[8] = fmax(acc[8], k2);
+    }
+    if (MODE == STAGE_ERROR) {
+        // lib.rs:1653-1674: |dt sum_j E_j k_j| in FP64
+        double e[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double s = a.anew * double(k[c]);
+            for (int j = 0; j < a.nprev; ++j) s += a.ap[j] * double(a.kp[j][c * n + i]);
+            e[c] = s * a.dt;
+        }
+        acc[0] = fmax(acc[0], e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    }
+}
+
+// One workgroup per (tile of TX x TY columns, chunk of zc planes); each lane
+// walks its column upward keeping src at z-1, z, z+1 in registers, so every
+// plane of src is read from memory once per chunk (plus one halo plane on
+// each side) and the x/y neighbours come from the same or an adjacent tile
+// on the same XCD.
+template <typename State, typename Real, typename Src, typename Out, int MODE>
+__global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Src, Out> a) {
+    const FieldArgs &f = a.f;
+    const int64_t n = f.n;
+    double acc[kNStat];
+#pragma unroll
+    for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
+    const uint32_t tile = f.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int chunk = (int)(tile % (uint32_t)f.nchunks);
+    const uint32_t t2 = tile / (uint32_t)f.nchunks;
+    const int txi = (int)(t2 % (uint32_t)f.tiles_x), tyi = (int)(t2 / (uint32_t)f.tiles_x);
+    const int TX = 1 << f.tx_log2;
+    const int x = txi * TX + (int)(threadIdx.x & (TX - 1));
+    const int y = tyi * (kBlock >> f.tx_log2) + (int)(threadIdx.x >> f.tx_log2);
+    const int z0 = chunk * f.zc, z1 = min(f.nz, z0 + f.zc);
+    if (x < f.nx && y < f.ny && z0 < z1) {
+        const int64_t nxy = (int64_t)f.nx * f.ny;
+        int64_t i = ((int64_t)z0 * f.ny + y) * f.nx + x;
+        Src prev[3], cur[3], nxt[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) cur[c] = a.src[c * n + i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) prev[c] = z0 > 0 ? a.src[c * n + i - nxy] : cur[c];
+        for (int z = z0; z < z1; ++z, i += nxy) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nxt[c] = z + 1 < f.nz ? a.src[c * n + i + nxy] : cur[c];
+            stage_cell<State, Real, Src, Out, MODE>(a, i, x, y, z, cur, prev, nxt, acc);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                prev[c] = cur[c];
+                cur[c] = nxt[c];
+            }
+        }
+    }
+    if (MODE == STAGE_PLAIN) return;
+    block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * (MODE == STAGE_STATS ? kNStat : 1));
+}
+
+// Column tiling of the stage kernels: tile width TX = 64 (or the smallest
+// power of two >= nx), TY = kBlock / TX rows; z split into chunks so that the
+// grid has about 2048 workgroups.
+void stage_tiling(const Context &c, int &tx_log2, int &tiles_x, int &tiles_y, int &zc, int &nchunks) {
+    tx_log2 = 0;
+    while ((1u << tx_log2) < c.nx && tx_log2 < 6) ++tx_log2;
+    const int tx = 1 << tx_log2, ty = kBlock / tx;
+    tiles_x = (int)cdiv(c.nx, tx);
+    tiles_y = (int)cdiv(c.ny, ty);
+    const uint64_t tiles = (uint64_t)tiles_x * tiles_y;
+    const int want = (int)std::min<uint64_t>(c.nz, std::max<uint64_t>(1, cdiv(2048, tiles)));
+    zc = (int)cdiv(c.nz, want);
+    nchunks = (int)cdiv(c.nz, zc);
+}
+
+unsigned stage_blocks(const Context &c) {
+    int tl, tx, ty, zc, nch;
+    stage_tiling(c, tl, tx, ty, zc, nch);
+    return (unsigned)((uint64_t)tx * ty * nch);
+}
+
+FieldArgs field_args(const Context &c) {
+    FieldArgs f{};
+    f.nx = (int)c.nx;
+    f.ny = (int)c.ny;
+    f.nz = (int)c.nz;
+    f.n = (int64_t)c.n;
+    f.div_nx = FastDiv(c.nx);
+    f.div_nxy = FastDiv(c.nx * c.ny);
+    stage_tiling(c, f.tx_log2, f.tiles_x, f.tiles_y, f.zc, f.nchunks);
+    f.xcd = c.dp.tune.xcd;
+    f.idx2 = 1.0 / (c.dx * c.dx);
+    f.idy2 = 1.0 / (c.dy * c.dy);
+    f.idz2 = 1.0 / (c.dz * c.dz);
+    f.a_uniform = c.use_lut ? c.a : c.a_exchange;
+    f.pref = 2.0 * f.a_uniform / (kMu0 * c.ms);
+    f.two_over_mu0 = 2.0 / kMu0;
+    f.pre_lut = 2.0 / (kMu0 * c.ms);
+    f.mask = c.d_mask;
+    f.region = c.use_lut ? c.d_region : nullptr;
+    f.lut = c.use_lut ? c.d_lut : nullptr;
+    f.ms_cell = c.d_ms_cell;
+    f.exchange = c.exchange ? 1 : 0;
+    f.has_ext = c.has_ext ? 1 : 0;
+    f.precession = c.precession ? 1 : 0;
+    f.hx = c.hext[0];
+    f.hy = c.hext[1];
+    f.hz = c.hext[2];
+    f.gamma_bar = c.gamma / (1.0 + c.alpha * c.alpha);
+    f.alpha = c.alpha;
+    return f;
+}
+
+struct Timed {
+    Context &c;
+    int entry = -1;
+    hipEvent_t ev{};
+    Timed(Context &cc, const char *name) : c(cc) {
+        if (c.timing) {
+            entry = timing_entry(c, name);
+            timing_begin(c, entry, &ev);
+        }
+    }
+    ~Timed() {
+        if (entry >= 0) timing_end(c, entry, ev);
+    }
+};
+
+}  // namespace
+}  // namespace fm

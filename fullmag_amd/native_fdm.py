@@ -356,6 +356,34 @@ class NativeFdmBackend:
     def precision_mode(self) -> int:
         return A.lib().fullmag_fdm_ext_precision(self._handle)
 
+    # -- device-resident state and the Heun adjoint (extension) -------------
+    # Device arrays are passed as raw pointers (e.g. torch.Tensor.data_ptr()
+    # of contiguous float64 tensors of 3 * cells in SoA order, or cells for
+    # grad_ms); the caller orders its own device work before these calls.
+    def copy_state_device(self, dst_ptr: int) -> None:
+        rc = A.lib().fullmag_fdm_ext_copy_state_device(self._handle, ctypes.c_void_p(dst_ptr), 3 * self.cell_count)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("copy_state_device failed")
+
+    def set_state_device(self, src_ptr: int) -> None:
+        rc = A.lib().fullmag_fdm_ext_set_state_device(self._handle, ctypes.c_void_p(src_ptr), 3 * self.cell_count)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("set_state_device failed")
+
+    def heun_adjoint(self, dt: float, m0_ptr: int, lam1_ptr: int, lam0_ptr: int, grad_ms_ptr: int = 0,
+                     grad_hext=None) -> None:
+        """lam0 = (dm1/dm0)^T lam1 for one Heun step from m0; accumulates
+        the per-cell Ms gradient (device) and the H_ext gradient (numpy (3,),
+        A/m) of <lam1, m1>."""
+        gh = (ctypes.c_double * 3)(0.0, 0.0, 0.0)
+        rc = A.lib().fullmag_fdm_ext_heun_adjoint(self._handle, float(dt), ctypes.c_void_p(m0_ptr),
+                                                  ctypes.c_void_p(lam1_ptr), ctypes.c_void_p(lam0_ptr),
+                                                  ctypes.c_void_p(grad_ms_ptr or None), gh)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("heun_adjoint failed")
+        if grad_hext is not None:
+            grad_hext += np.array(gh[:])
+
     def device_bytes(self) -> int:
         return int(A.lib().fullmag_fdm_ext_device_bytes(self._handle))
 

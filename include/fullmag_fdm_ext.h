@@ -73,6 +73,24 @@ int fullmag_fdm_ext_get_timing(fullmag_fdm_backend *handle, int index, const cha
  * the initial magnetisation of a plan. */
 void fullmag_fdm_ext_random_unit_vectors(uint64_t seed, uint64_t count, double *out_xyz);
 
+/* Discrete adjoint of one Heun step, for gradient-based inverse design
+ * (torch.autograd callers).  No counterpart in the reference ABI.  m0_dev:
+ * state before the step, lam1_dev: cotangent dL/dm of the state after it,
+ * lam0_dev: output dL/dm0; all device arrays of 3 * cells FP64 in SoA order
+ * ([x..][y..][z..]).  grad_ms_dev (device, cells FP64) and grad_hext (host,
+ * 3 FP64, A/m) are accumulated into and may be NULL.  Needs precision double
+ * or mixed (FP64 state); fields use the backend's precision.  Synchronous;
+ * the caller orders its own device work before the call. */
+int fullmag_fdm_ext_heun_adjoint(fullmag_fdm_backend *handle, double dt_seconds, const double *m0_dev,
+                                 const double *lam1_dev, double *lam0_dev, double *grad_ms_dev, double *grad_hext);
+
+/* Device-to-device copy of the FP64 state (SoA, 3 * cells) out of / into the
+ * backend, for checkpointing without host round trips.  set_state_device
+ * does not normalise; inactive cells must already be zero.  It resets the
+ * DP45 FSAL slope and the ABM3 history like upload_magnetization. */
+int fullmag_fdm_ext_copy_state_device(fullmag_fdm_backend *handle, double *dst_dev, uint64_t len);
+int fullmag_fdm_ext_set_state_device(fullmag_fdm_backend *handle, const double *src_dev, uint64_t len);
+
 /* Bytes of device memory the backend holds. */
 uint64_t fullmag_fdm_ext_device_bytes(fullmag_fdm_backend *handle);
 

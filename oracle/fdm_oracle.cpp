@@ -32,6 +32,9 @@
 //       checked against hand-computed xorshift64 states.
 //   per-cell Ms                                  extension with no reference
 //       counterpart: reduces to the uniform case when all Ms_i are equal.
+//   Heun adjoint (extension)                     pinned by central finite
+//       differences of the oracle's own heun_step (state, per-cell Ms and
+//       H_ext directions) and by the dot-product identity.
 
 #include <algorithm>
 #include <array>
@@ -563,8 +566,9 @@ std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m) {
     return h;
 }
 
-// lib.rs:2306-2400
-std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) {
+// lib.rs:2306-2400.  weight_ms = false applies the bare convolution -N * x
+// (the linear operator with Ms = 1), which the adjoint needs.
+std::vector<V3> demag_apply(Problem &p, const std::vector<V3> &m, bool weight_ms) {
     const size_t px = p.px, py = p.py, pz = p.pz, plen = px * py * pz;
     for (int c = 0; c < 3; ++c) {
         std::fill(p.bm[c].begin(), p.bm[c].end(), cd(0, 0));
@@ -574,7 +578,7 @@ std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) {
         for (size_t y = 0; y < p.ny; ++y)
             for (size_t x = 0; x < p.nx; ++x) {
                 size_t s = x + p.nx * (y + p.ny * z), d = padded_index(px, py, x, y, z);
-                V3 mo = p.active(s) ? scale(m[s], p.ms_at(s)) : V3{0, 0, 0};
+                V3 mo = p.active(s) ? scale(m[s], weight_ms ? p.ms_at(s) : 1.0) : V3{0, 0, 0};
                 for (int c = 0; c < 3; ++c) p.bm[c][d] = cd(mo[c], 0.0);
             }
     for (int c = 0; c < 3; ++c) fft3_core(p.bm[c].data(), (int)px, (int)py, (int)pz, false);
@@ -599,6 +603,34 @@ std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) {
                                         p.bh[2][s].real() * norm_}
                                    : V3{0, 0, 0};
             }
+    return h;
+}
+
+std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) { return demag_apply(p, m, true); }
+
+// Discrete Laplacian sum_j w_ij (q_j - q_i) with the exchange stencil's
+// clamped and masked neighbours (exchange_field = (2A / (mu0 Ms)) lap(m)).
+// Symmetric in i, j, so it is its own transpose.
+std::vector<V3> lap_field(const Problem &p, const std::vector<V3> &q) {
+    std::vector<V3> h(p.n);
+    const double dx2 = p.dx * p.dx, dy2 = p.dy * p.dy, dz2 = p.dz * p.dz;
+#pragma omp parallel for schedule(static)
+    for (long fl = 0; fl < (long)p.n; ++fl) {
+        size_t f = (size_t)fl;
+        if (!p.active(f)) { h[f] = {0, 0, 0}; continue; }
+        size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
+        const V3 c = q[f];
+        auto nb = [&](size_t a, size_t b, size_t cc) -> V3 {
+            size_t id = a + p.nx * (b + p.ny * cc);
+            return p.active(id) ? q[id] : c;
+        };
+        V3 xm = nb(x > 0 ? x - 1 : 0, y, z), xp = nb(std::min(x + 1, p.nx - 1), y, z);
+        V3 ym = nb(x, y > 0 ? y - 1 : 0, z), yp = nb(x, std::min(y + 1, p.ny - 1), z);
+        V3 zm = nb(x, y, z > 0 ? z - 1 : 0), zp = nb(x, y, std::min(z + 1, p.nz - 1));
+        for (int k = 0; k < 3; ++k)
+            h[f][k] = (xp[k] - 2.0 * c[k] + xm[k]) / dx2 + (yp[k] - 2.0 * c[k] + ym[k]) / dy2 +
+                      (zp[k] - 2.0 * c[k] + zm[k]) / dz2;
+    }
     return h;
 }
 
@@ -848,6 +880,104 @@ void report_to(const Report &r, double *out) {
 // ---------------------------------------------------------------------------
 // C entry points (ctypes from tests/ and bench.py's cpu_baseline leg)
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Discrete adjoint of one Heun step (extension; no counterpart in the
+// reference).  With f(m, H) = -gb (p m x H + alpha m x (m x H)):
+//   (df/dm)^T mu = -gb (p H x mu + alpha ((m.H) mu + H (m.mu) - 2 m (H.mu)))
+//   (df/dH)^T mu = -gb (p mu x m + alpha (m (m.mu) - |m|^2 mu))
+// normalisation n(v) = v/|v|: (dn/dv)^T l = (l - n (n.l)) / |v| (0 at v = 0);
+// H(m) = (2A/(mu0 Ms)) lap(m) - N * (Ms m) + H_ext on active cells, so
+// (dH/dm)^T g = lap(2A/(mu0 Ms) g) + Ms (-N * g)  (lap and the convolution
+// are symmetric), dH_i/dMs_i = -H_ex,i/Ms_i for exchange and
+// d(-N*(Ms m))/dMs_j = -N * (e_j m_j) for demag, dH/dH_ext = I.
+// ---------------------------------------------------------------------------
+inline V3 torque_adj_m(const Problem &p, const V3 &m, const V3 &h, const V3 &mu) {
+    const double gb = p.gamma / (1.0 + p.alpha * p.alpha);
+    V3 t = p.precession ? cross(h, mu) : V3{0, 0, 0};
+    const double mh = dot(m, h), mm = dot(m, mu), hm = dot(h, mu);
+    for (int k = 0; k < 3; ++k) t[k] += p.alpha * (mh * mu[k] + h[k] * mm - 2.0 * m[k] * hm);
+    return scale(t, -gb);
+}
+
+inline V3 torque_adj_h(const Problem &p, const V3 &m, const V3 &mu) {
+    const double gb = p.gamma / (1.0 + p.alpha * p.alpha);
+    V3 t = p.precession ? cross(mu, m) : V3{0, 0, 0};
+    const double mm = dot(m, mu), m2 = dot(m, m);
+    for (int k = 0; k < 3; ++k) t[k] += p.alpha * (m[k] * mm - m2 * mu[k]);
+    return scale(t, -gb);
+}
+
+inline V3 norm_adj(const V3 &v, const V3 &l) {
+    const double nv = norm(v);
+    if (nv <= 0.0) return {0, 0, 0};
+    const V3 y = scale(v, 1.0 / nv);
+    return scale(sub(l, scale(y, dot(y, l))), 1.0 / nv);
+}
+
+// Adds (dH/dm)^T g at state m to lam, and the parameter gradients.
+void field_adjoint(Problem &p, const std::vector<V3> &m, const std::vector<V3> &h_ex, const std::vector<V3> &g,
+                   std::vector<V3> &lam, std::vector<double> &grad_ms, V3 &grad_hext) {
+    const size_t n = p.n;
+    if (p.exchange) {
+        std::vector<V3> q(n);
+        for (size_t i = 0; i < n; ++i)
+            q[i] = p.active(i) ? scale(g[i], 2.0 * p.a / (MU0 * p.ms_at(i))) : V3{0, 0, 0};
+        std::vector<V3> lq = lap_field(p, q);
+        for (size_t i = 0; i < n; ++i) {
+            lam[i] = add(lam[i], lq[i]);
+            if (p.active(i)) grad_ms[i] += -dot(g[i], h_ex[i]) / p.ms_at(i);
+        }
+    }
+    if (p.demag) {
+        std::vector<V3> w = demag_apply(p, g, false);
+        for (size_t i = 0; i < n; ++i) {
+            if (!p.active(i)) continue;
+            lam[i] = add(lam[i], scale(w[i], p.ms_at(i)));
+            grad_ms[i] += dot(m[i], w[i]);
+        }
+    }
+    if (p.has_ext)
+        for (size_t i = 0; i < n; ++i)
+            if (p.active(i)) grad_hext = add(grad_hext, g[i]);
+}
+
+// lam0 = (d m1 / d m0)^T lam1 for m1 = heun(m0); grad_ms (n) and grad_hext
+// (3) accumulate d<lam1, m1>/dMs_i and d<lam1, m1>/dH_ext.
+void heun_adjoint(Problem &p, double dt, const std::vector<V3> &m0, const std::vector<V3> &lam1,
+                  std::vector<V3> &lam0, std::vector<double> &grad_ms, V3 &grad_hext) {
+    const size_t n = p.n;
+    Eval e0 = full_eval(p, m0, false);
+    std::vector<V3> v(n), mt(n);
+    for (size_t i = 0; i < n; ++i) {
+        v[i] = add(m0[i], scale(e0.rhs[i], dt));
+        mt[i] = normalized(v[i]);
+    }
+    Eval et = full_eval(p, mt, false);
+    std::vector<V3> u(n);
+    for (size_t i = 0; i < n; ++i) u[i] = add(m0[i], scale(add(e0.rhs[i], et.rhs[i]), 0.5 * dt));
+    // backward through m1 = n(u), u = m0 + dt/2 (k1 + k2)
+    std::vector<V3> lam_u(n), lam_k1(n), lam_mt(n), g(n);
+    for (size_t i = 0; i < n; ++i) {
+        lam_u[i] = norm_adj(u[i], lam1[i]);
+        const V3 mu = scale(lam_u[i], 0.5 * dt);  // lam_k2
+        lam_mt[i] = torque_adj_m(p, mt[i], et.h_eff[i], mu);
+        g[i] = p.active(i) ? torque_adj_h(p, mt[i], mu) : V3{0, 0, 0};
+    }
+    field_adjoint(p, mt, et.h_ex, g, lam_mt, grad_ms, grad_hext);
+    // mt = n(v), v = m0 + dt k1
+    lam0.assign(n, V3{0, 0, 0});
+    for (size_t i = 0; i < n; ++i) {
+        const V3 lv = norm_adj(v[i], lam_mt[i]);
+        lam0[i] = add(lam_u[i], lv);
+        const V3 mu = add(scale(lam_u[i], 0.5 * dt), scale(lv, dt));  // lam_k1
+        lam0[i] = add(lam0[i], torque_adj_m(p, m0[i], e0.h_eff[i], mu));
+        g[i] = p.active(i) ? torque_adj_h(p, m0[i], mu) : V3{0, 0, 0};
+    }
+    field_adjoint(p, m0, e0.h_ex, g, lam0, grad_ms, grad_hext);
+    for (size_t i = 0; i < n; ++i)
+        if (!p.active(i)) lam0[i] = {0, 0, 0};
+}
+
 extern "C" {
 
 int oracle_num_threads(void) {
@@ -1003,6 +1133,14 @@ void oracle_problem_set_m(void *h, const double *in) {
     }
 }
 
+// Raw state (no normalisation; inactive cells zero): finite-difference
+// checks of the adjoint perturb m0 off the unit sphere.
+void oracle_problem_set_m_raw(void *h, const double *in) {
+    auto *p = static_cast<Problem *>(h);
+    for (size_t i = 0; i < p->n; ++i)
+        p->m[i] = p->active(i) ? V3{in[3 * i], in[3 * i + 1], in[3 * i + 2]} : V3{0, 0, 0};
+}
+
 double oracle_problem_time(void *h) { return static_cast<Problem *>(h)->time; }
 
 // fields: 5 AoS arrays of 3N doubles back to back: H_ex, H_demag, H_ext, H_eff, rhs.
@@ -1018,6 +1156,31 @@ void oracle_problem_observe(void *h, double *fields, double *scalars) {
     e.r.time_seconds = p->time;
     e.r.dt_used = 0; e.r.suggested_next_dt = 0;
     if (scalars) report_to(e.r, scalars);
+}
+
+// Adjoint of one Heun step from state m0 (AoS 3N, not the problem's state).
+// lam1: AoS 3N cotangent of the state after the step; lam0_out: AoS 3N;
+// grad_ms (N) and grad_hext (3) are accumulated into.
+int oracle_problem_heun_adjoint(void *h, double dt, const double *m0, const double *lam1, double *lam0_out,
+                                double *grad_ms, double *grad_hext) {
+    auto *p = static_cast<Problem *>(h);
+    const size_t n = p->n;
+    std::vector<V3> vm0(n), vl1(n), vl0;
+    for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            vm0[i][k] = m0[3 * i + k];
+            vl1[i][k] = lam1[3 * i + k];
+        }
+    std::vector<double> gms(n, 0.0);
+    V3 gh = {0, 0, 0};
+    heun_adjoint(*p, dt, vm0, vl1, vl0, gms, gh);
+    for (size_t i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) lam0_out[3 * i + k] = vl0[i][k];
+        if (grad_ms) grad_ms[i] += gms[i];
+    }
+    if (grad_hext)
+        for (int k = 0; k < 3; ++k) grad_hext[k] += gh[k];
+    return 0;
 }
 
 }  // extern "C"

@@ -60,6 +60,10 @@ def lib():
         L.oracle_problem_step.argtypes = [ctypes.c_void_p, d, _dp]
         L.oracle_problem_get_m.argtypes = [ctypes.c_void_p, _dp]
         L.oracle_problem_set_m.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_problem_set_m_raw.argtypes = [ctypes.c_void_p, _dp]
+        L.oracle_problem_heun_adjoint.restype = i
+        L.oracle_problem_heun_adjoint.argtypes = [ctypes.c_void_p, d, _dp, _dp, _dp, ctypes.c_void_p,
+                                                  ctypes.c_void_p]
         L.oracle_problem_time.restype = d
         L.oracle_problem_time.argtypes = [ctypes.c_void_p]
         L.oracle_problem_observe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
@@ -208,6 +212,24 @@ class OracleProblem:
     def m(self, value: np.ndarray) -> None:
         v = np.ascontiguousarray(value, dtype=np.float64).reshape(self.n, 3)
         lib().oracle_problem_set_m(self._h, _ptr(v))
+
+    def set_m_raw(self, value: np.ndarray) -> None:
+        """State without normalisation (inactive cells zero)."""
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(self.n, 3)
+        lib().oracle_problem_set_m_raw(self._h, _ptr(v))
+
+    def heun_adjoint(self, dt: float, m0: np.ndarray, lam1: np.ndarray):
+        """Adjoint of one Heun step from m0: returns (lam0, grad_ms (N),
+        grad_hext (3)) with lam0 = (dm1/dm0)^T lam1 and the gradients of
+        <lam1, m1> with respect to per-cell Ms and H_ext."""
+        m0 = np.ascontiguousarray(m0, dtype=np.float64).reshape(self.n, 3)
+        lam1 = np.ascontiguousarray(lam1, dtype=np.float64).reshape(self.n, 3)
+        lam0 = np.zeros((self.n, 3))
+        gms = np.zeros(self.n)
+        gh = np.zeros(3)
+        lib().oracle_problem_heun_adjoint(self._h, dt, _ptr(m0), _ptr(lam1), _ptr(lam0),
+                                          gms.ctypes.data_as(ctypes.c_void_p), gh.ctypes.data_as(ctypes.c_void_p))
+        return lam0, gms, gh
 
     @property
     def time(self) -> float:

@@ -215,6 +215,42 @@ def test_demag_field_all_shapes(fa, oracle_mod, cells, kernel):
                                    rtol=1e-12)
 
 
+@pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11)])
+def test_demag_field_long_z(fa, oracle_mod, cells):
+    """nz > 8: the z convolution runs as LDS line transforms (padded lengths
+    24, 18, 32 and 22, with radices 3 and 11)."""
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(5 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(3e-9, 4e-9, 2e-9), ms=9e5, a=1e-11, alpha=0.1,
+                         m0=m0, exchange=False, kernel="device")
+    hd = cpu.observe()["h_demag"]
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+
+
+# Launch-schedule knobs change only how the transforms are scheduled; every
+# schedule must give the oracle's field and trajectory.
+SCHEDULES = ["", "z_lds=1", "fixed=0", "z_quad=1", "z_mirror=0", "x_rows=4,y_lpw=1", "xcd=0",
+             "x_radix=4,y_radix=8", "x_lanes=128,y_lanes=64"]
+
+
+@pytest.mark.parametrize("cells", [(32, 16, 4), (64, 32, 8), (16, 16, 6)])
+@pytest.mark.parametrize("tune", SCHEDULES)
+def test_demag_schedule_variants_match_oracle(fa, oracle_mod, monkeypatch, cells, tune):
+    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(17 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 4e-9, 2e-9), ms=8.6e5, a=1.3e-11, alpha=0.05,
+                         m0=m0, h_ext=(1e4, 0.0, 2e3), kernel="device")
+    hd = cpu.observe()["h_demag"]
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+    for _ in range(3):
+        gpu.step(5e-14)
+        cpu.step(5e-14)
+    assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12
+
+
 @pytest.mark.parametrize("cells,tf", [((8, 6, 1), True), ((6, 5, 4), False), ((45, 3, 2), False),
                                       ((7, 9, 1), False)])
 def test_in_library_newell_spectra_match_oracle(fa, oracle_mod, cells, tf):
