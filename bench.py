@@ -125,6 +125,21 @@ def accuracy(fa, base, dt, steps, perturb=None):
     return out
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_heun_film512x512x8.md")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH + WRITE, calibrated against a
+    copy kernel) from the committed rocprofv3 --pmc summary, or None."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        k = json.loads(block)["kernels"].get(kernel)
+        return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+    except (OSError, IndexError, KeyError, ValueError):
+        return None
+
+
 def time_steps(backend, steps, warmup, dt):
     import torch
 
@@ -167,9 +182,196 @@ def cpu_baseline(base, dt):
                       "FP64 oracle restating crates/fullmag-engine (setup and Newell build excluded)"}
 
 
+# ---------------------------------------------------------------------------
+# Other workloads (opt-in with --workload; the default line is the film)
+# ---------------------------------------------------------------------------
+def line(metric, value, unit, steps, warmup, ms_per_step, dtype, config, higher=True, **extra):
+    out = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+           "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def sweep96(fa, torch, args):
+    """96 independent instances: grids {32,64,128,256}^2 x {1,2,4,8} layers
+    (6 of each), cells 5x5x3 nm, Ms ~ U[4e5,1.4e6], A ~ U[5e-12,2e-11],
+    alpha ~ U[0.005,0.5], B = random unit direction x U[0,100] mT, seeded
+    random m0, dt 1e-13 s, mixed-precision Heun advanced together through
+    one hipGraph (20 steps per replay), schedules autotuned per grid."""
+    rng = np.random.default_rng(5)
+    shapes = [(s, s, l) for s in (32, 64, 128, 256) for l in (1, 2, 4, 8)]
+    members, cells = [], 0
+    t0 = time.perf_counter()
+    for i in range(96):
+        nx, ny, nz = shapes[i % 16]
+        n = nx * ny * nz
+        d = fa.generate_random_unit_vectors(1000 + i, 1)[0]
+        b_t = 0.1 * rng.uniform()
+        plan = fa.FdmPlan(cells=(nx, ny, nz), cell_size=(5e-9, 5e-9, 3e-9),
+                          initial_magnetization=fa.generate_random_unit_vectors(2000 + i, n),
+                          saturation_magnetisation=rng.uniform(4e5, 1.4e6), exchange_stiffness=rng.uniform(5e-12, 2e-11),
+                          damping=rng.uniform(0.005, 0.5), external_field=tuple(b_t / MU0 * d), precision="mixed",
+                          demag_kernel="device")
+        members.append(fa.NativeFdmBackend(plan))
+        cells += n
+    setup_s = time.perf_counter() - t0
+    schedules = {}
+    for i, b in enumerate(members):
+        key = shapes[i % 16]
+        if key not in schedules:
+            schedules[key] = b.autotune(reps=3)
+        else:
+            b.set_schedule(schedules[key])
+    dt = 1e-13
+    run = fa.BatchRun(members, [dt] * len(members), steps_per_graph=20)
+    run.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = run.run(args.steps)
+    el = time.perf_counter() - t0
+    # the same members stepped one at a time without graphs, on a bounded sample
+    ref_steps = max(1, args.steps // 10)
+    t1 = time.perf_counter()
+    for b in members:
+        b.step_many(dt, ref_steps)
+    el_eager = time.perf_counter() - t1
+    run.close()
+    finite = all(np.isfinite(s.e_total) for s in stats)
+    for b in members:
+        b.close()
+    line("Heun cell-steps/s aggregate over a 96-instance batched sweep (mixed precision)", cells * args.steps / el,
+         "cell-steps/s", args.steps, args.warmup, 1e3 * el / args.steps, "f32 fields / f64 state",
+         {"workload": "batched sweep: 96 instances, grids {32,64,128,256}^2 x {1,2,4,8} layers (6 each), 5x5x3 nm, "
+                      "Ms U[4e5,1.4e6], A U[5e-12,2e-11], alpha U[0.005,0.5], B random dir x U[0,100] mT, "
+                      "seeded m0, Heun dt 1e-13 s, one hipGraph per 20 steps, autotuned schedules",
+          "instances": 96, "cells_total": cells, "setup_s": setup_s, "all_finite": finite,
+          "eager_step_many_cell_steps_per_s": cells * ref_steps / el_eager,
+          "schedules": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in schedules.items()}})
+
+
+def vortex_m0(nx, ny, nz, c=0.1, noise=0.01, seed=3):
+    y, x = np.meshgrid(np.arange(ny) - (ny - 1) / 2, np.arange(nx) - (nx - 1) / 2, indexing="ij")
+    v = np.stack([-y, x, np.full_like(x, c)], -1)
+    v = v / np.linalg.norm(v, axis=-1, keepdims=True)
+    v = np.broadcast_to(v[None], (nz, ny, nx, 3)).reshape(-1, 3)
+    v = v + noise * np.random.default_rng(seed).standard_normal(v.shape)
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def dp45_block(fa, torch, args):
+    """256x256x64 cells of 2 nm, Ms 1.4e6, A 2.1e-11, alpha 0.01, no Zeeman,
+    m0 = normalise((-y, x, 0.1) + 0.01 N(0,1)) (coordinates in cells from the
+    grid centre, seed 3), DP45 with max error 1e-6 and dt feedback from
+    suggested_next_dt, demag padded 512x512x128; --steps accepted steps."""
+    nx, ny, nz = 256, 256, 64
+    n = nx * ny * nz
+    base = dict(cells=(nx, ny, nz), cell_size=(2e-9, 2e-9, 2e-9), initial_magnetization=vortex_m0(nx, ny, nz),
+                saturation_magnetisation=1.4e6, exchange_stiffness=2.1e-11, damping=0.01, integrator="dp45",
+                demag_kernel="device", adaptive_max_error=1e-6)
+    res = {}
+    for prec, steps in (("mixed", args.steps), ("double", max(2, args.steps // 5))):
+        b = fa.NativeFdmBackend(fa.FdmPlan(precision=prec, **base))
+        dt = 1e-13
+        for _ in range(args.warmup):
+            s = b.step(dt)
+            dt = s.dt_suggested or dt
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim0 = s.time if args.warmup else 0.0
+        for _ in range(steps):
+            s = b.step(dt)
+            dt = s.dt_suggested or dt
+        el = time.perf_counter() - t0
+        res[prec] = {"cell_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps,
+                     "mean_dt": (s.time - sim0) / steps, "e_total": s.e_total}
+        b.close()
+    m = res["mixed"]
+    line("DP45 accepted cell-steps/s, 256x256x64 block (mixed precision)", m["cell_steps_per_s"], "cell-steps/s",
+         args.steps, args.warmup, m["ms_per_step"], "f32 fields / f64 state",
+         {"workload": "3-D block 256x256x64 cells of 2 nm, Ms 1.4e6, A 2.1e-11, alpha 0.01, no Zeeman, vortex m0 "
+                      "(-y, x, 0.1) + 1% noise, DP45 max error 1e-6, padded 512x512x128",
+          "cells": n, "mixed": m, "double": res["double"],
+          "speedup_vs_fp64": m["cell_steps_per_s"] / res["double"]["cell_steps_per_s"]})
+
+
+def adjoint_film(fa, torch, args):
+    """128x128x4 film of 5x5x3 nm, per-cell Ms ~ U[6e5, 9e5] (seed 4),
+    A 1.3e-11, alpha 0.1, B = (20, 5, 0) mT, 2000 Heun steps of 1e-13 s;
+    loss = mean m_x over the centred 32x32 window (x, y in [48, 80)) of all
+    layers; checkpoints every 100 steps.  Gradients w.r.t. per-cell Ms and B
+    from the adjoint, checked against central differences (h = 1e-4
+    relative, FP64) at 32 seeded cells and the two in-plane B components."""
+    nx, ny, nz = 128, 128, 4
+    n = nx * ny * nz
+    steps = 2000 if args.steps >= 300 else args.steps
+    rng = np.random.default_rng(4)
+    ms_np = rng.uniform(6e5, 9e5, n)
+    m0 = fa.generate_random_unit_vectors(42, n)
+    b_mt = np.array([20.0, 5.0, 0.0])
+    res = {}
+    for prec in ("double", "mixed"):
+        plan = fa.FdmPlan(cells=(nx, ny, nz), cell_size=(5e-9, 5e-9, 3e-9), initial_magnetization=m0,
+                          saturation_magnetisation=8e5, exchange_stiffness=1.3e-11, damping=0.1, precision=prec,
+                          demag_kernel="device")
+        ms = torch.tensor(ms_np, dtype=torch.float64, requires_grad=True)
+        bt = torch.tensor(b_mt, dtype=torch.float64, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = fa.heun_rollout(m0, ms, bt * 1e-3 / MU0, plan, 1e-13, steps, 100)
+        loss = m[:, 0].reshape(nz, ny, nx)[:, 48:80, 48:80].mean()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        loss.backward()
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t0
+        res[prec] = {"loss": float(loss), "g_ms": ms.grad.numpy().copy(), "g_b": bt.grad.numpy().copy(),
+                     "forward_s": t_fwd, "backward_s": t_all - t_fwd, "plan": plan}
+    d = res["double"]
+
+    def L(ms_v, b_v):
+        with torch.no_grad():
+            m = fa.heun_rollout(m0, torch.tensor(ms_v), torch.tensor(b_v * 1e-3 / MU0), d["plan"], 1e-13, steps,
+                                steps)
+            return float(m[:, 0].reshape(nz, ny, nx)[:, 48:80, 48:80].mean())
+
+    cells = rng.choice(n, size=32, replace=False) if args.steps >= 300 else rng.choice(n, size=4, replace=False)
+    errs, fds = [], []
+    for i in cells:
+        h = 1e-4 * ms_np[i]
+        p, q = ms_np.copy(), ms_np.copy()
+        p[i] += h
+        q[i] -= h
+        fd = (L(p, b_mt) - L(q, b_mt)) / (2 * h)
+        fds.append(fd)
+        errs.append(abs(fd - d["g_ms"][i]))
+    scale = max(np.abs(fds).max(), 1e-300)
+    b_errs = []
+    for k in range(2):
+        h = 1e-4 * np.abs(b_mt).max()
+        p, q = b_mt.copy(), b_mt.copy()
+        p[k] += h
+        q[k] -= h
+        fd = (L(ms_np, p) - L(ms_np, q)) / (2 * h)
+        b_errs.append(abs(fd - d["g_b"][k]) / max(abs(fd), 1e-300))
+    x = res["mixed"]
+    mixed_vs_fp64 = float(np.abs(x["g_ms"] - d["g_ms"]).max() / np.abs(d["g_ms"]).max())
+    line("adjoint cell-steps/s (Heun backward pass with checkpoint recompute), 128x128x4 film",
+         n * steps / d["backward_s"], "cell-steps/s", steps, 0, 1e3 * d["backward_s"] / steps, "f64",
+         {"workload": "differentiable film 128x128x4 of 5x5x3 nm, per-cell Ms U[6e5,9e5], A 1.3e-11, alpha 0.1, "
+                      "B (20,5,0) mT, 2000 Heun steps dt 1e-13, loss mean m_x over centred 32x32 window, "
+                      "checkpoint every 100",
+          "fd_cells": len(cells), "max_abs_err_over_max_fd_ms": float(max(errs) / scale),
+          "max_rel_err_b": float(max(b_errs)), "loss": d["loss"],
+          "fp64": {"forward_s": d["forward_s"], "backward_s": d["backward_s"]},
+          "mixed": {"forward_s": x["forward_s"], "backward_s": x["backward_s"],
+                    "max_rel_dev_g_ms_vs_fp64": mixed_vs_fp64}})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["film512", "sweep96", "dp45-block", "adjoint-film"], default="film512")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=100)
@@ -177,6 +379,8 @@ def main():
     ap.add_argument("--smooth-steps", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -188,6 +392,15 @@ def main():
     import fullmag_amd as fa
 
     torch.cuda.init()
+    if args.copy_calibration:
+        src = torch.ones(64 * 2**20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload != "film512":
+        {"sweep96": sweep96, "dp45-block": dp45_block, "adjoint-film": adjoint_film}[args.workload](fa, torch, args)
+        return
     dt = 5e-14
     base, extra = workload()
     nx, ny, nz = base["cells"]
@@ -207,7 +420,7 @@ def main():
     fp64.close()
 
     # accuracy of the mixed path against the FP64 GPU path from the same m0
-    acc = accuracy(fa, base, dt, args.acc_steps, perturb=1e-12)
+    acc = accuracy(fa, base, dt, args.acc_steps, perturb=1e-12) if args.acc_steps > 0 else None
     acc_smooth = None
     if args.smooth_steps > 0:
         acc_smooth = accuracy(fa, dict(base, initial_magnetization=smooth_m0(nx, ny, nz)), dt, args.smooth_steps)
@@ -219,7 +432,7 @@ def main():
         avg_s = tot_ms / cnt / 1e3
         achieved = ab[name] / avg_s / 1e9 if name in ab else None
         roofline = {"bound": "hbm", "kernel": name, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                    "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": measured_traffic(name),
                     "avg_launch_us": 1e6 * avg_s, "alg_bytes_per_launch": ab.get(name),
                     "kernels_us": {k: round(1e3 * v[0] / v[1], 2) for k, v in sorted(prof.items())}}
 

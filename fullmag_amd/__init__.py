@@ -7,6 +7,7 @@ ABI (crates/fullmag-runner/src/native_fdm.rs).
 """
 
 from .native_fdm import (  # noqa: F401
+    BatchRun,
     FdmPlan,
     FieldSnapshot,
     NativeFdmBackend,

@@ -170,6 +170,12 @@ SIGNATURES = {
     "fullmag_fdm_ext_heun_adjoint": (_i32, [ctypes.c_void_p, _d, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p, _pd]),
     "fullmag_fdm_ext_copy_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
+    "fullmag_fdm_ext_batch_create": (ctypes.c_void_p, [ctypes.POINTER(ctypes.c_void_p), _u32, _pd, _u32]),
+    "fullmag_fdm_ext_batch_run": (_i32, [ctypes.c_void_p, _u64, ctypes.POINTER(StepStats)]),
+    "fullmag_fdm_ext_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "fullmag_fdm_ext_batch_destroy": (None, [ctypes.c_void_p]),
+    "fullmag_fdm_ext_autotune": (_i32, [ctypes.c_void_p, _i32, ctypes.c_char_p, _u64]),
+    "fullmag_fdm_ext_set_schedule": (_i32, [ctypes.c_void_p, ctypes.c_char_p]),
     "fullmag_fdm_ext_set_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
 }
 

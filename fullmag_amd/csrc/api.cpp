@@ -434,6 +434,7 @@ int upload_impl(fullmag_fdm_backend *h, const void *m, uint64_t len, bool f64) {
 }  // namespace
 
 namespace fm {
+Context *backend_context(fullmag_fdm_backend *h) { return as_ctx(h); }
 int integrator_step_many(Context &c, double dt, uint64_t n, fullmag_fdm_step_stats *st);
 }
 

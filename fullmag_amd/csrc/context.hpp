@@ -40,6 +40,9 @@ struct DemagTuning {
     int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
 };
 
+// Applies "key=value,..." (unknown keys ignored) on top of t.
+void parse_tuning_string(DemagTuning &t, const char *text);
+
 struct DemagPlan {
     DemagTuning tune;
     bool enabled = false;
@@ -183,6 +186,14 @@ bool materialize_observables(Context &c);  // fills h_ex_obs / h_eff_obs at m
 bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
                    uint32_t z0, uint32_t zs, void *dev_out, bool out_f64);
 bool alloc_integrator_buffers(Context &c);
+int integrator_step_many(Context &c, double dt, uint64_t n, fullmag_fdm_step_stats *st);
+// Graph-captured batches: prepare (H_demag(m) valid, timing off), enqueue
+// n Heun steps on c.stream without host synchronisation (capturable),
+// finish (stats at the final state, counters advanced by nsteps).
+bool integrator_prepare(Context &c);
+void integrator_enqueue_heun(Context &c, double dt, int nsteps);
+int integrator_finish(Context &c, double dt, uint64_t nsteps, fullmag_fdm_step_stats *st);
+double integrator_probe_ms(Context &c, int reps);
 
 // adjoint.hip: discrete adjoint of one Heun step from the state m0 (device,
 // FP64 SoA 3n): lam0 = (dm1/dm0)^T lam1; grad_ms (device, n, may be null)

@@ -956,10 +956,11 @@ bool plan_lines(DemagPlan &dp, std::string &err) {
     return true;
 }
 
-void parse_tuning(DemagTuning &t) {
-    const char *env = std::getenv("FULLMAG_FDM_TUNE");
-    if (!env) return;
-    std::string s(env);
+}  // namespace
+
+void parse_tuning_string(DemagTuning &t, const char *text) {
+    if (!text) return;
+    std::string s(text);
     size_t pos = 0;
     while (pos < s.size()) {
         size_t end = s.find(',', pos);
@@ -985,6 +986,10 @@ void parse_tuning(DemagTuning &t) {
         pos = end + 1;
     }
 }
+
+namespace {
+
+void parse_tuning(DemagTuning &t) { parse_tuning_string(t, std::getenv("FULLMAG_FDM_TUNE")); }
 
 // Padded forward transform of the Newell kernel, FP64, into a temporary
 // spectrum spec[6][pz][hx][py].

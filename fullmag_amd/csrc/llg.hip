@@ -495,6 +495,86 @@ bool alloc_integrator_buffers(Context &c) {
     return true;
 }
 
+// -- graph-captured batches (fixed-dt Heun) ----------------------------------
+bool integrator_prepare(Context &c) {
+    if (c.integrator != FULLMAG_FDM_INTEGRATOR_HEUN) {
+        c.last_error = "graph-captured batches support the Heun integrator only";
+        return false;
+    }
+    c.timing = false;
+    if (c.mode == MODE_DOUBLE) Driver<double, double>(c).ensure_hd();
+    else if (c.mode == MODE_MIXED) Driver<double, float>(c).ensure_hd();
+    else Driver<float, float>(c).ensure_hd();
+    return hip_ok(hipStreamSynchronize(c.stream), "batch prepare", c.last_error);
+}
+
+void integrator_enqueue_heun(Context &c, double dt, int nsteps) {
+    for (int s = 0; s < nsteps; ++s) {
+        if (c.mode == MODE_DOUBLE) Driver<double, double>(c).heun(dt);
+        else if (c.mode == MODE_MIXED) Driver<double, float>(c).heun(dt);
+        else Driver<float, float>(c).heun(dt);
+    }
+}
+
+int integrator_finish(Context &c, double dt, uint64_t nsteps, fullmag_fdm_step_stats *st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (c.mode == MODE_DOUBLE) Driver<double, double>(c).stats(nullptr);
+    else if (c.mode == MODE_MIXED) Driver<double, float>(c).stats(nullptr);
+    else Driver<float, float>(c).stats(nullptr);
+    c.obs_valid = false;
+    c.obs_frozen = false;
+    if (!hip_ok(hipStreamSynchronize(c.stream), "batch steps", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+    if (!hip_ok(hipGetLastError(), "batch steps launch", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+    c.step_count += nsteps;
+    c.time += dt * (double)nsteps;
+    if (st) {
+        if (c.mode == MODE_DOUBLE) Driver<double, double>(c).fill(st, dt, 0.0);
+        else if (c.mode == MODE_MIXED) Driver<double, float>(c).fill(st, dt, 0.0);
+        else Driver<float, float>(c).fill(st, dt, 0.0);
+        st->wall_time_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                               std::chrono::steady_clock::now() - t0)
+                               .count();
+    }
+    return FULLMAG_FDM_OK;
+}
+
+// Device time of `reps` (demag + stats) evaluations at the current state with
+// the context's present launch schedule (read-only for the state).
+double integrator_probe_ms(Context &c, int reps) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    const bool timing = c.timing;
+    c.timing = false;
+    auto once = [&]() {
+        if (c.mode == MODE_DOUBLE) {
+            Driver<double, double> d(c);
+            d.demag_at_state(d.m());
+            d.stats(nullptr);
+        } else if (c.mode == MODE_MIXED) {
+            Driver<double, float> d(c);
+            d.demag_at_state(d.m());
+            d.stats(nullptr);
+        } else {
+            Driver<float, float> d(c);
+            d.demag_at_state(d.m());
+            d.stats(nullptr);
+        }
+    };
+    once();  // warm
+    hipEventRecord(a, c.stream);
+    for (int r = 0; r < reps; ++r) once();
+    hipEventRecord(b, c.stream);
+    hipEventSynchronize(b);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, a, b);
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    c.timing = timing;
+    c.hd_at_m = true;
+    return (double)ms;
+}
+
 int integrator_step(Context &c, double dt, fullmag_fdm_step_stats *st) {
     if (c.mode == MODE_DOUBLE) return Driver<double, double>(c).step(dt, st);
     if (c.mode == MODE_MIXED) return Driver<double, float>(c).step(dt, st);

@@ -118,6 +118,13 @@ class StepStats:
     wall_time_ns: int
     dt_suggested: Optional[float]
 
+    @staticmethod
+    def from_raw(s) -> "StepStats":
+        return StepStats(s.step, s.time_seconds, s.dt_seconds, s.exchange_energy_joules, s.demag_energy_joules,
+                         s.external_energy_joules, s.total_energy_joules, s.max_effective_field_amplitude,
+                         s.max_demag_field_amplitude, s.max_rhs_amplitude, s.wall_time_ns,
+                         s.suggested_next_dt if s.suggested_next_dt > 0.0 else None)
+
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
@@ -251,20 +258,14 @@ class NativeFdmBackend:
         rc = A.lib().fullmag_fdm_backend_step(self._handle, dt, ctypes.byref(s))
         if rc != A.FULLMAG_FDM_OK:
             raise self.last_error_or("step failed")
-        return StepStats(s.step, s.time_seconds, s.dt_seconds, s.exchange_energy_joules, s.demag_energy_joules,
-                         s.external_energy_joules, s.total_energy_joules, s.max_effective_field_amplitude,
-                         s.max_demag_field_amplitude, s.max_rhs_amplitude, s.wall_time_ns,
-                         s.suggested_next_dt if s.suggested_next_dt > 0.0 else None)
+        return StepStats.from_raw(s)
 
     def step_many(self, dt: float, n_steps: int) -> StepStats:
         s = A.StepStats()
         rc = A.lib().fullmag_fdm_ext_step_many(self._handle, dt, n_steps, ctypes.byref(s))
         if rc != A.FULLMAG_FDM_OK:
             raise self.last_error_or("step_many failed")
-        return StepStats(s.step, s.time_seconds, s.dt_seconds, s.exchange_energy_joules, s.demag_energy_joules,
-                         s.external_energy_joules, s.total_energy_joules, s.max_effective_field_amplitude,
-                         s.max_demag_field_amplitude, s.max_rhs_amplitude, s.wall_time_ns,
-                         s.suggested_next_dt if s.suggested_next_dt > 0.0 else None)
+        return StepStats.from_raw(s)
 
     # -- fields ------------------------------------------------------------
     def copy_field(self, observable: str) -> np.ndarray:
@@ -356,6 +357,21 @@ class NativeFdmBackend:
     def precision_mode(self) -> int:
         return A.lib().fullmag_fdm_ext_precision(self._handle)
 
+    def autotune(self, reps: int = 5) -> str:
+        """Pick the fastest launch schedule for this backend's grid (see
+        fullmag_fdm_ext_autotune); returns it as "key=value,..."."""
+        buf = ctypes.create_string_buffer(256)
+        rc = A.lib().fullmag_fdm_ext_autotune(self._handle, int(reps), buf, 256)
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("autotune failed")
+        return buf.value.decode()
+
+    def set_schedule(self, schedule: str) -> None:
+        """Apply a schedule string as autotune() returns it."""
+        rc = A.lib().fullmag_fdm_ext_set_schedule(self._handle, schedule.encode())
+        if rc != A.FULLMAG_FDM_OK:
+            raise self.last_error_or("set_schedule failed")
+
     # -- device-resident state and the Heun adjoint (extension) -------------
     # Device arrays are passed as raw pointers (e.g. torch.Tensor.data_ptr()
     # of contiguous float64 tensors of 3 * cells in SoA order, or cells for
@@ -403,6 +419,44 @@ class NativeFdmBackend:
             out[name.value.decode()] = (ms.value, cnt.value)
             i += 1
         return out
+
+
+class BatchRun:
+    """Independent Heun backends advanced together through one hipGraph
+    (fullmag_fdm_ext_batch_create / _run).  The members stay owned by the
+    caller and must not be stepped outside the batch while it exists."""
+
+    def __init__(self, backends: Sequence["NativeFdmBackend"], dts: Sequence[float], steps_per_graph: int = 10):
+        self.backends = list(backends)
+        n = len(self.backends)
+        handles = (ctypes.c_void_p * n)(*[b.handle for b in self.backends])
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        if d.size != n:
+            raise ValueError("one dt per backend")
+        self._h = A.lib().fullmag_fdm_ext_batch_create(handles, n, _dp(d), int(steps_per_graph))
+        if not self._h:
+            raise RunError("batch_create returned NULL")
+        err = A.lib().fullmag_fdm_ext_batch_last_error(self._h)
+        if err:
+            msg = err.decode(errors="replace")
+            self.close()
+            raise RunError(msg)
+
+    def run(self, n_steps: int) -> list:
+        stats = (A.StepStats * len(self.backends))()
+        rc = A.lib().fullmag_fdm_ext_batch_run(self._h, int(n_steps), stats)
+        if rc != A.FULLMAG_FDM_OK:
+            err = A.lib().fullmag_fdm_ext_batch_last_error(self._h)
+            raise RunError(err.decode(errors="replace") if err else f"batch_run failed ({rc})")
+        return [StepStats.from_raw(s) for s in stats]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            A.lib().fullmag_fdm_ext_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class FieldSnapshot:

@@ -91,6 +91,34 @@ int fullmag_fdm_ext_heun_adjoint(fullmag_fdm_backend *handle, double dt_seconds,
 int fullmag_fdm_ext_copy_state_device(fullmag_fdm_backend *handle, double *dst_dev, uint64_t len);
 int fullmag_fdm_ext_set_state_device(fullmag_fdm_backend *handle, const double *src_dev, uint64_t len);
 
+/* Batched sweeps: many independent backends (any grids, Heun integrator,
+ * one device) advanced together.  batch_create captures steps_per_graph
+ * fixed-dt Heun steps of every member, each on its own stream, as parallel
+ * branches of one hipGraph; batch_run(n) replays it n / steps_per_graph
+ * times, runs the remainder eagerly and fills one step_stats per member
+ * (stats of the final state).  Members must not be stepped or destroyed
+ * outside the batch while it exists.  A failed create returns a handle whose
+ * batch_last_error is set.  No counterpart in the reference ABI; for
+ * parameter sweeps that run many small problems on one GPU. */
+typedef struct fullmag_fdm_batch fullmag_fdm_batch;
+fullmag_fdm_batch *fullmag_fdm_ext_batch_create(fullmag_fdm_backend *const *handles, uint32_t count,
+                                                const double *dts, uint32_t steps_per_graph);
+int fullmag_fdm_ext_batch_run(fullmag_fdm_batch *batch, uint64_t n_steps, fullmag_fdm_step_stats *stats_out);
+const char *fullmag_fdm_ext_batch_last_error(fullmag_fdm_batch *batch);
+void fullmag_fdm_ext_batch_destroy(fullmag_fdm_batch *batch);
+
+/* Launch-schedule autotuner: times (demag + stats) evaluations of the
+ * current state under candidate schedules of the demag transforms and cell
+ * kernels (coordinate search over the FULLMAG_FDM_TUNE launch-time keys),
+ * keeps the fastest and writes it as "key=value,..." into chosen.  Results
+ * of every schedule agree with the oracle; only speed differs. */
+int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen, uint64_t chosen_len);
+
+/* Applies a schedule string (as autotune writes it; FULLMAG_FDM_TUNE syntax)
+ * to the launch-time knobs of a backend, e.g. to reuse one tuning across
+ * backends of the same grid. */
+int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedule);
+
 /* Bytes of device memory the backend holds. */
 uint64_t fullmag_fdm_ext_device_bytes(fullmag_fdm_backend *handle);
 

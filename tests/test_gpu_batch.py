@@ -1,0 +1,74 @@
+"""Graph-captured batches (fullmag_fdm_ext_batch_*) and the launch-schedule
+autotuner.  A batch replays captured Heun steps; every member must end
+bit-identical to the same backend stepped eagerly, over several replays with
+eager work in between, and the autotuned schedule must still match the
+oracle."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = [((32, 32, 1), 5e-14, "mixed"), ((64, 32, 2), 1e-13, "mixed"), ((24, 20, 3), 8e-14, "double"),
+           ((16, 16, 8), 1e-13, "mixed")]
+
+
+def member(fa, cells, prec, seed):
+    n = cells[0] * cells[1] * cells[2]
+    m0 = fa.generate_random_unit_vectors(seed, n)
+    return fa.NativeFdmBackend(fa.FdmPlan(cells=cells, cell_size=(5e-9, 5e-9, 3e-9), initial_magnetization=m0,
+                                          saturation_magnetisation=8e5, exchange_stiffness=1.3e-11, damping=0.1,
+                                          external_field=(1e4, 2e3, 0.0), precision=prec, demag_kernel="device"))
+
+
+def test_batch_graph_matches_eager_stepping(fa):
+    batch_members = [member(fa, c, p, 40 + i) for i, (c, _, p) in enumerate(MEMBERS)]
+    eager = [member(fa, c, p, 40 + i) for i, (c, _, p) in enumerate(MEMBERS)]
+    dts = [dt for _, dt, _ in MEMBERS]
+    run = fa.BatchRun(batch_members, dts, steps_per_graph=4)
+    total = 0
+    for n_steps in (9, 4, 13):  # graph replays plus an eager remainder, three times
+        stats = run.run(n_steps)
+        total += n_steps
+        for b, e, dt, s in zip(batch_members, eager, dts, stats):
+            for _ in range(n_steps):
+                r = e.step(dt)
+            np.testing.assert_array_equal(b.copy_m(), e.copy_m())
+            assert s.step == total and s.e_total == r.e_total
+            assert abs(s.time - r.time) <= 1e-12 * r.time
+        # eager work on the members between replays
+        for b in batch_members:
+            b.copy_h_demag()
+    run.close()
+    for b in batch_members + eager:
+        b.close()
+
+
+def test_batch_rejects_bad_members(fa):
+    b = member(fa, (8, 8, 1), "mixed", 3)
+    with pytest.raises(fa.RunError):
+        fa.BatchRun([b, b], [1e-13, 1e-13])
+    with pytest.raises(fa.RunError):
+        fa.BatchRun([b], [0.0])
+    b.close()
+
+
+@pytest.mark.parametrize("cells", [(64, 32, 8), (48, 40, 1)])
+def test_autotuned_schedule_matches_oracle(fa, oracle_mod, cells):
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(7 + n, n)
+    b = fa.NativeFdmBackend(fa.FdmPlan(cells=cells, cell_size=(4e-9, 4e-9, 2e-9), initial_magnetization=m0,
+                                       saturation_magnetisation=8.6e5, exchange_stiffness=1.3e-11, damping=0.05,
+                                       precision="double", demag_kernel="device"))
+    chosen = b.autotune(reps=2)
+    assert "y_pipe=" in chosen and "fixed=" in chosen
+    cfg = oracle_mod.OracleConfig(nx=nx, ny=ny, nz=nz, dx=4e-9, dy=4e-9, dz=2e-9, ms=8.6e5, a=1.3e-11, alpha=0.05)
+    cpu = oracle_mod.OracleProblem(cfg, m0)
+    hd = cpu.observe()["h_demag"]
+    np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+    for _ in range(3):
+        b.step(5e-14)
+        cpu.step(5e-14)
+    assert float(np.abs(b.copy_m() - cpu.m).max()) <= 1e-12
+    b.close()
