@@ -194,6 +194,14 @@ def lib():
             raise LibraryNotBuilt(
                 f"{LIB_PATH} is missing: build the HIP extension first (python -c "
                 f"'import __graft_entry__ as g; g.build()')")
+        # torch wheels bundle their own HIP runtime under the same SONAME
+        # (libamdhip64.so.7); whichever loads first serves the whole process.
+        # Load torch's first when torch is installed, so that tensors and this
+        # library share one runtime (device pointers pass between them).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

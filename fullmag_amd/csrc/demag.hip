@@ -620,11 +620,12 @@ struct ZLdsArgs {
     KernelView<Real> K;
 };
 
-template <typename Real>
+template <typename Real, int N = 0>
 __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
+    using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
+    const int n = L::n(a.lp), ls = n + kPad, T = L::threads(a.lp);
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int ncomp = a.conv ? 3 : 1;
     const int cbase = a.conv ? 0 : blockIdx.y;
@@ -636,10 +637,10 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
         const int64_t col = col0 + (s - c * a.tile);
         cplx<Real> v = {Real(0), Real(0)};
         if (e < a.n_in && col < a.columns) v = a.data[(cbase + c) * a.comp_stride + e * a.zstride + col];
-        lds[s * ls + lslot(a.lp, e)] = v;
+        lds[s * ls + L::slot(a.lp, e)] = v;
     }
     __syncthreads();
-    line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
     if (a.conv) {
         for (int idx = tid; idx < a.tile * n; idx += nthr) {
             const int j = idx % a.tile, e = idx / a.tile;
@@ -648,17 +649,17 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
             const int kx = (int)(col / a.py), ky = (int)(col - (int64_t)kx * a.py);
             Real t[6];
             tensor_at(a.K, kx, ky, e, t);
-            const int se = lslot(a.lp, e);
+            const int se = L::slot(a.lp, e);
             apply_tensor(t, lds[j * ls + se], lds[(a.tile + j) * ls + se], lds[(2 * a.tile + j) * ls + se]);
         }
         __syncthreads();
-        line_fft(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
+        L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
     }
     for (int idx = tid; idx < nslots * a.n_out; idx += nthr) {
         const int s = idx % nslots, e = idx / nslots;
         const int c = s / a.tile;
         const int64_t col = col0 + (s - c * a.tile);
-        if (col < a.columns) a.data[(cbase + c) * a.comp_stride + e * a.zstride + col] = lds[s * ls + lslot(a.lp, e)];
+        if (col < a.columns) a.data[(cbase + c) * a.comp_stride + e * a.zstride + col] = lds[s * ls + L::slot(a.lp, e)];
     }
 }
 
@@ -898,9 +899,12 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
                 za.py = dp.py;
                 za.K = K;
-                hipLaunchKernelGGL((zlds_kernel<Real>), dim3((unsigned)cdiv(za.columns, za.tile)),
-                                   dim3((unsigned)(3 * za.tile * dp.lz.threads)),
-                                   (size_t)3 * za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+                with_line(dp.lz, dp.tune.fixed != 0, [&](auto NC) {
+                    hipLaunchKernelGGL((zlds_kernel<Real, decltype(NC)::value>),
+                                       dim3((unsigned)cdiv(za.columns, za.tile)),
+                                       dim3((unsigned)(3 * za.tile * dp.lz.threads)),
+                                       (size_t)3 * za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+                });
             }
         }
         {

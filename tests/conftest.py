@@ -34,3 +34,15 @@ def oracle_mod():
 
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def fa():
+    """The HIP library's Python mirror; GPU tests only (fails loudly without
+    the built library or a gfx950 device)."""
+    import fullmag_amd
+    from fullmag_amd import _abi
+
+    _abi.lib()
+    assert fullmag_amd.is_available(), "no gfx950 device visible"
+    return fullmag_amd

@@ -11,11 +11,11 @@ DT = 1e-13
 
 
 @pytest.fixture(scope="module")
-def torch_cuda():
+def torch_cuda(fa):
     import torch
 
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+    assert torch.cuda.is_available(), (f"torch {torch.__version__} sees no GPU (hip {torch.version.hip}, "
+                                       f"device_count {torch.cuda.device_count()})")
     return torch
 
 

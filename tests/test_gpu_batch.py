@@ -66,9 +66,11 @@ def test_autotuned_schedule_matches_oracle(fa, oracle_mod, cells):
     cfg = oracle_mod.OracleConfig(nx=nx, ny=ny, nz=nz, dx=4e-9, dy=4e-9, dz=2e-9, ms=8.6e5, a=1.3e-11, alpha=0.05)
     cpu = oracle_mod.OracleProblem(cfg, m0)
     hd = cpu.observe()["h_demag"]
-    np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+    # other FFT factorisations round differently: 1e-11 of max|H| (the
+    # trajectory bar is max|dm| <= 1e-10)
+    np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max(), err_msg=chosen)
     for _ in range(3):
         b.step(5e-14)
         cpu.step(5e-14)
-    assert float(np.abs(b.copy_m() - cpu.m).max()) <= 1e-12
+    assert float(np.abs(b.copy_m() - cpu.m).max()) <= 1e-12, chosen
     b.close()

@@ -21,17 +21,6 @@ pytestmark = pytest.mark.gpu
 MU0 = 4.0 * math.pi * 1e-7
 
 
-@pytest.fixture(scope="module")
-def fa():
-    import fullmag_amd
-
-    from fullmag_amd import _abi
-
-    _abi.lib()
-    assert fullmag_amd.is_available(), "no gfx950 device visible"
-    return fullmag_amd
-
-
 def make_pair(fa, O, *, cells, cell, ms, a, alpha, m0, h_ext=None, exchange=True, demag=True,
               integrator="heun", precision="double", mask=None, kernel="spectra", disable_precession=False,
               adaptive=None, ms_cell=None):
@@ -244,7 +233,9 @@ def test_demag_schedule_variants_match_oracle(fa, oracle_mod, monkeypatch, cells
     gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 4e-9, 2e-9), ms=8.6e5, a=1.3e-11, alpha=0.05,
                          m0=m0, h_ext=(1e4, 0.0, 2e3), kernel="device")
     hd = cpu.observe()["h_demag"]
-    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+    # other FFT factorisations round differently: 1e-11 of max|H| (the
+    # trajectory bar is max|dm| <= 1e-10)
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max())
     for _ in range(3):
         gpu.step(5e-14)
         cpu.step(5e-14)
