@@ -28,16 +28,20 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe).
+// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, x_pipe, y_pad, zc).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
     int z_lds = 0;   // 1: LDS z kernel even when the register path applies
     int fixed = 1;   // 0: runtime-planned line kernels even for fixed schedules
-    int z_quad = 0;  // 1: four lanes per z column instead of two (even nz)
+    int z_quad = -1;  // 1: four lanes per z column instead of two (even nz); -1: FP64 only
     int xcd = 1;     // 0: no per-XCD renumbering of workgroups
-    int z_mirror = 1;  // 0: two-lane z kernel without mirrored-column pairing
+    int z_mirror = 0;  // 1: two-lane z kernel pairs mirrored ky columns in one workgroup
     int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
+    int z_stage = 1;   // 1: register z convolution staged through LDS tiles
+    int x_pipe = 0;    // row blocks per workgroup of the pipelined x kernels (0: off)
+    int y_pad = 0;     // complex slots of padding after each kx row of the 3-D spectrum
+    int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)
 };
 
 // Applies "key=value,..." (unknown keys ignored) on top of t.

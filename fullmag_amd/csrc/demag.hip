@@ -140,6 +140,8 @@ struct XArgs {
     int n_out;        // inverse: outputs stored per row
     Real scale;       // inverse: 1/Npad
     SpecGeom g;
+    int iters;        // pipelined kernels: row blocks per workgroup
+    int yblocks;      // row blocks per (c, z) plane: ceil(rows / xr)
 };
 
 // Ms m at (c, z, y, x) of the state; zero on inactive cells.
@@ -274,6 +276,124 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
         const cplx<Real> v = line[L::slot(a.lp, x, lx)];
         if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
         if (ya + 1 < a.rows) sink(c, z, ya + 1, x, v.y * a.scale);
+    }
+}
+
+// Software-pipelined x passes for fixed power-of-two lengths N: a workgroup
+// walks `iters` consecutive row blocks of one (c, z) plane; the global loads
+// of the next block are issued into registers before the current block's
+// LDS transform, so memory traffic overlaps the FFT.  Per block the work is
+// exactly that of xfwd_kernel / xinv_kernel.
+template <typename Real, class Src, int N>
+__global__ void __launch_bounds__(kMaxBlock) xfwd_pipe_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
+    using L = LineOps<N>;
+    constexpr int T = N / 16, NS = N / T, NI = (N / 2 + T - 1) / T;  // slots / loaded slots per lane
+    extern __shared__ unsigned char smem_raw[];
+    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
+    const int ls = a.ls, tid = threadIdx.x, nthr = blockDim.x;
+    const int jl = tid / T, t = tid - jl * T, lx = L::line_xor(jl);
+    cplx<Real> *line = lds + jl * ls;
+    const int z = blockIdx.y, c = blockIdx.z, b0 = blockIdx.x * a.iters;
+    cplx<Real> buf[NI];
+    auto fetch = [&](int yb) {
+        const int ya = yb * a.xr + 2 * jl;
+        const bool live = yb < a.yblocks;
+#pragma unroll
+        for (int e = 0; e < NI; ++e) {
+            const int x = t + e * T;
+            cplx<Real> v = {Real(0), Real(0)};
+            if (live && x < a.n_in) {
+                if (ya < a.rows) v.x = src(c, z, ya, x);
+                if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
+            }
+            buf[e] = v;
+        }
+    };
+    fetch(b0);
+    const int hx = a.g.hx;
+    for (int it = 0; it < a.iters; ++it) {
+        const int yb = b0 + it;
+#pragma unroll
+        for (int e = 0; e < NS; ++e)
+            line[L::slot(a.lp, t + e * T, lx)] = e < NI ? buf[e] : cplx<Real>{Real(0), Real(0)};
+        __syncthreads();
+        if (it + 1 < a.iters) fetch(yb + 1);
+        L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
+        if (yb < a.yblocks) {
+            const int y0 = yb * a.xr;
+            for (int idx = tid; idx < hx * a.xr; idx += nthr) {
+                const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
+                const int y = y0 + yl;
+                if (y >= a.rows) continue;
+                const int j = yl >> 1, jx = L::line_xor(j);
+                const cplx<Real> zk = lds[j * ls + L::slot(a.lp, k, jx)];
+                const cplx<Real> zn = lds[j * ls + L::slot(a.lp, k == 0 ? 0 : N - k, jx)];
+                const Real h = Real(0.5);
+                cplx<Real> v;
+                if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
+                else v = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
+                spec[a.g.at(c, z, k, y)] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename Real, class Sink, int N>
+__global__ void __launch_bounds__(kMaxBlock) xinv_pipe_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
+    using L = LineOps<N>;
+    constexpr int T = N / 16;
+    constexpr int MI = (N / 2 + 1 + T - 1) / T;  // (kx, line) pairs per lane: ceil(hx / T)
+    extern __shared__ unsigned char smem_raw[];
+    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
+    const int LINES = a.xr >> 1;
+    const int ls = a.ls, tid = threadIdx.x, nthr = blockDim.x;
+    const int z = blockIdx.y, c = blockIdx.z, b0 = blockIdx.x * a.iters;
+    const int hx = a.g.hx;
+    cplx<Real> bA[MI], bB[MI];
+    auto fetch = [&](int yb) {
+        const bool live = yb < a.yblocks;
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            const int idx = tid + i * nthr;
+            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+            const int ya = yb * a.xr + 2 * j;
+            cplx<Real> A = {Real(0), Real(0)}, B = {Real(0), Real(0)};
+            if (live && idx < hx * LINES) {
+                if (ya < a.rows) A = spec[a.g.at(c, z, k, ya)];
+                if (ya + 1 < a.rows) B = spec[a.g.at(c, z, k, ya + 1)];
+            }
+            bA[i] = A;
+            bB[i] = B;
+        }
+    };
+    fetch(b0);
+    const int jl = tid / T, t = tid - jl * T, lx = L::line_xor(jl);
+    const cplx<Real> *line = lds + jl * ls;
+    for (int it = 0; it < a.iters; ++it) {
+        const int yb = b0 + it;
+#pragma unroll
+        for (int i = 0; i < MI; ++i) {
+            const int idx = tid + i * nthr;
+            if (idx < hx * LINES) {
+                const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1), jx = L::line_xor(j);
+                const cplx<Real> A = bA[i], B = bB[i];
+                lds[j * ls + L::slot(a.lp, k, jx)] = {A.x - B.y, A.y + B.x};
+                if (k > 0 && k < N - k) lds[j * ls + L::slot(a.lp, N - k, jx)] = {A.x + B.y, B.x - A.y};
+            }
+        }
+        __syncthreads();
+        if (it + 1 < a.iters) fetch(yb + 1);
+        L::template fft<Real>(lds + jl * ls, a.lp, t, a.tw, true, lx);
+        if (yb < a.yblocks) {
+            const int ya = yb * a.xr + 2 * jl;
+            for (int x = t; x < a.n_out; x += T) {
+                const cplx<Real> v = line[L::slot(a.lp, x, lx)];
+                if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
+                if (ya + 1 < a.rows) sink(c, z, ya + 1, x, v.y * a.scale);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -416,9 +536,10 @@ struct ZArgs {
     cplx<Real> *data;
     const cplx<Real> *tw;  // roots of length pz
     int nz, hx, py;
-    int64_t columns;       // hx * py
-    int64_t zstride;       // hx * py (3-D demag layout)
-    int64_t comp_stride;   // nz * hx * py
+    int64_t columns;       // hx * pitch (pad columns ky >= py are skipped)
+    int64_t zstride;       // hx * pitch (3-D demag layout)
+    int64_t comp_stride;   // nz * hx * pitch
+    int pitch;             // distance between kx rows of the spectrum (>= py)
     int xcd;               // renumber workgroups per XCD
     int half, groups;      // py / 2; groups of 16 direct ky (0..half) per kx
     KernelView<Real> K;
@@ -479,10 +600,11 @@ __device__ __forceinline__ void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, 
 // NZ non-zero inputs exist); tensor at its own kz; inverse:
 // y_z = IDFT_NZ(Y_even)[z] + w_pz^(-z) IDFT_NZ(Y_odd)[z], z < NZ, summed
 // across the lane pair with a shuffle.  All three components in registers.
-// Lane layout: wave w of kx row w / groups holds the 16 direct columns
-// ky = 16 g + i (g = w mod groups, ky <= py/2) and their mirrors py - ky, so
-// a direct column and its mirror read the same tensor octant entries in one
-// instruction; lane = 4 i + 2 mirror + p.
+// Lane layout (groups > 0): workgroup g of kx row blk / groups holds the 64
+// direct columns ky = 64 g + 32 (wave & 1) + lane / 2 (ky <= py/2) in waves
+// 0-1 and their mirrors py - ky in waves 2-3; a column and its mirror read
+// the same tensor octant entries.  groups = 0: consecutive lane pairs take
+// consecutive columns.
 template <typename Real, int NZ>
 __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,20 +613,24 @@ __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
     int kx, ky;
     int64_t col;
     if (a.groups > 0) {
-        const int64_t wave = gid >> 6;
-        const int lane = (int)(threadIdx.x & 63);
-        const int mir = (lane >> 1) & 1, i = lane >> 2;
-        kx = (int)(wave / a.groups);
-        const int kd = 16 * (int)(wave - (int64_t)kx * a.groups) + i;
+        // workgroup = one kx and 64 direct ky (waves 0-1) plus their mirrors
+        // (waves 2-3): full 128-byte tensor lines per instruction, and the
+        // mirrored waves re-read them from the workgroup's L1
+        const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        const int w = (int)(threadIdx.x >> 6), i = (int)((threadIdx.x & 63) >> 1);
+        const int mir = w >> 1;
+        kx = (int)(blk / (uint32_t)a.groups);
+        const int kd = 64 * (int)(blk - (uint32_t)kx * a.groups) + 32 * (w & 1) + i;
         valid = kx < a.hx && kd <= a.half && (!mir || (kd >= 1 && kd < a.half));
         ky = valid ? (mir ? a.py - kd : kd) : 0;
-        col = valid ? (int64_t)kx * a.py + ky : 0;
+        col = valid ? (int64_t)kx * a.pitch + ky : 0;
     } else {
         // consecutive lane pairs take consecutive columns
         col = gid >> 1;
         valid = col < a.columns;
-        kx = valid ? (int)(col / a.py) : 0;
-        ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+        kx = valid ? (int)(col / a.pitch) : 0;
+        ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+        valid = valid && ky < a.py;
     }
     cplx<Real> v[3][NZ];
 #pragma unroll
@@ -543,6 +669,108 @@ __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
             if ((z & 1) == p) a.data[c * a.comp_stride + z * a.zstride + col] = v[c][z];
 }
 
+// The two-lane z convolution staged through LDS: a workgroup of 256 lanes
+// owns 128 consecutive columns.  It loads the tile's 3 NZ rows of 128
+// complex values (one row = one (component, z) plane segment) with 16-byte
+// loads, all issued before any arithmetic; lane pair j computes column j
+// from LDS exactly as zconv_pair_kernel does; the results go back through
+// LDS and out with 16-byte stores.  The tile's whole input is in flight at
+// once and no lane keeps loaded data in registers across the transform.
+constexpr int kZTile = 128;
+
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+    using C = cplx<Real>;
+    constexpr int ROWS = 3 * NZ;
+    constexpr int V = sizeof(Real) == 4 ? 2 : 1;  // complex values per 16-byte access
+    constexpr int UNITS = kZTile / V;             // accesses per row
+    __shared__ C tile[ROWS][kZTile + V];
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int64_t col0 = (int64_t)blk * kZTile;
+    const int t = (int)threadIdx.x;
+    // load: row r = (c, z) at data[c comp_stride + z zstride + col0 ...]
+#pragma unroll
+    for (int i = 0; i < (ROWS * UNITS + 255) / 256; ++i) {
+        const int idx = t + 256 * i;
+        if (idx < ROWS * UNITS) {
+            const int r = idx / UNITS, u = idx - r * UNITS;
+            const int c = r / NZ, z = r - c * NZ;
+            const int64_t col = col0 + (int64_t)u * V;
+            const C *src = a.data + c * a.comp_stride + z * a.zstride + col;
+            if (col + V <= a.columns) {
+                if constexpr (V == 2) {
+                    const float4 q = *reinterpret_cast<const float4 *>(src);
+                    tile[r][u * 2] = C{Real(q.x), Real(q.y)};
+                    tile[r][u * 2 + 1] = C{Real(q.z), Real(q.w)};
+                } else {
+                    tile[r][u] = *src;
+                }
+            } else {
+                for (int e = 0; e < V; ++e)
+                    tile[r][u * V + e] = col + e < a.columns ? src[e] : C{Real(0), Real(0)};
+            }
+        }
+    }
+    __syncthreads();
+    const int j = t >> 1, p = t & 1;
+    const int64_t col = col0 + j;
+    bool valid = col < a.columns;
+    const int kx = valid ? (int)(col / a.pitch) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+    valid = valid && ky < a.py;  // pad columns: transformed with a zero tensor, never read
+    C v[3][NZ];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            C x = tile[c * NZ + z][j];
+            if (p && z) x = cmul(x, a.tw[z]);
+            v[c][z] = x;
+        }
+        half_dft<Real, NZ>(v[c], a.tw, false);
+    }
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) {
+        Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+        if (valid) tensor_at(a.K, kx, ky, 2 * k + p, tt);
+        apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+    }
+    __syncthreads();  // every lane has read its inputs before the tile is overwritten
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        half_dft<Real, NZ>(v[c], a.tw, true);
+#pragma unroll
+        for (int z = 0; z < NZ; ++z) {
+            C y = v[c][z];
+            if (p && z) y = cmul(y, cconj(a.tw[z]));
+            const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+            if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < (ROWS * UNITS + 255) / 256; ++i) {
+        const int idx = t + 256 * i;
+        if (idx < ROWS * UNITS) {
+            const int r = idx / UNITS, u = idx - r * UNITS;
+            const int c = r / NZ, z = r - c * NZ;
+            const int64_t cl = col0 + (int64_t)u * V;
+            C *dst = a.data + c * a.comp_stride + z * a.zstride + cl;
+            if (cl + V <= a.columns) {
+                if constexpr (V == 2) {
+                    const C e0 = tile[r][u * 2], e1 = tile[r][u * 2 + 1];
+                    *reinterpret_cast<float4 *>(dst) = float4{float(e0.x), float(e0.y), float(e1.x), float(e1.y)};
+                } else {
+                    *dst = tile[r][u];
+                }
+            } else {
+                for (int e = 0; e < V; ++e)
+                    if (cl + e < a.columns) dst[e] = tile[r][u * V + e];
+            }
+        }
+    }
+}
+
 // z convolution with four lanes per (kx, ky) column (pz = 2 NZ = 4 M, NZ
 // even): lane q owns kz = 4k + q.  Forward: X[4k+q] = DFT_M(u)[k] with the
 // folded input u_z = x_z w^(q z) + x_(z+M) w^(q (z+M)), z < M; tensor at its
@@ -558,9 +786,10 @@ __global__ void __launch_bounds__(256) zconv_quad_kernel(ZArgs<Real> a) {
     const int64_t gid = (int64_t)blk * blockDim.x + threadIdx.x;
     const int64_t col = gid >> 2;
     const int q = (int)(gid & 3);
-    const bool valid = col < a.columns;  // all four lanes of a quad stay for the shuffles
-    const int kx = valid ? (int)(col / a.py) : 0;
-    const int ky = valid ? (int)(col - (int64_t)kx * a.py) : 0;
+    bool valid = col < a.columns;  // all four lanes of a quad stay for the shuffles
+    const int kx = valid ? (int)(col / a.pitch) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+    valid = valid && ky < a.py;
     // w^(q z), z < NZ: one gather per z, reused by all components and the inverse
     cplx<Real> wq[NZ];
 #pragma unroll
@@ -616,7 +845,7 @@ struct ZLdsArgs {
     int n_in, n_out;
     int tile, conv;
     int64_t columns, zstride, comp_stride;
-    int py;
+    int py, pitch;
     KernelView<Real> K;
 };
 
@@ -646,7 +875,8 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
             const int j = idx % a.tile, e = idx / a.tile;
             const int64_t col = col0 + j;
             if (col >= a.columns) continue;
-            const int kx = (int)(col / a.py), ky = (int)(col - (int64_t)kx * a.py);
+            const int kx = (int)(col / a.pitch), ky = (int)(col - (int64_t)kx * a.pitch);
+            if (ky >= a.py) continue;
             Real t[6];
             tensor_at(a.K, kx, ky, e, t);
             const int se = L::slot(a.lp, e);
@@ -823,13 +1053,25 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
     const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
     const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
+    xa.yblocks = (int)cdiv(dp.ny, dp.xr);
+    xa.iters = std::max(1, dp.tune.x_pipe);
+    const dim3 xpgrid((unsigned)cdiv(xa.yblocks, xa.iters), (unsigned)dp.nz, 3);
+    const bool xpipe = dp.tune.x_pipe > 0;
     {
         MagSource<Real, S> ms{src, c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
                               dp.nx, dp.ny, dp.nz, (int64_t)c.n};
         Timed t(c, "demag_x_r2c");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
-            hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, decltype(NC)::value>), xgrid, xblock, xlds,
-                               c.stream, xa, ms, spec);
+            constexpr int N = decltype(NC)::value;
+            if constexpr (N >= 32) {
+                if (xpipe) {
+                    hipLaunchKernelGGL((xfwd_pipe_kernel<Real, MagSource<Real, S>, N>), xpgrid, xblock, xlds,
+                                       c.stream, xa, ms, spec);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, N>), xgrid, xblock, xlds, c.stream, xa, ms,
+                               spec);
         });
     }
     const KernelView<Real> K = kview<Real>(c);
@@ -839,7 +1081,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
         la.lp = dp.ly;
         la.lines = 3ll * dp.nz * dp.hx;
-        la.line_pitch = dp.py;
+        la.line_pitch = dp.rows;
         la.lpw = dp.y_lines_per_wg;
         {
             la.n_in = dp.ny;
@@ -857,19 +1099,31 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.nz = dp.nz;
                 za.hx = dp.hx;
                 za.py = dp.py;
-                za.columns = (int64_t)dp.hx * dp.py;
-                za.zstride = (int64_t)dp.hx * dp.py;
-                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
+                za.pitch = dp.rows;
+                za.columns = (int64_t)dp.hx * dp.rows;
+                za.zstride = (int64_t)dp.hx * dp.rows;
+                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.rows;
                 za.K = K;
                 za.xcd = dp.tune.xcd;
                 za.half = dp.py / 2;
-                za.groups = dp.tune.z_mirror ? (int)cdiv(za.half + 1, 16) : 0;
-                const dim3 zg((unsigned)(za.groups ? cdiv((uint64_t)dp.hx * za.groups * 64, 256)
-                                                   : cdiv(2 * za.columns, 256))),
+                za.groups = dp.tune.z_mirror ? (int)cdiv(za.half + 1, 64) : 0;
+                const dim3 zg((unsigned)(za.groups ? (uint64_t)dp.hx * za.groups : cdiv(2 * za.columns, 256))),
                     zb(256);
                 const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
                 const bool quad = dp.tune.z_quad && (dp.nz == 4 || dp.nz == 6 || dp.nz == 8);
-                if (quad) {
+                if (dp.tune.z_stage && !quad) {
+                    const dim3 zt((unsigned)cdiv(za.columns, kZTile));
+                    switch (dp.nz) {
+                        case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1>), zt, zb, 0, c.stream, za); break;
+                        case 2: hipLaunchKernelGGL((zconv_tile_kernel<Real, 2>), zt, zb, 0, c.stream, za); break;
+                        case 3: hipLaunchKernelGGL((zconv_tile_kernel<Real, 3>), zt, zb, 0, c.stream, za); break;
+                        case 4: hipLaunchKernelGGL((zconv_tile_kernel<Real, 4>), zt, zb, 0, c.stream, za); break;
+                        case 5: hipLaunchKernelGGL((zconv_tile_kernel<Real, 5>), zt, zb, 0, c.stream, za); break;
+                        case 6: hipLaunchKernelGGL((zconv_tile_kernel<Real, 6>), zt, zb, 0, c.stream, za); break;
+                        case 7: hipLaunchKernelGGL((zconv_tile_kernel<Real, 7>), zt, zb, 0, c.stream, za); break;
+                        default: hipLaunchKernelGGL((zconv_tile_kernel<Real, 8>), zt, zb, 0, c.stream, za); break;
+                    }
+                } else if (quad) {
                     switch (dp.nz) {
                         case 4: hipLaunchKernelGGL((zconv_quad_kernel<Real, 4>), zq, zb, 0, c.stream, za); break;
                         case 6: hipLaunchKernelGGL((zconv_quad_kernel<Real, 6>), zq, zb, 0, c.stream, za); break;
@@ -894,10 +1148,11 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.n_out = dp.nz;
                 za.tile = dp.z_tile;
                 za.conv = 1;
-                za.columns = (int64_t)dp.hx * dp.py;
-                za.zstride = (int64_t)dp.hx * dp.py;
-                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.py;
+                za.columns = (int64_t)dp.hx * dp.rows;
+                za.zstride = (int64_t)dp.hx * dp.rows;
+                za.comp_stride = (int64_t)dp.nz * dp.hx * dp.rows;
                 za.py = dp.py;
+                za.pitch = dp.rows;
                 za.K = K;
                 with_line(dp.lz, dp.tune.fixed != 0, [&](auto NC) {
                     hipLaunchKernelGGL((zlds_kernel<Real, decltype(NC)::value>),
@@ -933,8 +1188,16 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
         Timed t(c, "demag_x_c2r");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
-            hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, decltype(NC)::value>), xgrid, xblock, xlds,
-                               c.stream, xa, spec, sink);
+            constexpr int N = decltype(NC)::value;
+            if constexpr (N >= 32) {
+                if (xpipe) {
+                    hipLaunchKernelGGL((xinv_pipe_kernel<Real, FieldSink<Real>, N>), xpgrid, xblock, xlds,
+                                       c.stream, xa, spec, sink);
+                    return;
+                }
+            }
+            hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream, xa, spec,
+                               sink);
         });
     }
 }
@@ -986,6 +1249,10 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "xcd") t.xcd = v;
             else if (k == "z_mirror") t.z_mirror = v;
             else if (k == "y_pipe") t.y_pipe = v;
+            else if (k == "z_stage") t.z_stage = v;
+            else if (k == "x_pipe") t.x_pipe = v;
+            else if (k == "y_pad") t.y_pad = v;
+            else if (k == "zc") t.zc = v;
         }
         pos = end + 1;
     }
@@ -1068,6 +1335,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
             za.zstride = (int64_t)dp.hx * dp.py;
             za.comp_stride = (int64_t)dp.pz * dp.hx * dp.py;
             za.py = dp.py;
+            za.pitch = dp.py;
             if ((size_t)za.tile * (dp.lz.n + kPad) * ce > lds_budget()) {
                 err = "Newell setup: z line does not fit in LDS";
                 ok = false;
@@ -1235,11 +1503,16 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     dp.py = 2 * c.ny;
     dp.pz = dp.three_d ? 2 * c.nz : 1;
     dp.hx = c.nx + 1;
-    dp.rows = dp.three_d ? dp.py : dp.ny;
+    // 3-D: kx rows of the spectrum are padded by y_pad complex slots so the
+    // x passes' transposed accesses are not a power-of-two stride apart
+    dp.rows = dp.three_d ? dp.py + std::max(0, dp.tune.y_pad) : dp.ny;
     dp.kcomp = dp.three_d ? 6 : 4;
     dp.kz_n = dp.three_d ? dp.nz + 1 : 1;
     dp.inv_npad = 1.0 / ((double)dp.px * dp.py * dp.pz);
     parse_tuning(dp.tune);
+    // measured at 512x512x8: four lanes per column are faster in FP64 (half
+    // the registers per lane), the LDS-staged two-lane tile in FP32
+    if (dp.tune.z_quad < 0) dp.tune.z_quad = c.mode == MODE_DOUBLE ? 1 : 0;
     if (!plan_lines(dp, c.last_error)) return false;
     if (c.mode == MODE_DOUBLE) return setup_t<double>(c, plan);
     return setup_t<float>(c, plan);

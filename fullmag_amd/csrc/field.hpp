@@ -357,8 +357,9 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
 }
 
 // Column tiling of the stage kernels: tile width TX = 64 (or the smallest
-// power of two >= nx), TY = kBlock / TX rows; z split into chunks so that the
-// grid has about 2048 workgroups.
+// power of two >= nx), TY = kBlock / TX rows; z is split into chunks only
+// when the tiles alone give fewer than about 1024 workgroups (longer chunks
+// re-read fewer halo planes: measured faster at 512x512x8).
 void stage_tiling(const Context &c, int &tx_log2, int &tiles_x, int &tiles_y, int &zc, int &nchunks) {
     tx_log2 = 0;
     while ((1u << tx_log2) < c.nx && tx_log2 < 6) ++tx_log2;
@@ -366,8 +367,8 @@ void stage_tiling(const Context &c, int &tx_log2, int &tiles_x, int &tiles_y, in
     tiles_x = (int)cdiv(c.nx, tx);
     tiles_y = (int)cdiv(c.ny, ty);
     const uint64_t tiles = (uint64_t)tiles_x * tiles_y;
-    const int want = (int)std::min<uint64_t>(c.nz, std::max<uint64_t>(1, cdiv(2048, tiles)));
-    zc = (int)cdiv(c.nz, want);
+    const int want = (int)std::min<uint64_t>(c.nz, std::max<uint64_t>(1, cdiv(1024, tiles)));
+    zc = c.dp.tune.zc > 0 ? std::min<int>(c.dp.tune.zc, (int)c.nz) : (int)cdiv(c.nz, want);
     nchunks = (int)cdiv(c.nz, zc);
 }
 

@@ -485,7 +485,12 @@ bool alloc_integrator_buffers(Context &c) {
     for (int i = 0; i < nmt; ++i)
         if (!ctx_alloc(c, &c.mt[i], rb, "stage point")) return false;
     if (need_mst && !ctx_alloc(c, &c.mst, sb, "candidate state")) return false;
-    c.max_partial_blocks = (int)std::max<uint64_t>(stage_blocks(c), cdiv(c.n, kBlock));
+    {
+        // any z chunking (down to one plane per workgroup) fits
+        int tl, tx, ty, zc, nch;
+        stage_tiling(c, tl, tx, ty, zc, nch);
+        c.max_partial_blocks = (int)std::max<uint64_t>((uint64_t)tx * ty * c.nz, cdiv(c.n, kBlock));
+    }
     if (!ctx_alloc(c, (void **)&c.d_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double), "partials"))
         return false;
     if (!ctx_alloc(c, (void **)&c.d_result, 16 * sizeof(double), "result")) return false;
