@@ -28,7 +28,7 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, x_pipe, y_pad, zc).
+// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, x_pipe, y_pad, zc, yz_slab).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
@@ -42,6 +42,7 @@ struct DemagTuning {
     int x_pipe = 0;    // row blocks per workgroup of the pipelined x kernels (0: off)
     int y_pad = 0;     // complex slots of padding after each kx row of the 3-D spectrum
     int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)
+    int yz_slab = 0;   // 1: fused y transform + z convolution per kx slab where eligible
 };
 
 // Applies "key=value,..." (unknown keys ignored) on top of t.
@@ -66,7 +67,7 @@ struct DemagPlan {
     int yconv_kpw = 1;           // thin-film y convolution: kx columns per workgroup
     int z_tile = 1;              // long z lines: columns per workgroup
     // device buffers (element type depends on Real)
-    void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
+    void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr, *tw_p = nullptr;  // tw_p: length py / 4
     void *spec = nullptr;
     void *kern = nullptr;
     uint64_t spec_elems = 0;  // complex elements

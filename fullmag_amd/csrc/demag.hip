@@ -26,6 +26,8 @@
 // kern[comp][kz][kx][ky] (kz <= nz, ky <= ny) and mirrored with parity signs.
 #include "context.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -893,6 +895,227 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fused y transform + z convolution + inverse y transform of one kx slab
+// ---------------------------------------------------------------------------
+// With py = 2 ny and the y input zero beyond ny, the ky = S k + s outputs of
+// the length-py transform are a length-P = py / S transform of the folded
+// input u_s[n] = sum_{b < S/2} x[n + b P] w^(s (n + b P)) (w = exp(-2 pi i /
+// py)), and the first ny outputs of the inverse are
+//   y[n] = sum_s w^(-s n) IDFT_P(Y_s)[n mod P].
+// So one workgroup owns a whole kx slab (3 components x nz planes x ny) and
+// walks the S classes: fold into LDS, forward P-point transforms of the 3 nz
+// rows, the register z convolution of each of the P columns (two lanes per
+// column, as zconv_pair_kernel), inverse transforms, and accumulation of
+// w^(-s n) IDFT_P into registers; the slab is written back once.  Per demag
+// evaluation this replaces the separate y-forward, z and y-inverse passes
+// (three round trips of the 2 ny spectrum) with one read and one write of
+// the ny spectrum and one read of the tensor.
+template <typename Real>
+struct SlabArgs {
+    cplx<Real> *data;          // spec[c][z][kx][pitch], first ny used
+    const cplx<Real> *tw_y;    // roots of length py
+    const cplx<Real> *tw_p;    // roots of length P
+    const cplx<Real> *tw_z;    // roots of length pz
+    int hx, ny, py, pitch;
+    int64_t zstride;           // hx * pitch
+    int64_t comp_stride;       // nz * hx * pitch
+    KernelView<Real> K;
+};
+
+// Fixed power-of-two line transform where only lanes with `active` own a
+// line; every lane of the workgroup reaches the barriers.
+template <typename T, int N, int R, int NS>
+__device__ __forceinline__ void guarded_pass(cplx<T> *line, int t, const cplx<T> *tw, bool inv, bool active, int lx) {
+    constexpr int MB = lane_butterflies<R>();
+    constexpr int NT = N / 16, NR = N / R, ST = N / (NS * R);
+    cplx<T> v[MB][R];
+    int base[MB];
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            const int j = t + b * NT;
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = line[p2slot(j + r * NR) ^ lx];
+            if (NS > 1 && k != 0) twiddle<T, R>(v[b], tw, k * ST, inv);
+            dft_any<T, R>(v[b], tw, N, inv);
+            base[b] = (j / NS) * NS * R + k;
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int b = 0; b < MB; ++b)
+#pragma unroll
+            for (int r = 0; r < R; ++r) line[p2slot(base[b] + r * NS) ^ lx] = v[b][r];
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int P, int NS>
+__device__ __forceinline__ void guarded_passes(cplx<T> *line, int t, const cplx<T> *tw, bool inv, bool active,
+                                               int lx) {
+    if constexpr (P < p2_npass(N)) {
+        constexpr int R = p2_radix(N, P);
+        guarded_pass<T, N, R, NS>(line, t, tw, inv, active, lx);
+        guarded_passes<T, N, P + 1, NS * R>(line, t, tw, inv, active, lx);
+    }
+}
+
+// Bank rotation of row r of the slab class buffer (rows are P apart, a
+// multiple of the bank count): XOR the slot with 4 (r mod 8).
+template <int P>
+__device__ __forceinline__ int row_xor(int r) {
+    return P >= 32 ? ((r & 7) << 2) : 0;
+}
+
+template <typename Real, int NZ, int P, int W>
+__global__ void __launch_bounds__(W * P) yz_slab_kernel(SlabArgs<Real> a) {
+    using C = cplx<Real>;
+    constexpr int S = 4;                 // ky classes
+    constexpr int ROWS = 3 * NZ;         // (component, z) rows
+    constexpr int NT = W * P;            // threads (W = 2 or 4 per class column)
+    constexpr int NY = 2 * P;            // ny = S P / 2
+    constexpr int T = P / 16;            // lanes per row transform
+    constexpr bool QUAD = NZ % 2 == 0;   // four lanes per z column (else two)
+    static_assert(ROWS * T <= NT, "every row has its lanes");
+    extern __shared__ unsigned char smem_raw[];
+    C *acc = reinterpret_cast<C *>(smem_raw);  // [ROWS][NY] output accumulator
+    C *cls = acc + ROWS * NY;                  // [ROWS][P] one ky class
+    const int t = (int)threadIdx.x;
+    const int kx = (int)((blockIdx.x + (unsigned)a.hx - 1) % (unsigned)a.hx);  // the odd kx = nx first
+    const int64_t colbase = (int64_t)kx * a.pitch;
+    auto gaddr = [&](int r, int n) -> int64_t {
+        const int c = r / NZ, z = r - c * NZ;
+        return c * a.comp_stride + z * a.zstride + colbase + n;
+    };
+    const int frow = t / T, ft = t - frow * T;  // row transform lanes
+    const bool factive = frow < ROWS;
+    C *fline = cls + (factive ? frow : 0) * P;
+    const int flx = row_xor<P>(factive ? frow : 0);
+#pragma unroll 1
+    for (int s = 0; s < S; ++s) {
+        // fold class s from the slab in memory: u[n] = sum_b x[n + b P] w^(s (n + b P));
+        // thread t folds position n = t mod P of rows t / P + (NT / P) i
+        {
+            const int n = t & (P - 1);
+            const C w0 = a.tw_y[(s * n) & (a.py - 1)], w1 = a.tw_y[(s * (n + P)) & (a.py - 1)];
+#pragma unroll 2
+            for (int r = t / P; r < ROWS; r += NT / P) {
+                C u = a.data[gaddr(r, n)], x1 = a.data[gaddr(r, n + P)];
+                if (s) {
+                    u = cmul(u, w0);
+                    x1 = cmul(x1, w1);
+                }
+                cls[r * P + (p2slot(n) ^ row_xor<P>(r))] = cadd(u, x1);
+            }
+        }
+        __syncthreads();
+        guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, false, factive, flx);
+        if constexpr (QUAD) {
+            // four lanes per column (zconv_quad_kernel's folded transform)
+            constexpr int M = NZ / 2, PZ = 2 * NZ;
+            const int q = t & 3;
+            C wq[NZ];
+#pragma unroll
+            for (int z = 0; z < NZ; ++z) wq[z] = a.tw_z[(q * z) % PZ];
+#pragma unroll 1
+            for (int col = t >> 2; col < P; col += NT / 4) {
+            C v[3][M];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                for (int z = 0; z < M; ++z) {
+                    const int r0 = c * NZ + z, r1 = r0 + M;
+                    v[c][z] = cadd(cmul(cls[r0 * P + (p2slot(col) ^ row_xor<P>(r0))], wq[z]),
+                                   cmul(cls[r1 * P + (p2slot(col) ^ row_xor<P>(r1))], wq[z + M]));
+                }
+                strided_dft<Real, M, 4>(v[c], a.tw_z, false);
+            }
+            const int ky = S * col + s;
+#pragma unroll
+            for (int k = 0; k < M; ++k) {
+                Real tt[6];
+                tensor_at(a.K, kx, ky, 4 * k + q, tt);
+                apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+            }
+            // the quad (one wave) read its whole column above, so writes may follow
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                strided_dft<Real, M, 4>(v[c], a.tw_z, true);
+#pragma unroll
+                for (int z = 0; z < NZ; ++z) {
+                    C sum = cmul(v[c][z % M], cconj(wq[z]));
+                    sum.x += __shfl_xor(sum.x, 1);
+                    sum.y += __shfl_xor(sum.y, 1);
+                    sum.x += __shfl_xor(sum.x, 2);
+                    sum.y += __shfl_xor(sum.y, 2);
+                    const int r = c * NZ + z;
+                    if ((z & 3) == q) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = sum;
+                }
+            }
+            }
+        } else {
+            // two lanes per column (zconv_pair_kernel); lanes >= 2 P idle
+            const int col = (t >> 1) & (P - 1), p = t & 1;
+            const bool zactive = t < 2 * P;
+            C v[3][NZ];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                for (int z = 0; z < NZ; ++z) {
+                    const int r = c * NZ + z;
+                    C x = zactive ? cls[r * P + (p2slot(col) ^ row_xor<P>(r))] : C{Real(0), Real(0)};
+                    if (p && z) x = cmul(x, a.tw_z[z]);
+                    v[c][z] = x;
+                }
+                half_dft<Real, NZ>(v[c], a.tw_z, false);
+            }
+            const int ky = S * col + s;
+#pragma unroll
+            for (int k = 0; k < NZ; ++k) {
+                Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+                if (zactive) tensor_at(a.K, kx, ky, 2 * k + p, tt);
+                apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                half_dft<Real, NZ>(v[c], a.tw_z, true);
+#pragma unroll
+                for (int z = 0; z < NZ; ++z) {
+                    C y = v[c][z];
+                    if (p && z) y = cmul(y, cconj(a.tw_z[z]));
+                    const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+                    const int r = c * NZ + z;
+                    if (zactive && (z & 1) == p) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = cadd(y, o);
+                }
+            }
+        }
+        __syncthreads();
+        guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, true, factive, flx);
+        // accumulate w^(-s n) IDFT_P(Y_s)[n mod P]: thread t owns position
+        // n = t mod NY of rows t / NY + (NT / NY) i
+        {
+            const int n = t & (NY - 1);
+            const C w = cconj(a.tw_y[(s * n) & (a.py - 1)]);
+            const int sl = p2slot(n & (P - 1));
+#pragma unroll 4
+            for (int r = t / NY; r < ROWS; r += NT / NY) {
+                C y = cls[r * P + (sl ^ row_xor<P>(r))];
+                if (s) y = cmul(y, w);
+                acc[r * NY + n] = s ? cadd(acc[r * NY + n], y) : y;
+            }
+        }
+        __syncthreads();
+    }
+    {
+        const int n = t & (NY - 1);
+#pragma unroll 4
+        for (int r = t / NY; r < ROWS; r += NT / NY) a.data[gaddr(r, n)] = acc[r * NY + n];
+    }
+}
+
 // Octant extraction after the setup transform of layout spec[c][z][kx][ky]
 // (z < pz, ky < py): kern[c][oz][kx][oy] = Re spec[c][oz][kx][oy].
 template <typename Real>
@@ -1006,6 +1229,21 @@ void with_line(const LinePlan &lp, bool allow_fixed, F &&f) {
     f(std::integral_constant<int, 0>{});
 }
 
+// The fused slab path (yz_slab_kernel): 3-D, nz <= 8, py a power of two with
+// P = py / 4 in 32..256 and the slab's LDS within 64 KB.
+size_t slab_lds(const DemagPlan &dp, size_t ce) { return (size_t)3 * dp.nz * (3 * (dp.py / 4)) * ce; }  // acc + class
+
+bool slab_geometry(const DemagPlan &dp, size_t ce) {
+    if (!dp.three_d || dp.nz > 8 || dp.projection) return false;
+    if (dp.py & (dp.py - 1)) return false;
+    const int P = dp.py / 4;
+    return P >= 16 && P <= 256 && slab_lds(dp, ce) <= 160 * 1024;
+}
+
+bool slab_eligible(const DemagPlan &dp, size_t ce) {
+    return dp.tune.yz_slab && dp.tune.fixed && dp.tw_p && slab_geometry(dp, ce);
+}
+
 // One contiguous-line pass (the 3-D y transforms): the pipelined kernel for
 // fixed power-of-two schedules with n_in = N or N / 2, else line_kernel.
 template <typename Real>
@@ -1075,7 +1313,53 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         });
     }
     const KernelView<Real> K = kview<Real>(c);
-    if (dp.three_d) {
+    if (slab_eligible(dp, ce)) {
+        SlabArgs<Real> sa{};
+        sa.data = spec;
+        sa.tw_y = static_cast<const cplx<Real> *>(dp.tw_y);
+        sa.tw_p = static_cast<const cplx<Real> *>(dp.tw_p);
+        sa.tw_z = static_cast<const cplx<Real> *>(dp.tw_z);
+        sa.hx = dp.hx;
+        sa.ny = dp.ny;
+        sa.py = dp.py;
+        sa.pitch = dp.rows;
+        sa.zstride = (int64_t)dp.hx * dp.rows;
+        sa.comp_stride = (int64_t)dp.nz * dp.hx * dp.rows;
+        sa.K = K;
+        const int P = dp.py / 4;
+        const size_t lds = slab_lds(dp, ce);
+        Timed t(c, "demag_yz_slab");
+        auto go = [&](auto PC) {
+            constexpr int PP = decltype(PC)::value;
+            auto launch = [&](auto kernel) {
+                // above 64 KB of dynamic LDS the kernel must opt in (once per kernel)
+                // (idempotent; a failed opt-in surfaces as the launch error)
+                static std::atomic<bool> opted{false};
+                if (!opted.load(std::memory_order_relaxed))
+                    opted = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+                hipLaunchKernelGGL(kernel, dim3(dp.hx), dim3(2 * PP), lds, c.stream, sa);
+            };
+            auto pick = [&](auto NZC) { launch(yz_slab_kernel<Real, decltype(NZC)::value, PP, 2>); };
+            switch (dp.nz) {
+                case 1: pick(std::integral_constant<int, 1>{}); break;
+                case 2: pick(std::integral_constant<int, 2>{}); break;
+                case 3: pick(std::integral_constant<int, 3>{}); break;
+                case 4: pick(std::integral_constant<int, 4>{}); break;
+                case 5: pick(std::integral_constant<int, 5>{}); break;
+                case 6: pick(std::integral_constant<int, 6>{}); break;
+                case 7: pick(std::integral_constant<int, 7>{}); break;
+                default: pick(std::integral_constant<int, 8>{}); break;
+            }
+        };
+        switch (P) {
+            case 16: go(std::integral_constant<int, 16>{}); break;
+            case 32: go(std::integral_constant<int, 32>{}); break;
+            case 64: go(std::integral_constant<int, 64>{}); break;
+            case 128: go(std::integral_constant<int, 128>{}); break;
+            default: go(std::integral_constant<int, 256>{}); break;
+        }
+    } else if (dp.three_d) {
         LineArgs<Real> la{};
         la.data = spec;
         la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
@@ -1253,6 +1537,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "x_pipe") t.x_pipe = v;
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
+            else if (k == "yz_slab") t.yz_slab = v;
         }
         pos = end + 1;
     }
@@ -1458,6 +1743,7 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
     }
     if (!upload_twiddles<Real>(c, dp.px, &dp.tw_x) || !upload_twiddles<Real>(c, dp.py, &dp.tw_y)) return false;
     if (dp.three_d && !upload_twiddles<Real>(c, dp.pz, &dp.tw_z)) return false;
+    if (slab_geometry(dp, sizeof(cplx<Real>)) && !upload_twiddles<Real>(c, dp.py / 4, &dp.tw_p)) return false;
     dp.spec_elems = 3ull * (dp.three_d ? dp.nz : 1) * dp.hx * dp.rows;
     if (!ctx_alloc(c, &dp.spec, dp.spec_elems * ce, "demag spectrum")) return false;
     if (dp.projection) return true;

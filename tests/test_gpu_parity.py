@@ -222,7 +222,8 @@ def test_demag_field_long_z(fa, oracle_mod, cells):
 SCHEDULES = ["", "z_lds=1", "fixed=0", "z_quad=1", "z_quad=0", "z_mirror=1", "z_stage=0", "x_rows=4,y_lpw=1",
              "xcd=0", "x_radix=4,y_radix=8", "x_lanes=128,y_lanes=64", "y_pipe=0", "y_pipe=3",
              "x_pipe=2", "x_pipe=3,x_rows=4", "x_pipe=4,x_rows=8", "y_pad=16", "y_pad=8,z_stage=0",
-             "y_pad=24,z_quad=1", "y_pad=8,z_lds=1", "y_pad=8,z_mirror=1", "zc=1", "zc=3,xcd=0"]
+             "y_pad=24,z_quad=1", "y_pad=8,z_lds=1", "y_pad=8,z_mirror=1", "zc=1", "zc=3,xcd=0", "yz_slab=1",
+             "yz_slab=1,z_quad=1"]
 
 
 @pytest.mark.parametrize("cells", [(32, 16, 4), (64, 32, 8), (16, 16, 6)])
@@ -237,6 +238,25 @@ def test_demag_schedule_variants_match_oracle(fa, oracle_mod, monkeypatch, cells
     hd = cpu.observe()["h_demag"]
     # other FFT factorisations round differently: 1e-11 of max|H| (the
     # trajectory bar is max|dm| <= 1e-10)
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max())
+    for _ in range(3):
+        gpu.step(5e-14)
+        cpu.step(5e-14)
+    assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12
+
+
+@pytest.mark.parametrize("cells", [(64, 32, 8), (16, 128, 3), (20, 256, 5), (8, 512, 2), (24, 64, 7), (12, 64, 1)])
+@pytest.mark.parametrize("tune", ["yz_slab=1", "yz_slab=1,y_pad=8"])
+def test_demag_fused_slab_matches_oracle(fa, oracle_mod, monkeypatch, cells, tune):
+    """The fused y transform + z convolution per kx slab (ky split into four
+    classes of py/4 = 16..256 points, odd and even nz)."""
+    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(29 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=8.6e5, a=1.3e-11, alpha=0.05,
+                         m0=m0, h_ext=(1e4, 0.0, 2e3), kernel="device")
+    hd = cpu.observe()["h_demag"]
     np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max())
     for _ in range(3):
         gpu.step(5e-14)
