@@ -216,13 +216,29 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
     const int lx = L::line_xor(jl);
     {
         const int ya = y0 + 2 * jl;
-        for (int x = t; x < n; x += T) {
-            cplx<Real> v = {Real(0), Real(0)};
-            if (x < a.n_in) {
-                if (ya < a.rows) v.x = src(c, z, ya, x);
-                if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
+        if constexpr (N > 0) {
+            // fixed length: all 16 loads of a lane in flight before the LDS writes
+            cplx<Real> v[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int x = t + e * (N / 16);
+                v[e] = {Real(0), Real(0)};
+                if (x < a.n_in) {
+                    if (ya < a.rows) v[e].x = src(c, z, ya, x);
+                    if (ya + 1 < a.rows) v[e].y = src(c, z, ya + 1, x);
+                }
             }
-            line[L::slot(a.lp, x, lx)] = v;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) line[L::slot(a.lp, t + e * (N / 16), lx)] = v[e];
+        } else {
+            for (int x = t; x < n; x += T) {
+                cplx<Real> v = {Real(0), Real(0)};
+                if (x < a.n_in) {
+                    if (ya < a.rows) v.x = src(c, z, ya, x);
+                    if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
+                }
+                line[L::slot(a.lp, x, lx)] = v;
+            }
         }
     }
     __syncthreads();
@@ -257,15 +273,31 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
     const int hx = a.g.hx;
-    // consecutive lanes read consecutive rows of one kx column
-    for (int idx = tid; idx < hx * LINES; idx += nthr) {
-        const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
-        const int ya = y0 + 2 * j, jx = L::line_xor(j);
-        cplx<Real> A = {Real(0), Real(0)}, B = {Real(0), Real(0)};
-        if (ya < a.rows) A = spec[a.g.at(c, z, k, ya)];
-        if (ya + 1 < a.rows) B = spec[a.g.at(c, z, k, ya + 1)];
-        lds[j * ls + L::slot(a.lp, k, jx)] = {A.x - B.y, A.y + B.x};
-        if (k > 0 && k < n - k) lds[j * ls + L::slot(a.lp, n - k, jx)] = {A.x + B.y, B.x - A.y};
+    // consecutive lanes read consecutive rows of one kx column; batches of
+    // four loads per lane are in flight before their LDS writes
+    constexpr int U = 4;
+    for (int base = tid; base < hx * LINES; base += U * nthr) {
+        cplx<Real> A[U], B[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * nthr;
+            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+            const int ya = y0 + 2 * j;
+            A[u] = B[u] = {Real(0), Real(0)};
+            if (idx < hx * LINES) {
+                if (ya < a.rows) A[u] = spec[a.g.at(c, z, k, ya)];
+                if (ya + 1 < a.rows) B[u] = spec[a.g.at(c, z, k, ya + 1)];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * nthr;
+            if (idx >= hx * LINES) break;
+            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+            const int jx = L::line_xor(j);
+            lds[j * ls + L::slot(a.lp, k, jx)] = {A[u].x - B[u].y, A[u].y + B[u].x};
+            if (k > 0 && k < n - k) lds[j * ls + L::slot(a.lp, n - k, jx)] = {A[u].x + B[u].y, B[u].x - A[u].y};
+        }
     }
     __syncthreads();
     const int jl = tid / T, t = tid - jl * T;
@@ -680,40 +712,92 @@ __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
 // once and no lane keeps loaded data in registers across the transform.
 constexpr int kZTile = 128;
 
+// Tile of kZTile consecutive columns starting at col0: 3 NZ rows, each row
+// one contiguous run of the spectrum.  Lane t moves 16-byte units t + 256 i.
 template <typename Real, int NZ>
-__global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+struct ZTile {
     using C = cplx<Real>;
-    constexpr int ROWS = 3 * NZ;
-    constexpr int V = sizeof(Real) == 4 ? 2 : 1;  // complex values per 16-byte access
-    constexpr int UNITS = kZTile / V;             // accesses per row
-    __shared__ C tile[ROWS][kZTile + V];
-    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t col0 = (int64_t)blk * kZTile;
-    const int t = (int)threadIdx.x;
-    // load: row r = (c, z) at data[c comp_stride + z zstride + col0 ...]
+    static constexpr int ROWS = 3 * NZ;
+    static constexpr int V = sizeof(Real) == 4 ? 2 : 1;  // complex values per 16-byte access
+    static constexpr int UNITS = kZTile / V;             // accesses per row
+    static constexpr int PER = (ROWS * UNITS + 255) / 256;
+    C v[PER][V];
+
+    __device__ __forceinline__ void fetch(const ZArgs<Real> &a, int64_t col0, int t) {
 #pragma unroll
-    for (int i = 0; i < (ROWS * UNITS + 255) / 256; ++i) {
-        const int idx = t + 256 * i;
-        if (idx < ROWS * UNITS) {
-            const int r = idx / UNITS, u = idx - r * UNITS;
-            const int c = r / NZ, z = r - c * NZ;
-            const int64_t col = col0 + (int64_t)u * V;
-            const C *src = a.data + c * a.comp_stride + z * a.zstride + col;
-            if (col + V <= a.columns) {
-                if constexpr (V == 2) {
-                    const float4 q = *reinterpret_cast<const float4 *>(src);
-                    tile[r][u * 2] = C{Real(q.x), Real(q.y)};
-                    tile[r][u * 2 + 1] = C{Real(q.z), Real(q.w)};
+        for (int i = 0; i < PER; ++i) {
+            const int idx = t + 256 * i;
+#pragma unroll
+            for (int e = 0; e < V; ++e) v[i][e] = C{Real(0), Real(0)};
+            if (idx < ROWS * UNITS) {
+                const int r = idx / UNITS, u = idx - r * UNITS;
+                const int c = r / NZ, z = r - c * NZ;
+                const int64_t col = col0 + (int64_t)u * V;
+                const C *src = a.data + c * a.comp_stride + z * a.zstride + col;
+                if (col + V <= a.columns) {
+                    if constexpr (V == 2) {
+                        const float4 q = *reinterpret_cast<const float4 *>(src);
+                        v[i][0] = C{Real(q.x), Real(q.y)};
+                        v[i][1] = C{Real(q.z), Real(q.w)};
+                    } else {
+                        v[i][0] = *src;
+                    }
                 } else {
-                    tile[r][u] = *src;
+                    for (int e = 0; e < V; ++e)
+                        if (col + e < a.columns) v[i][e] = src[e];
                 }
-            } else {
-                for (int e = 0; e < V; ++e)
-                    tile[r][u * V + e] = col + e < a.columns ? src[e] : C{Real(0), Real(0)};
             }
         }
     }
-    __syncthreads();
+    __device__ __forceinline__ void put(C (*tile)[kZTile + V], int t) const {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = t + 256 * i;
+            if (idx < ROWS * UNITS) {
+                const int r = idx / UNITS, u = idx - r * UNITS;
+#pragma unroll
+                for (int e = 0; e < V; ++e) tile[r][u * V + e] = v[i][e];
+            }
+        }
+    }
+};
+
+template <typename Real, int NZ>
+__device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*tile)[kZTile + ZTile<Real, NZ>::V],
+                                            int64_t col0, int t) {
+    using C = cplx<Real>;
+    using TL = ZTile<Real, NZ>;
+    constexpr int V = TL::V, UNITS = TL::UNITS;
+#pragma unroll
+    for (int i = 0; i < TL::PER; ++i) {
+        const int idx = t + 256 * i;
+        if (idx < TL::ROWS * UNITS) {
+            const int r = idx / UNITS, u = idx - r * UNITS;
+            const int c = r / NZ, z = r - c * NZ;
+            const int64_t cl = col0 + (int64_t)u * V;
+            C *dst = a.data + c * a.comp_stride + z * a.zstride + cl;
+            if (cl + V <= a.columns) {
+                if constexpr (V == 2) {
+                    const C e0 = tile[r][u * 2], e1 = tile[r][u * 2 + 1];
+                    *reinterpret_cast<float4 *>(dst) = float4{float(e0.x), float(e0.y), float(e1.x), float(e1.y)};
+                } else {
+                    *dst = tile[r][u];
+                }
+            } else {
+                for (int e = 0; e < V; ++e)
+                    if (cl + e < a.columns) dst[e] = tile[r][u * V + e];
+            }
+        }
+    }
+}
+
+// The register convolution of the tile's columns (two lanes per column, as
+// zconv_pair_kernel), in place in LDS.  Contains the barrier between the
+// reads of the tile and the writes of the results.
+template <typename Real, int NZ>
+__device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> (*tile)[kZTile + ZTile<Real, NZ>::V],
+                                               int64_t col0, int t) {
+    using C = cplx<Real>;
     const int j = t >> 1, p = t & 1;
     const int64_t col = col0 + j;
     bool valid = col < a.columns;
@@ -749,27 +833,48 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
             if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
         }
     }
+}
+
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+    using TL = ZTile<Real, NZ>;
+    __shared__ cplx<Real> tile[TL::ROWS][kZTile + TL::V];
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int64_t col0 = (int64_t)blk * kZTile;
+    const int t = (int)threadIdx.x;
+    {
+        TL in;
+        in.fetch(a, col0, t);
+        in.put(tile, t);
+    }
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < (ROWS * UNITS + 255) / 256; ++i) {
-        const int idx = t + 256 * i;
-        if (idx < ROWS * UNITS) {
-            const int r = idx / UNITS, u = idx - r * UNITS;
-            const int c = r / NZ, z = r - c * NZ;
-            const int64_t cl = col0 + (int64_t)u * V;
-            C *dst = a.data + c * a.comp_stride + z * a.zstride + cl;
-            if (cl + V <= a.columns) {
-                if constexpr (V == 2) {
-                    const C e0 = tile[r][u * 2], e1 = tile[r][u * 2 + 1];
-                    *reinterpret_cast<float4 *>(dst) = float4{float(e0.x), float(e0.y), float(e1.x), float(e1.y)};
-                } else {
-                    *dst = tile[r][u];
-                }
-            } else {
-                for (int e = 0; e < V; ++e)
-                    if (cl + e < a.columns) dst[e] = tile[r][u * V + e];
-            }
-        }
+    ztile_convolve<Real, NZ>(a, tile, col0, t);
+    __syncthreads();
+    ztile_store<Real, NZ>(a, tile, col0, t);
+}
+
+// Persistent form: each workgroup convolves `iters` consecutive tiles and
+// holds the next tile's input in registers while the current one is
+// transformed, so its loads overlap the arithmetic and the stores.
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256, 2) zconv_tile_pipe_kernel(ZArgs<Real> a, int iters) {
+    using TL = ZTile<Real, NZ>;
+    __shared__ cplx<Real> tile[TL::ROWS][kZTile + TL::V];
+    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int t = (int)threadIdx.x;
+    const int64_t first = (int64_t)blk * iters * kZTile;
+    TL buf;
+    buf.fetch(a, first, t);
+    for (int it = 0; it < iters; ++it) {
+        const int64_t col0 = first + (int64_t)it * kZTile;
+        if (col0 >= a.columns) break;  // uniform over the workgroup
+        buf.put(tile, t);
+        __syncthreads();
+        if (it + 1 < iters) buf.fetch(a, col0 + kZTile, t);
+        ztile_convolve<Real, NZ>(a, tile, col0, t);
+        __syncthreads();
+        ztile_store<Real, NZ>(a, tile, col0, t);
+        __syncthreads();
     }
 }
 
@@ -1395,7 +1500,20 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                     zb(256);
                 const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
                 const bool quad = dp.tune.z_quad && (dp.nz == 4 || dp.nz == 6 || dp.nz == 8);
-                if (dp.tune.z_stage && !quad) {
+                if (dp.tune.z_stage && !quad && dp.tune.z_pipe > 0) {
+                    const int it = dp.tune.z_pipe;
+                    const dim3 zt((unsigned)cdiv(cdiv(za.columns, kZTile), it));
+                    switch (dp.nz) {
+                        case 1: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 1>), zt, zb, 0, c.stream, za, it); break;
+                        case 2: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 2>), zt, zb, 0, c.stream, za, it); break;
+                        case 3: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 3>), zt, zb, 0, c.stream, za, it); break;
+                        case 4: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 4>), zt, zb, 0, c.stream, za, it); break;
+                        case 5: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 5>), zt, zb, 0, c.stream, za, it); break;
+                        case 6: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 6>), zt, zb, 0, c.stream, za, it); break;
+                        case 7: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 7>), zt, zb, 0, c.stream, za, it); break;
+                        default: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 8>), zt, zb, 0, c.stream, za, it); break;
+                    }
+                } else if (dp.tune.z_stage && !quad) {
                     const dim3 zt((unsigned)cdiv(za.columns, kZTile));
                     switch (dp.nz) {
                         case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1>), zt, zb, 0, c.stream, za); break;
@@ -1534,6 +1652,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "z_mirror") t.z_mirror = v;
             else if (k == "y_pipe") t.y_pipe = v;
             else if (k == "z_stage") t.z_stage = v;
+            else if (k == "z_pipe") t.z_pipe = v;
             else if (k == "x_pipe") t.x_pipe = v;
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
