@@ -425,6 +425,7 @@ int upload_impl(fullmag_fdm_backend *h, const void *m, uint64_t len, bool f64) {
     c.hd_at_m = false;
     // an externally replaced state invalidates integrator history
     c.fsal_valid = false;
+    c.pre_valid = false;
     c.abm_count = 0;
     c.abm_startup = 0;
     c.abm_last_dt = 0.0;
@@ -653,6 +654,7 @@ int fullmag_fdm_ext_set_external_field(fullmag_fdm_backend *handle, const double
     c.obs_valid = false;
     // stored slopes were evaluated with the old field
     c.fsal_valid = false;
+    c.pre_valid = false;
     c.abm_count = 0;
     c.abm_startup = 0;
     return FULLMAG_FDM_OK;
@@ -698,6 +700,7 @@ int fullmag_fdm_ext_set_state_device(fullmag_fdm_backend *handle, const double *
     c.obs_valid = false;
     c.obs_frozen = false;
     c.fsal_valid = false;
+    c.pre_valid = false;
     c.abm_count = 0;
     c.abm_startup = 0;
     return FULLMAG_FDM_OK;

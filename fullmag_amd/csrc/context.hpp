@@ -125,6 +125,11 @@ struct Context {
     void *k[7] = {};
     void *h_ex_obs = nullptr, *h_eff_obs = nullptr;
     bool hd_at_m = false;     // hd holds H_demag(m)
+    // Heun: k[0] = f(m, H(m)) and mt[0] = n(m + pre_dt k[0]) were written by
+    // the stats pass of the previous step (its torque is the next step's
+    // first slope, evaluated on the same inputs)
+    bool pre_valid = false;
+    double pre_dt = 0.0;
     bool obs_frozen = false;  // observables kept from before an upload
     bool obs_valid = false;   // h_ex_obs / h_eff_obs hold the fields at m
     DemagPlan dp;

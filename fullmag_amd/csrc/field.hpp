@@ -271,7 +271,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
 #pragma unroll
         for (int c = 0; c < 3; ++c) a.kout[c * n + i] = k[c];
     }
-    if (MODE == STAGE_PLAIN && a.out) {
+    if (MODE != STAGE_ERROR && a.out) {
         State v[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {

@@ -222,10 +222,14 @@ struct Driver {
         c.hd_at_m = true;
     }
 
-    // Diagnostics at m (hd must hold H_demag(m)); optionally stores dm/dt.
-    void stats(Real *kout) {
+    // Diagnostics at m (hd must hold H_demag(m)); optionally stores dm/dt
+    // and, with `out`, the predictor n(m + dt dm/dt) as stage() would.
+    void stats(Real *kout, Real *out = nullptr, double dt = 0.0) {
         StageArgs<State, Real, State, Real> a = args<State, Real>(m(), hd());
         a.kout = kout;
+        a.out = out;
+        a.anew = 1.0;
+        a.dt = dt;
         const unsigned g = grid_reduce();
         {
             Timed t(c, "llg_stats");
@@ -237,14 +241,28 @@ struct Driver {
         hipMemcpyAsync(c.h_result, c.d_result, kNStat * sizeof(double), hipMemcpyDeviceToHost, c.stream);
     }
 
-    // lib.rs:1217-1249
-    void heun(double dt) {
+    // lib.rs:1217-1249.  With use_pre, a predictor left by the previous
+    // step's stats pass (same m, same H, same dt) replaces the first stage.
+    void heun(double dt, bool use_pre = false) {
         ensure_hd();
-        stage<State, Real>(m(), k(0), mt(0), {}, 1.0, dt);
+        if (!(use_pre && c.pre_valid && c.pre_dt == dt)) stage<State, Real>(m(), k(0), mt(0), {}, 1.0, dt);
+        c.pre_valid = false;
         demag_at_stage(mt(0));
         stage<Real, State>(mt(0), nullptr, m(), {{0, 0.5}}, 0.5, dt);
         demag_at_state(m());
         c.hd_at_m = true;
+    }
+
+    // Stats at the new state.  For Heun the same pass stores the next step's
+    // first slope and predictor for a step of the same dt.
+    void heun_aware_stats(double dt) {
+        if (c.integrator == FULLMAG_FDM_INTEGRATOR_HEUN) {
+            stats(k(0), mt(0), dt);
+            c.pre_valid = true;
+            c.pre_dt = dt;
+        } else {
+            stats(nullptr);
+        }
     }
 
     // lib.rs:1254-1302
@@ -373,9 +391,9 @@ struct Driver {
             case FULLMAG_FDM_INTEGRATOR_DP45: suggested = dp45(dt); break;
             case FULLMAG_FDM_INTEGRATOR_ABM3: abm3(dt); stats_pending = true; break;
             case FULLMAG_FDM_INTEGRATOR_HEUN:
-            default: heun(dt); break;
+            default: heun(dt, true); break;
         }
-        if (!stats_pending) stats(nullptr);
+        if (!stats_pending) heun_aware_stats(dt);
         c.obs_valid = false;
         c.obs_frozen = false;
         if (!hip_ok(hipStreamSynchronize(c.stream), "integrator step", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
@@ -401,12 +419,12 @@ struct Driver {
                 case FULLMAG_FDM_INTEGRATOR_RK23: suggested = rk23(d); break;
                 case FULLMAG_FDM_INTEGRATOR_DP45: suggested = dp45(d); break;
                 case FULLMAG_FDM_INTEGRATOR_ABM3: abm3(d); break;
-                default: heun(d); break;
+                default: heun(d, s == 0); break;
             }
             c.step_count++;
             c.time += d;
             if (s + 1 == nsteps) {
-                if (c.integrator != FULLMAG_FDM_INTEGRATOR_ABM3) stats(nullptr);
+                if (c.integrator != FULLMAG_FDM_INTEGRATOR_ABM3) heun_aware_stats(d);
                 fill_pending_dt_ = d;
             }
         }
@@ -507,6 +525,7 @@ bool integrator_prepare(Context &c) {
         return false;
     }
     c.timing = false;
+    c.pre_valid = false;  // captured steps always run their first stage
     if (c.mode == MODE_DOUBLE) Driver<double, double>(c).ensure_hd();
     else if (c.mode == MODE_MIXED) Driver<double, float>(c).ensure_hd();
     else Driver<float, float>(c).ensure_hd();
@@ -526,6 +545,7 @@ int integrator_finish(Context &c, double dt, uint64_t nsteps, fullmag_fdm_step_s
     if (c.mode == MODE_DOUBLE) Driver<double, double>(c).stats(nullptr);
     else if (c.mode == MODE_MIXED) Driver<double, float>(c).stats(nullptr);
     else Driver<float, float>(c).stats(nullptr);
+    c.pre_valid = false;  // the replays moved m
     c.obs_valid = false;
     c.obs_frozen = false;
     if (!hip_ok(hipStreamSynchronize(c.stream), "batch steps", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
@@ -577,6 +597,7 @@ double integrator_probe_ms(Context &c, int reps) {
     hipEventDestroy(b);
     c.timing = timing;
     c.hd_at_m = true;
+    c.pre_valid = false;  // hd was recomputed under other schedules
     return (double)ms;
 }
 
