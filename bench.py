@@ -69,9 +69,10 @@ def alg_bytes(nx, ny, nz, q, r):
         "demag_z_conv": 3 * a2 * c + kern * r + 3 * a2 * c,
         "demag_y_inv": 3 * a2 * c + 3 * a1 * c,
         "demag_x_c2r": 3 * a1 * c + 3 * n * r,
-        # predictor: m, Hd -> k1, mt;  corrector: mt, Hd, m0, k1 -> m
-        "llg_stage": 3 * n * (q + 3 * r + 2 * q + 3 * r) / 2,
-        "llg_stats": 3 * n * (q + r),
+        # corrector: mt, Hd, m0, k1 -> m (the predictor runs in the stats
+        # pass of the previous step: m, Hd -> stats, k1, mt)
+        "llg_stage": 3 * n * (2 * q + 3 * r),
+        "llg_stats": 3 * n * (q + 3 * r),
     }
 
 
@@ -250,6 +251,15 @@ def sweep96(fa, torch, args):
           "schedules": {f"{k[0]}x{k[1]}x{k[2]}": v for k, v in schedules.items()}})
 
 
+def film_m0(nx, ny, nz):
+    """In-plane state with gentle x and y modulations (no domain walls)."""
+    x = np.arange(nx)[None, None, :] * np.ones((nz, ny, 1))
+    y = np.arange(ny)[None, :, None] * np.ones((nz, 1, nx))
+    th = 0.4 * np.sin(2 * np.pi * x / nx) + 0.2 * np.cos(2 * np.pi * y / ny)
+    m = np.stack([np.cos(th), np.sin(th), 0.1 * np.ones_like(th)], -1).reshape(-1, 3)
+    return m / np.linalg.norm(m, axis=1, keepdims=True)
+
+
 def vortex_m0(nx, ny, nz, c=0.1, noise=0.01, seed=3):
     y, x = np.meshgrid(np.arange(ny) - (ny - 1) / 2, np.arange(nx) - (nx - 1) / 2, indexing="ij")
     v = np.stack([-y, x, np.full_like(x, c)], -1)
@@ -297,17 +307,20 @@ def dp45_block(fa, torch, args):
 
 def adjoint_film(fa, torch, args):
     """128x128x4 film of 5x5x3 nm, per-cell Ms ~ U[6e5, 9e5] (seed 4),
-    A 1.3e-11, alpha 0.1, B = (20, 5, 0) mT, 2000 Heun steps of 1e-13 s;
-    loss = mean m_x over the centred 32x32 window (x, y in [48, 80)) of all
-    layers; checkpoints every 100 steps.  Gradients w.r.t. per-cell Ms and B
-    from the adjoint, checked against central differences (h = 1e-4
-    relative, FP64) at 32 seeded cells and the two in-plane B components."""
+    A 1.3e-11, alpha 0.1, B = (20, 5, 0) mT, 2000 Heun steps of 1e-13 s from
+    a smooth in-plane m0 (film_m0); loss = mean m_x over the centred 32x32
+    window (x, y in [48, 80)) of all layers; checkpoints every 100 steps.
+    Gradients w.r.t. per-cell Ms and B from the adjoint, checked against
+    central differences (h = 1e-4 relative, FP64) at 32 seeded cells and the
+    two in-plane B components.  (From a uniformly random m0 the relaxation is
+    turbulent and the h = 1e-4 difference itself carries a 1e-3 truncation
+    error; the difference converges to the adjoint as h shrinks.)"""
     nx, ny, nz = 128, 128, 4
     n = nx * ny * nz
     steps = 2000 if args.steps >= 300 else args.steps
     rng = np.random.default_rng(4)
     ms_np = rng.uniform(6e5, 9e5, n)
-    m0 = fa.generate_random_unit_vectors(42, n)
+    m0 = film_m0(nx, ny, nz)
     b_mt = np.array([20.0, 5.0, 0.0])
     res = {}
     for prec in ("double", "mixed"):
@@ -346,6 +359,7 @@ def adjoint_film(fa, torch, args):
         fds.append(fd)
         errs.append(abs(fd - d["g_ms"][i]))
     scale = max(np.abs(fds).max(), 1e-300)
+    rel = [e / max(abs(f), 1e-3 * scale) for e, f in zip(errs, fds)]
     b_errs = []
     for k in range(2):
         h = 1e-4 * np.abs(b_mt).max()
@@ -359,9 +373,10 @@ def adjoint_film(fa, torch, args):
     line("adjoint cell-steps/s (Heun backward pass with checkpoint recompute), 128x128x4 film",
          n * steps / d["backward_s"], "cell-steps/s", steps, 0, 1e3 * d["backward_s"] / steps, "f64",
          {"workload": "differentiable film 128x128x4 of 5x5x3 nm, per-cell Ms U[6e5,9e5], A 1.3e-11, alpha 0.1, "
-                      "B (20,5,0) mT, 2000 Heun steps dt 1e-13, loss mean m_x over centred 32x32 window, "
-                      "checkpoint every 100",
-          "fd_cells": len(cells), "max_abs_err_over_max_fd_ms": float(max(errs) / scale),
+                      "B (20,5,0) mT, smooth in-plane m0, 2000 Heun steps dt 1e-13, loss mean m_x over centred 32x32 "
+                      "window, checkpoint every 100",
+          "fd_cells": len(cells), "max_rel_err_ms": float(max(rel)),
+          "max_abs_err_over_max_fd_ms": float(max(errs) / scale),
           "max_rel_err_b": float(max(b_errs)), "loss": d["loss"],
           "fp64": {"forward_s": d["forward_s"], "backward_s": d["backward_s"]},
           "mixed": {"forward_s": x["forward_s"], "backward_s": x["backward_s"],
