@@ -219,6 +219,7 @@ int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedu
     cp->dp.tune.xcd = t.xcd;
     cp->dp.tune.zc = t.zc;
     cp->dp.tune.z_pipe = t.z_pipe;
+    cp->dp.tune.z_wide = t.z_wide;
     return FULLMAG_FDM_OK;
 }
 

@@ -714,12 +714,12 @@ constexpr int kZTile = 128;
 
 // Tile of kZTile consecutive columns starting at col0: 3 NZ rows, each row
 // one contiguous run of the spectrum.  Lane t moves 16-byte units t + 256 i.
-template <typename Real, int NZ>
+template <typename Real, int NZ, int TW = kZTile>
 struct ZTile {
     using C = cplx<Real>;
     static constexpr int ROWS = 3 * NZ;
     static constexpr int V = sizeof(Real) == 4 ? 2 : 1;  // complex values per 16-byte access
-    static constexpr int UNITS = kZTile / V;             // accesses per row
+    static constexpr int UNITS = TW / V;                 // accesses per row
     static constexpr int PER = (ROWS * UNITS + 255) / 256;
     C v[PER][V];
 
@@ -749,7 +749,7 @@ struct ZTile {
             }
         }
     }
-    __device__ __forceinline__ void put(C (*tile)[kZTile + V], int t) const {
+    __device__ __forceinline__ void put(C (*tile)[TW + V], int t) const {
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = t + 256 * i;
@@ -762,11 +762,11 @@ struct ZTile {
     }
 };
 
-template <typename Real, int NZ>
-__device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*tile)[kZTile + ZTile<Real, NZ>::V],
+template <typename Real, int NZ, int TW = kZTile>
+__device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*tile)[TW + ZTile<Real, NZ, TW>::V],
                                             int64_t col0, int t) {
     using C = cplx<Real>;
-    using TL = ZTile<Real, NZ>;
+    using TL = ZTile<Real, NZ, TW>;
     constexpr int V = TL::V, UNITS = TL::UNITS;
 #pragma unroll
     for (int i = 0; i < TL::PER; ++i) {
@@ -792,55 +792,58 @@ __device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*t
 }
 
 // The register convolution of the tile's columns (two lanes per column, as
-// zconv_pair_kernel), in place in LDS.  Contains the barrier between the
-// reads of the tile and the writes of the results.
-template <typename Real, int NZ>
-__device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> (*tile)[kZTile + ZTile<Real, NZ>::V],
+// zconv_pair_kernel), in place in LDS; lane pair t/2 takes columns t/2,
+// t/2 + 128, ...  A column is read and written only by its own lane pair,
+// which sits in one wave, so no barrier separates the reads from the writes.
+template <typename Real, int NZ, int TW = kZTile>
+__device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> (*tile)[TW + ZTile<Real, NZ, TW>::V],
                                                int64_t col0, int t) {
     using C = cplx<Real>;
-    const int j = t >> 1, p = t & 1;
-    const int64_t col = col0 + j;
-    bool valid = col < a.columns;
-    const int kx = valid ? (int)(col / a.pitch) : 0;
-    const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
-    valid = valid && ky < a.py;  // pad columns: transformed with a zero tensor, never read
-    C v[3][NZ];
+    const int p = t & 1;
+#pragma unroll 1
+    for (int j = t >> 1; j < TW; j += 128) {
+        const int64_t col = col0 + j;
+        bool valid = col < a.columns;
+        const int kx = valid ? (int)(col / a.pitch) : 0;
+        const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+        valid = valid && ky < a.py;  // pad columns: transformed with a zero tensor, never read
+        C v[3][NZ];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < 3; ++c) {
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            C x = tile[c * NZ + z][j];
-            if (p && z) x = cmul(x, a.tw[z]);
-            v[c][z] = x;
+            for (int z = 0; z < NZ; ++z) {
+                C x = tile[c * NZ + z][j];
+                if (p && z) x = cmul(x, a.tw[z]);
+                v[c][z] = x;
+            }
+            half_dft<Real, NZ>(v[c], a.tw, false);
         }
-        half_dft<Real, NZ>(v[c], a.tw, false);
-    }
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) {
-        Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
-        if (valid) tensor_at(a.K, kx, ky, 2 * k + p, tt);
-        apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
-    }
-    __syncthreads();  // every lane has read its inputs before the tile is overwritten
+        for (int k = 0; k < NZ; ++k) {
+            Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+            if (valid) tensor_at(a.K, kx, ky, 2 * k + p, tt);
+            apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+        }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        half_dft<Real, NZ>(v[c], a.tw, true);
+        for (int c = 0; c < 3; ++c) {
+            half_dft<Real, NZ>(v[c], a.tw, true);
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            C y = v[c][z];
-            if (p && z) y = cmul(y, cconj(a.tw[z]));
-            const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
-            if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
+            for (int z = 0; z < NZ; ++z) {
+                C y = v[c][z];
+                if (p && z) y = cmul(y, cconj(a.tw[z]));
+                const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+                if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
+            }
         }
     }
 }
 
-template <typename Real, int NZ>
+template <typename Real, int NZ, int TW = kZTile>
 __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
-    using TL = ZTile<Real, NZ>;
-    __shared__ cplx<Real> tile[TL::ROWS][kZTile + TL::V];
+    using TL = ZTile<Real, NZ, TW>;
+    __shared__ cplx<Real> tile[TL::ROWS][TW + TL::V];
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int64_t col0 = (int64_t)blk * kZTile;
+    const int64_t col0 = (int64_t)blk * TW;
     const int t = (int)threadIdx.x;
     {
         TL in;
@@ -848,9 +851,9 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
         in.put(tile, t);
     }
     __syncthreads();
-    ztile_convolve<Real, NZ>(a, tile, col0, t);
+    ztile_convolve<Real, NZ, TW>(a, tile, col0, t);
     __syncthreads();
-    ztile_store<Real, NZ>(a, tile, col0, t);
+    ztile_store<Real, NZ, TW>(a, tile, col0, t);
 }
 
 // Persistent form: each workgroup convolves `iters` consecutive tiles and
@@ -1009,13 +1012,15 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
 // py)), and the first ny outputs of the inverse are
 //   y[n] = sum_s w^(-s n) IDFT_P(Y_s)[n mod P].
 // So one workgroup owns a whole kx slab (3 components x nz planes x ny) and
-// walks the S classes: fold into LDS, forward P-point transforms of the 3 nz
-// rows, the register z convolution of each of the P columns (two lanes per
-// column, as zconv_pair_kernel), inverse transforms, and accumulation of
-// w^(-s n) IDFT_P into registers; the slab is written back once.  Per demag
-// evaluation this replaces the separate y-forward, z and y-inverse passes
-// (three round trips of the 2 ny spectrum) with one read and one write of
-// the ny spectrum and one read of the tensor.
+// walks the S classes: fold from memory into LDS, forward P-point transforms
+// of the 3 nz rows, the register z convolution of each of the P columns (two
+// lanes per column, as zconv_pair_kernel), inverse transforms, and
+// accumulation of w^(-s n) IDFT_P into an LDS accumulator; the slab is
+// written back once.  Per demag evaluation this replaces the separate
+// y-forward, z and y-inverse passes (three round trips of the 2 ny spectrum)
+// with one write and four (mostly L2) reads of the ny spectrum and one read
+// of the tensor.  At one 144 KB workgroup per CU its phases do not overlap
+// each other's memory latency, which is why it is not the default schedule.
 template <typename Real>
 struct SlabArgs {
     cplx<Real> *data;          // spec[c][z][kx][pitch], first ny used
@@ -1080,144 +1085,129 @@ __global__ void __launch_bounds__(W * P) yz_slab_kernel(SlabArgs<Real> a) {
     using C = cplx<Real>;
     constexpr int S = 4;                 // ky classes
     constexpr int ROWS = 3 * NZ;         // (component, z) rows
-    constexpr int NT = W * P;            // threads (W = 2 or 4 per class column)
+    constexpr int NT = W * P;            // threads: one lane pair per class column
     constexpr int NY = 2 * P;            // ny = S P / 2
     constexpr int T = P / 16;            // lanes per row transform
-    constexpr bool QUAD = NZ % 2 == 0;   // four lanes per z column (else two)
+    static_assert(W == 2, "one lane pair per class column");
     static_assert(ROWS * T <= NT, "every row has its lanes");
     extern __shared__ unsigned char smem_raw[];
     C *acc = reinterpret_cast<C *>(smem_raw);  // [ROWS][NY] output accumulator
     C *cls = acc + ROWS * NY;                  // [ROWS][P] one ky class
     const int t = (int)threadIdx.x;
-    const int kx = (int)((blockIdx.x + (unsigned)a.hx - 1) % (unsigned)a.hx);  // the odd kx = nx first
-    const int64_t colbase = (int64_t)kx * a.pitch;
-    auto gaddr = [&](int r, int n) -> int64_t {
-        const int c = r / NZ, z = r - c * NZ;
-        return c * a.comp_stride + z * a.zstride + colbase + n;
-    };
     const int frow = t / T, ft = t - frow * T;  // row transform lanes
     const bool factive = frow < ROWS;
     C *fline = cls + (factive ? frow : 0) * P;
     const int flx = row_xor<P>(factive ? frow : 0);
+    const int col = t >> 1, p = t & 1;          // z convolution: lane pair per column
+    // The grid has hx - 1 workgroups; workgroup 0 also takes the last slab
+    // (kx = hx - 1), so an even hx - 1 fills whole rounds of the CUs.
+    const int nslab = blockIdx.x == 0 ? 2 : 1;
 #pragma unroll 1
-    for (int s = 0; s < S; ++s) {
-        // fold class s from the slab in memory: u[n] = sum_b x[n + b P] w^(s (n + b P));
-        // thread t folds position n = t mod P of rows t / P + (NT / P) i
-        {
-            const int n = t & (P - 1);
-            const C w0 = a.tw_y[(s * n) & (a.py - 1)], w1 = a.tw_y[(s * (n + P)) & (a.py - 1)];
-#pragma unroll 2
-            for (int r = t / P; r < ROWS; r += NT / P) {
-                C u = a.data[gaddr(r, n)], x1 = a.data[gaddr(r, n + P)];
-                if (s) {
-                    u = cmul(u, w0);
-                    x1 = cmul(x1, w1);
-                }
-                cls[r * P + (p2slot(n) ^ row_xor<P>(r))] = cadd(u, x1);
-            }
-        }
-        __syncthreads();
-        guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, false, factive, flx);
-        if constexpr (QUAD) {
-            // four lanes per column (zconv_quad_kernel's folded transform)
-            constexpr int M = NZ / 2, PZ = 2 * NZ;
-            const int q = t & 3;
-            C wq[NZ];
-#pragma unroll
-            for (int z = 0; z < NZ; ++z) wq[z] = a.tw_z[(q * z) % PZ];
+    for (int sl = 0; sl < nslab; ++sl) {
+        const int kx = sl == 0 ? (int)blockIdx.x : a.hx - 1;
+        const int64_t colbase = (int64_t)kx * a.pitch;
+        auto gaddr = [&](int r, int n) -> int64_t {
+            const int c = r / NZ, z = r - c * NZ;
+            return c * a.comp_stride + z * a.zstride + colbase + n;
+        };
 #pragma unroll 1
-            for (int col = t >> 2; col < P; col += NT / 4) {
-            C v[3][M];
+        for (int s = 0; s < S; ++s) {
+            // fold class s from the slab in memory: u[n] = sum_b x[n + b P] w^(s (n + b P));
+            // thread t folds position n = t mod P of rows t / P + (NT / P) i
+            {
+                const int n = t & (P - 1);
+                const C w0 = a.tw_y[(s * n) & (a.py - 1)], w1 = a.tw_y[(s * (n + P)) & (a.py - 1)];
+                // rows in batches of FB, each batch's loads in flight together;
+                // for P >= 64 a wave's lanes share one row (a scalar offset)
+                constexpr int RI = (ROWS + NT / P - 1) / (NT / P), FB = RI < 6 ? RI : 6;
+                const int rb = P >= 64 ? __builtin_amdgcn_readfirstlane(t / P) : t / P;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
+                for (int i0 = 0; i0 < RI; i0 += FB) {
+                    C u[FB], x1[FB];
 #pragma unroll
-                for (int z = 0; z < M; ++z) {
-                    const int r0 = c * NZ + z, r1 = r0 + M;
-                    v[c][z] = cadd(cmul(cls[r0 * P + (p2slot(col) ^ row_xor<P>(r0))], wq[z]),
-                                   cmul(cls[r1 * P + (p2slot(col) ^ row_xor<P>(r1))], wq[z + M]));
-                }
-                strided_dft<Real, M, 4>(v[c], a.tw_z, false);
-            }
-            const int ky = S * col + s;
+                    for (int i = 0; i < FB; ++i) {
+                        const int r = rb + (i0 + i) * (NT / P);
+                        u[i] = x1[i] = C{Real(0), Real(0)};
+                        if (i0 + i < RI && r < ROWS) {
+                            u[i] = a.data[gaddr(r, n)];
+                            x1[i] = a.data[gaddr(r, n + P)];
+                        }
+                    }
 #pragma unroll
-            for (int k = 0; k < M; ++k) {
-                Real tt[6];
-                tensor_at(a.K, kx, ky, 4 * k + q, tt);
-                apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
-            }
-            // the quad (one wave) read its whole column above, so writes may follow
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                strided_dft<Real, M, 4>(v[c], a.tw_z, true);
-#pragma unroll
-                for (int z = 0; z < NZ; ++z) {
-                    C sum = cmul(v[c][z % M], cconj(wq[z]));
-                    sum.x += __shfl_xor(sum.x, 1);
-                    sum.y += __shfl_xor(sum.y, 1);
-                    sum.x += __shfl_xor(sum.x, 2);
-                    sum.y += __shfl_xor(sum.y, 2);
-                    const int r = c * NZ + z;
-                    if ((z & 3) == q) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = sum;
-                }
-            }
-            }
-        } else {
-            // two lanes per column (zconv_pair_kernel); lanes >= 2 P idle
-            const int col = (t >> 1) & (P - 1), p = t & 1;
-            const bool zactive = t < 2 * P;
-            C v[3][NZ];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                for (int z = 0; z < NZ; ++z) {
-                    const int r = c * NZ + z;
-                    C x = zactive ? cls[r * P + (p2slot(col) ^ row_xor<P>(r))] : C{Real(0), Real(0)};
-                    if (p && z) x = cmul(x, a.tw_z[z]);
-                    v[c][z] = x;
-                }
-                half_dft<Real, NZ>(v[c], a.tw_z, false);
-            }
-            const int ky = S * col + s;
-#pragma unroll
-            for (int k = 0; k < NZ; ++k) {
-                Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
-                if (zactive) tensor_at(a.K, kx, ky, 2 * k + p, tt);
-                apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                half_dft<Real, NZ>(v[c], a.tw_z, true);
-#pragma unroll
-                for (int z = 0; z < NZ; ++z) {
-                    C y = v[c][z];
-                    if (p && z) y = cmul(y, cconj(a.tw_z[z]));
-                    const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
-                    const int r = c * NZ + z;
-                    if (zactive && (z & 1) == p) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = cadd(y, o);
+                    for (int i = 0; i < FB; ++i) {
+                        const int r = rb + (i0 + i) * (NT / P);
+                        if (i0 + i < RI && r < ROWS) {
+                            C v0 = u[i], v1 = x1[i];
+                            if (s) {
+                                v0 = cmul(v0, w0);
+                                v1 = cmul(v1, w1);
+                            }
+                            cls[r * P + (p2slot(n) ^ row_xor<P>(r))] = cadd(v0, v1);
+                        }
+                    }
                 }
             }
+            __syncthreads();
+            guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, false, factive, flx);
+            __syncthreads();
+            {
+                // two lanes per column (zconv_pair_kernel); the column is read
+                // and written only by its own lane pair
+                C v[3][NZ];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z) {
+                        const int r = c * NZ + z;
+                        C x = cls[r * P + (p2slot(col) ^ row_xor<P>(r))];
+                        if (p && z) x = cmul(x, a.tw_z[z]);
+                        v[c][z] = x;
+                    }
+                    half_dft<Real, NZ>(v[c], a.tw_z, false);
+                }
+#pragma unroll
+                for (int k = 0; k < NZ; ++k) {
+                    Real tt[6];
+                    tensor_at(a.K, kx, S * col + s, 2 * k + p, tt);
+                    apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    half_dft<Real, NZ>(v[c], a.tw_z, true);
+#pragma unroll
+                    for (int z = 0; z < NZ; ++z) {
+                        C y = v[c][z];
+                        if (p && z) y = cmul(y, cconj(a.tw_z[z]));
+                        const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+                        const int r = c * NZ + z;
+                        if ((z & 1) == p) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = cadd(y, o);
+                    }
+                }
+            }
+            __syncthreads();
+            guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, true, factive, flx);
+            __syncthreads();
+            // accumulate w^(-s n) IDFT_P(Y_s)[n mod P]: thread t owns position
+            // n = t mod NY of rows t / NY + (NT / NY) i
+            {
+                const int n = t & (NY - 1);
+                const C w = cconj(a.tw_y[(s * n) & (a.py - 1)]);
+                const int sl2 = p2slot(n & (P - 1));
+#pragma unroll 4
+                for (int r = t / NY; r < ROWS; r += NT / NY) {
+                    C y = cls[r * P + (sl2 ^ row_xor<P>(r))];
+                    if (s) y = cmul(y, w);
+                    acc[r * NY + n] = s ? cadd(acc[r * NY + n], y) : y;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, true, factive, flx);
-        // accumulate w^(-s n) IDFT_P(Y_s)[n mod P]: thread t owns position
-        // n = t mod NY of rows t / NY + (NT / NY) i
         {
             const int n = t & (NY - 1);
-            const C w = cconj(a.tw_y[(s * n) & (a.py - 1)]);
-            const int sl = p2slot(n & (P - 1));
 #pragma unroll 4
-            for (int r = t / NY; r < ROWS; r += NT / NY) {
-                C y = cls[r * P + (sl ^ row_xor<P>(r))];
-                if (s) y = cmul(y, w);
-                acc[r * NY + n] = s ? cadd(acc[r * NY + n], y) : y;
-            }
+            for (int r = t / NY; r < ROWS; r += NT / NY) a.data[gaddr(r, n)] = acc[r * NY + n];
         }
-        __syncthreads();
-    }
-    {
-        const int n = t & (NY - 1);
-#pragma unroll 4
-        for (int r = t / NY; r < ROWS; r += NT / NY) a.data[gaddr(r, n)] = acc[r * NY + n];
+        __syncthreads();  // the accumulator is rewritten by the next slab
     }
 }
 
@@ -1443,7 +1433,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 if (!opted.load(std::memory_order_relaxed))
                     opted = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
-                hipLaunchKernelGGL(kernel, dim3(dp.hx), dim3(2 * PP), lds, c.stream, sa);
+                hipLaunchKernelGGL(kernel, dim3(dp.hx - 1), dim3(2 * PP), lds, c.stream, sa);
             };
             auto pick = [&](auto NZC) { launch(yz_slab_kernel<Real, decltype(NZC)::value, PP, 2>); };
             switch (dp.nz) {
@@ -1512,6 +1502,18 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                         case 6: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 6>), zt, zb, 0, c.stream, za, it); break;
                         case 7: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 7>), zt, zb, 0, c.stream, za, it); break;
                         default: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 8>), zt, zb, 0, c.stream, za, it); break;
+                    }
+                } else if (dp.tune.z_stage && !quad && dp.tune.z_wide) {
+                    const dim3 zt((unsigned)cdiv(za.columns, 2 * kZTile));
+                    switch (dp.nz) {
+                        case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 2: hipLaunchKernelGGL((zconv_tile_kernel<Real, 2, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 3: hipLaunchKernelGGL((zconv_tile_kernel<Real, 3, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 4: hipLaunchKernelGGL((zconv_tile_kernel<Real, 4, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 5: hipLaunchKernelGGL((zconv_tile_kernel<Real, 5, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 6: hipLaunchKernelGGL((zconv_tile_kernel<Real, 6, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        case 7: hipLaunchKernelGGL((zconv_tile_kernel<Real, 7, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
+                        default: hipLaunchKernelGGL((zconv_tile_kernel<Real, 8, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
                     }
                 } else if (dp.tune.z_stage && !quad) {
                     const dim3 zt((unsigned)cdiv(za.columns, kZTile));
@@ -1653,6 +1655,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "y_pipe") t.y_pipe = v;
             else if (k == "z_stage") t.z_stage = v;
             else if (k == "z_pipe") t.z_pipe = v;
+            else if (k == "z_wide") t.z_wide = v;
             else if (k == "x_pipe") t.x_pipe = v;
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
