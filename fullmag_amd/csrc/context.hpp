@@ -74,6 +74,9 @@ struct DemagPlan {
     void *kern = nullptr;
     uint64_t spec_elems = 0;  // complex elements
     uint64_t kern_elems = 0;  // real elements
+    // tensor layout: one block of kblock reals per (kx, oy); the even-oz
+    // entries first, the odd-oz ones from kodd (see kern_at)
+    int kblock = 0, kodd = 0;
 };
 
 struct TimingEntry {

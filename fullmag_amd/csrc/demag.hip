@@ -23,7 +23,8 @@
 // (2-D), kx <= nx, ky < pitch (2ny in 3-D, ny in 2-D).
 // The Newell spectra are real (the padded kernel is even or odd along every
 // axis); the tensor is stored on the non-negative octant as
-// kern[comp][kz][kx][ky] (kz <= nz, ky <= ny) and mirrored with parity signs.
+// blocks per (kx, ky) column (kz <= nz, ky <= ny, kern_at) and mirrored with
+// parity signs.
 #include "context.hpp"
 
 #include <atomic>
@@ -60,9 +61,20 @@ template <typename Real>
 struct KernelView {
     const Real *k = nullptr;  // octant tensor, nullptr for the projection fallback
     int kcomp = 6, kz_n = 1;
+    int kb = 0, kodd = 0;     // block per (kx, oy) and offset of its odd-oz half
     int ny = 0, nz = 0, py = 0, pz = 1, hx = 0;
     double kxs = 0, kys = 0, kzs = 0;  // 2 pi / (p d) per axis (projection)
 };
+
+// Octant tensor layout: one block of `kblock` reals per (kx, oy) column,
+// holding the entries of even oz (0, 2, ...) and, from `kodd`, those of odd
+// oz, each entry `kcomp` consecutive components, each half padded to 16
+// bytes.  A z-transform lane that owns kz of one parity reads its whole half
+// with 16-byte loads (kz and pz - kz share an entry and a parity).
+__host__ __device__ __forceinline__ int64_t kern_at(int kblock, int kodd, int kcomp, int64_t ly, int c, int oz,
+                                                    int kx, int oy) {
+    return ((int64_t)kx * ly + oy) * kblock + ((oz & 1) ? kodd : 0) + (int64_t)(oz >> 1) * kcomp + c;
+}
 
 // t = (xx, yy, zz, xy, xz, yz) at frequency (kx, ky, kz); kx <= nx.
 template <typename Real>
@@ -78,16 +90,14 @@ __device__ __forceinline__ void tensor_at(const KernelView<Real> &K, int kx, int
             oz = zlo ? kz : K.pz - kz;
             sz = zlo ? Real(1) : Real(-1);
         }
-        const int64_t ly = K.ny + 1;
-        const int64_t off = ((int64_t)oz * K.hx + kx) * ly + oy;
-        const int64_t cs = (int64_t)K.kz_n * K.hx * ly;
-        t[0] = K.k[off];
-        t[1] = K.k[cs + off];
-        t[2] = K.k[2 * cs + off];
-        t[3] = sy * K.k[3 * cs + off];
+        const Real *e = K.k + kern_at(K.kb, K.kodd, K.kcomp, K.ny + 1, 0, oz, kx, oy);
+        t[0] = e[0];
+        t[1] = e[1];
+        t[2] = e[2];
+        t[3] = sy * e[3];
         if (K.kcomp == 6) {
-            t[4] = sz * K.k[4 * cs + off];
-            t[5] = sy * sz * K.k[5 * cs + off];
+            t[4] = sz * e[4];
+            t[5] = sy * sz * e[5];
         } else {
             t[4] = Real(0);
             t[5] = Real(0);
@@ -818,10 +828,31 @@ __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> 
             }
             half_dft<Real, NZ>(v[c], a.tw, false);
         }
+        // The lane's half of the column's tensor block (its kz parity): entry
+        // (kz <= nz ? kz : pz - kz) / 2, six components in three 8-byte (FP32)
+        // or 16-byte (FP64) loads.
+        constexpr int VEC = 16 / (int)sizeof(Real), NE = NZ / 2 + 1;
+        constexpr int B0 = (NE * 6 + VEC - 1) / VEC * VEC;
+        using V2 = typename std::conditional<sizeof(Real) == 4, float2, double2>::type;
+        const int oy = ky <= a.K.ny ? ky : a.py - ky;
+        const Real *blk = a.K.k + (valid ? ((int64_t)kx * (a.K.ny + 1) + oy) * a.K.kb + (p ? B0 : 0) : 0);
+        const Real sy = ky <= a.K.ny ? Real(1) : Real(-1);
 #pragma unroll
         for (int k = 0; k < NZ; ++k) {
             Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
-            if (valid) tensor_at(a.K, kx, ky, 2 * k + p, tt);
+            if (valid) {
+                const int kz0 = 2 * k, kz1 = 2 * k + 1;
+                const int i0 = (kz0 <= NZ ? kz0 : 2 * NZ - kz0) >> 1, i1 = (kz1 <= NZ ? kz1 : 2 * NZ - kz1) >> 1;
+                const Real sz = p ? (kz1 <= NZ ? Real(1) : Real(-1)) : (kz0 <= NZ ? Real(1) : Real(-1));
+                const V2 *e = reinterpret_cast<const V2 *>(blk + 6 * (p ? i1 : i0));
+                const V2 t01 = e[0], t23 = e[1], t45 = e[2];
+                tt[0] = t01.x;
+                tt[1] = t01.y;
+                tt[2] = t23.x;
+                tt[3] = sy * t23.y;
+                tt[4] = sz * t45.x;
+                tt[5] = sy * sz * t45.y;
+            }
             apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
         }
 #pragma unroll
@@ -1212,10 +1243,11 @@ __global__ void __launch_bounds__(W * P) yz_slab_kernel(SlabArgs<Real> a) {
 }
 
 // Octant extraction after the setup transform of layout spec[c][z][kx][ky]
-// (z < pz, ky < py): kern[c][oz][kx][oy] = Re spec[c][oz][kx][oy].
+// (z < pz, ky < py): entry (c, oz, kx, oy) of the tensor = Re spec[c][oz][kx][oy],
+// stored at kern_at.  `total` counts kcomp * kz_n * hx * (ny + 1) entries.
 template <typename Real>
 __global__ void extract_octant_kernel(const cplx<double> *spec, Real *kern, int kz_n, int hx, int ny, int pz,
-                                      int py, int64_t total) {
+                                      int py, int64_t total, int kblock, int kodd, int kcomp) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
     const int64_t ly = ny + 1;
@@ -1225,7 +1257,7 @@ __global__ void extract_octant_kernel(const cplx<double> *spec, Real *kern, int 
     const int oz = (int)(r / ((int64_t)hx * ly));
     r -= (int64_t)oz * hx * ly;
     const int kx = (int)(r / ly), oy = (int)(r - (int64_t)kx * ly);
-    kern[i] = Real(spec[(((int64_t)c * pz + oz) * hx + kx) * py + oy].x);
+    kern[kern_at(kblock, kodd, kcomp, ly, c, oz, kx, oy)] = Real(spec[(((int64_t)c * pz + oz) * hx + kx) * py + oy].x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1276,6 +1308,8 @@ KernelView<Real> kview(const Context &c) {
     KernelView<Real> K;
     K.k = dp.projection ? nullptr : static_cast<const Real *>(dp.kern);
     K.kcomp = dp.kcomp;
+    K.kb = dp.kblock;
+    K.kodd = dp.kodd;
     K.kz_n = dp.kz_n;
     K.ny = dp.ny;
     K.nz = dp.nz;
@@ -1490,7 +1524,9 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                     zb(256);
                 const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
                 const bool quad = dp.tune.z_quad && (dp.nz == 4 || dp.nz == 6 || dp.nz == 8);
-                if (dp.tune.z_stage && !quad && dp.tune.z_pipe > 0) {
+                // the staged tiles read the tensor blocks directly (no projection fallback)
+                const bool staged = dp.tune.z_stage && !quad && !dp.projection;
+                if (staged && dp.tune.z_pipe > 0) {
                     const int it = dp.tune.z_pipe;
                     const dim3 zt((unsigned)cdiv(cdiv(za.columns, kZTile), it));
                     switch (dp.nz) {
@@ -1503,7 +1539,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                         case 7: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 7>), zt, zb, 0, c.stream, za, it); break;
                         default: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 8>), zt, zb, 0, c.stream, za, it); break;
                     }
-                } else if (dp.tune.z_stage && !quad && dp.tune.z_wide) {
+                } else if (staged && dp.tune.z_wide) {
                     const dim3 zt((unsigned)cdiv(za.columns, 2 * kZTile));
                     switch (dp.nz) {
                         case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
@@ -1515,7 +1551,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                         case 7: hipLaunchKernelGGL((zconv_tile_kernel<Real, 7, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
                         default: hipLaunchKernelGGL((zconv_tile_kernel<Real, 8, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
                     }
-                } else if (dp.tune.z_stage && !quad) {
+                } else if (staged) {
                     const dim3 zt((unsigned)cdiv(za.columns, kZTile));
                     switch (dp.nz) {
                         case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1>), zt, zb, 0, c.stream, za); break;
@@ -1775,7 +1811,7 @@ bool host_octant_from_spectra(const DemagPlan &dp, const fullmag_fdm_plan_desc &
                             plan.demag_kernel_xz_spectrum, plan.demag_kernel_yz_spectrum};
     const int pz = dp.three_d ? dp.pz : 1;
     const int64_t ly = dp.ny + 1;
-    kern.assign((size_t)dp.kcomp * dp.kz_n * dp.hx * ly, 0.0);
+    kern.assign((size_t)dp.hx * ly * dp.kblock, 0.0);
     // parity of the six components along (x, y, z)
     const int odd[6][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}};
     double kmax = 0.0, bad = 0.0;
@@ -1798,7 +1834,7 @@ bool host_octant_from_spectra(const DemagPlan &dp, const fullmag_fdm_plan_desc &
                     const int64_t j = ((int64_t)oz * dp.py + oy) * dp.px + ox;
                     bad = std::max(bad, std::fabs(re - sg * s[2 * j]));
                     if (x <= dp.nx && y <= dp.ny && (pz == 1 || z <= dp.nz) && c < dp.kcomp)
-                        kern[(((size_t)c * dp.kz_n + z) * dp.hx + x) * ly + y] = re;
+                        kern[kern_at(dp.kblock, dp.kodd, dp.kcomp, ly, c, z, x, y)] = re;
                 }
     }
     if (bad > 1e-8 * std::max(kmax, 1e-300)) {
@@ -1837,6 +1873,13 @@ template <typename Real>
 bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
     DemagPlan &dp = c.dp;
     const size_t ce = sizeof(cplx<Real>);
+    {
+        // tensor blocks: even-oz half, then odd-oz half, each padded to 16 bytes
+        const int vec = 16 / (int)sizeof(Real);
+        const int ne = dp.three_d ? dp.nz / 2 + 1 : 1, no = dp.three_d ? (dp.nz + 1) / 2 : 0;
+        dp.kodd = (int)cdiv(ne * dp.kcomp, vec) * vec;
+        dp.kblock = dp.kodd + (int)cdiv(no * dp.kcomp, vec) * vec;
+    }
     if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, dp.tune.x_rows > 0 ? dp.tune.x_rows : 16)) {
         c.last_error = "x line FFT does not fit in one workgroup";
         return false;
@@ -1879,13 +1922,16 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
     newell_octant(dp.nx, dp.ny, dp.three_d ? dp.nz : 1, c.dx, c.dy, c.dz, oct);
     cplx<double> *spec = nullptr;
     if (!newell_forward(c, dp, oct, &spec, c.last_error)) return false;
-    dp.kern_elems = (uint64_t)dp.kcomp * dp.kz_n * dp.hx * (dp.ny + 1);
-    bool ok = ctx_alloc(c, &dp.kern, dp.kern_elems * sizeof(Real), "demag tensor");
+    dp.kern_elems = (uint64_t)dp.hx * (dp.ny + 1) * dp.kblock;
+    const int64_t entries = (int64_t)dp.kcomp * dp.kz_n * dp.hx * (dp.ny + 1);
+    bool ok = ctx_alloc(c, &dp.kern, dp.kern_elems * sizeof(Real), "demag tensor") &&
+              hip_ok(hipMemsetAsync(dp.kern, 0, dp.kern_elems * sizeof(Real), c.stream), "hipMemsetAsync(tensor)",
+                     c.last_error);
     if (ok) {
         const int pz = dp.three_d ? dp.pz : 1;
-        hipLaunchKernelGGL((extract_octant_kernel<Real>), dim3((unsigned)cdiv(dp.kern_elems, 256)), dim3(256), 0,
-                           c.stream, spec, static_cast<Real *>(dp.kern), dp.kz_n, dp.hx, dp.ny, pz, dp.py,
-                           (int64_t)dp.kern_elems);
+        hipLaunchKernelGGL((extract_octant_kernel<Real>), dim3((unsigned)cdiv(entries, 256)), dim3(256), 0,
+                           c.stream, spec, static_cast<Real *>(dp.kern), dp.kz_n, dp.hx, dp.ny, pz, dp.py, entries,
+                           dp.kblock, dp.kodd, dp.kcomp);
         ok = hip_ok(hipStreamSynchronize(c.stream), "extract octant", c.last_error);
     }
     (void)hipFree(spec);
