@@ -220,6 +220,7 @@ int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedu
     cp->dp.tune.zc = t.zc;
     cp->dp.tune.z_pipe = t.z_pipe;
     cp->dp.tune.z_wide = t.z_wide;
+    cp->dp.tune.z_kstage = t.z_kstage;
     return FULLMAG_FDM_OK;
 }
 

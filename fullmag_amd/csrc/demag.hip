@@ -586,6 +586,7 @@ struct ZArgs {
     int pitch;             // distance between kx rows of the spectrum (>= py)
     int xcd;               // renumber workgroups per XCD
     int half, groups;      // py / 2; groups of 16 direct ky (0..half) per kx
+    int kstage;            // staged tile: tensor halves through per-wave LDS
     KernelView<Real> K;
 };
 
@@ -805,9 +806,19 @@ __device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*t
 // zconv_pair_kernel), in place in LDS; lane pair t/2 takes columns t/2,
 // t/2 + 128, ...  A column is read and written only by its own lane pair,
 // which sits in one wave, so no barrier separates the reads from the writes.
+// Per-wave LDS staging of the tensor halves (FP32): a column's block is
+// stored compactly (KS reals: NE even-oz then NO odd-oz entries) for the 32
+// columns of each wave.
+template <typename Real, int NZ>
+struct ZTensorStage {
+    static constexpr int NE = NZ / 2 + 1, NO = (NZ + 1) / 2, KS = 6 * (NE + NO);
+    static constexpr bool ON = sizeof(Real) == 4;
+    static constexpr int ELEMS = ON ? 4 * 32 * KS : 1;  // four waves of 32 columns
+};
+
 template <typename Real, int NZ, int TW = kZTile>
 __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> (*tile)[TW + ZTile<Real, NZ, TW>::V],
-                                               int64_t col0, int t) {
+                                               int64_t col0, int t, Real *kstage = nullptr) {
     using C = cplx<Real>;
     const int p = t & 1;
 #pragma unroll 1
@@ -837,6 +848,30 @@ __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> 
         const int oy = ky <= a.K.ny ? ky : a.py - ky;
         const Real *blk = a.K.k + (valid ? ((int64_t)kx * (a.K.ny + 1) + oy) * a.K.kb + (p ? B0 : 0) : 0);
         const Real sy = ky <= a.K.ny ? Real(1) : Real(-1);
+        // staged: the lane copies its half of the block into this wave's
+        // LDS region with 16-byte loads, then reads entries by offset
+        using S = ZTensorStage<Real, NZ>;
+        Real *kw = nullptr;
+        if constexpr (S::ON) {
+            if (kstage) {
+                kw = kstage + ((t >> 6) * 32 + ((t >> 1) & 31)) * S::KS + (p ? 6 * S::NE : 0);
+                if (valid) {
+                    const float4 *src = reinterpret_cast<const float4 *>(blk);
+                    const int nv = p ? (6 * S::NO + 3) / 4 : (6 * S::NE + 3) / 4;
+                    const int nreal = p ? 6 * S::NO : 6 * S::NE;
+#pragma unroll
+                    for (int e = 0; e < (6 * S::NE + 3) / 4; ++e) {
+                        if (e < nv) {
+                            const float4 q = src[e];
+                            // compact copy: drop the half's padding (8-byte stores)
+                            if (4 * e + 1 < nreal) reinterpret_cast<float2 *>(kw)[2 * e] = float2{q.x, q.y};
+                            if (4 * e + 3 < nreal) reinterpret_cast<float2 *>(kw)[2 * e + 1] = float2{q.z, q.w};
+                        }
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
 #pragma unroll
         for (int k = 0; k < NZ; ++k) {
             Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
@@ -844,8 +879,18 @@ __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> 
                 const int kz0 = 2 * k, kz1 = 2 * k + 1;
                 const int i0 = (kz0 <= NZ ? kz0 : 2 * NZ - kz0) >> 1, i1 = (kz1 <= NZ ? kz1 : 2 * NZ - kz1) >> 1;
                 const Real sz = p ? (kz1 <= NZ ? Real(1) : Real(-1)) : (kz0 <= NZ ? Real(1) : Real(-1));
-                const V2 *e = reinterpret_cast<const V2 *>(blk + 6 * (p ? i1 : i0));
-                const V2 t01 = e[0], t23 = e[1], t45 = e[2];
+                V2 t01, t23, t45;
+                if (kstage) {
+                    const V2 *e = reinterpret_cast<const V2 *>(kw + 6 * (p ? i1 : i0));
+                    t01 = e[0];
+                    t23 = e[1];
+                    t45 = e[2];
+                } else {
+                    const V2 *e = reinterpret_cast<const V2 *>(blk + 6 * (p ? i1 : i0));
+                    t01 = e[0];
+                    t23 = e[1];
+                    t45 = e[2];
+                }
                 tt[0] = t01.x;
                 tt[1] = t01.y;
                 tt[2] = t23.x;
@@ -872,7 +917,9 @@ __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> 
 template <typename Real, int NZ, int TW = kZTile>
 __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
     using TL = ZTile<Real, NZ, TW>;
+    using S = ZTensorStage<Real, NZ>;
     __shared__ cplx<Real> tile[TL::ROWS][TW + TL::V];
+    __shared__ __align__(16) Real kstage[S::ELEMS];
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int64_t col0 = (int64_t)blk * TW;
     const int t = (int)threadIdx.x;
@@ -882,7 +929,7 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
         in.put(tile, t);
     }
     __syncthreads();
-    ztile_convolve<Real, NZ, TW>(a, tile, col0, t);
+    ztile_convolve<Real, NZ, TW>(a, tile, col0, t, S::ON && TW == kZTile && a.kstage ? kstage : nullptr);
     __syncthreads();
     ztile_store<Real, NZ, TW>(a, tile, col0, t);
 }
@@ -1519,6 +1566,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.K = K;
                 za.xcd = dp.tune.xcd;
                 za.half = dp.py / 2;
+                za.kstage = dp.tune.z_kstage;
                 za.groups = dp.tune.z_mirror ? (int)cdiv(za.half + 1, 64) : 0;
                 const dim3 zg((unsigned)(za.groups ? (uint64_t)dp.hx * za.groups : cdiv(2 * za.columns, 256))),
                     zb(256);
@@ -1692,6 +1740,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "z_stage") t.z_stage = v;
             else if (k == "z_pipe") t.z_pipe = v;
             else if (k == "z_wide") t.z_wide = v;
+            else if (k == "z_kstage") t.z_kstage = v;
             else if (k == "x_pipe") t.x_pipe = v;
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
