@@ -253,6 +253,7 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
     }
     __syncthreads();
     L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
+    __syncthreads();  // the unpack reads every line
     (void)LINES;
     const int hx = a.g.hx;
     for (int idx = tid; idx < hx * a.xr; idx += nthr) {
@@ -363,6 +364,7 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_pipe_kernel(XArgs<Real> a, Src
         __syncthreads();
         if (it + 1 < a.iters) fetch(yb + 1);
         L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
+        __syncthreads();  // the unpack reads every line
         if (yb < a.yblocks) {
             const int y0 = yb * a.xr;
             for (int idx = tid; idx < hx * a.xr; idx += nthr) {
@@ -1058,6 +1060,7 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
     }
     __syncthreads();
     L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    __syncthreads();  // the tensor product and the stores cross lines
     if (a.conv) {
         for (int idx = tid; idx < a.tile * n; idx += nthr) {
             const int j = idx % a.tile, e = idx / a.tile;
@@ -1072,6 +1075,7 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
         }
         __syncthreads();
         L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
+        __syncthreads();
     }
     for (int idx = tid; idx < nslots * a.n_out; idx += nthr) {
         const int s = idx % nslots, e = idx / nslots;

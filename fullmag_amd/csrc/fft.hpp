@@ -293,6 +293,22 @@ inline bool p2_matches(const LinePlan &lp, int n) {
 
 __host__ __device__ constexpr int p2slot(int i) { return i ^ ((i >> 4) & 15); }
 
+// Ordering point between the reads and the writes of a pass.  With N / 16
+// <= 64 lanes per line, every line lies inside one wavefront, whose LDS
+// operations execute in program order, so a wavefront-scope fence (which
+// also keeps the compiler from moving LDS accesses across it) replaces the
+// workgroup barrier and the other lines' waves run on independently.
+// Callers that read other lines after the transform add their own barrier.
+template <int N>
+__device__ __forceinline__ void pass_sync() {
+    if constexpr (N / 16 <= 64) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        __syncthreads();
+    }
+}
+
 // `lx` XORs every slot of the line (a per-line bank rotation for workgroups
 // that read several lines at one element index; 0 or a multiple of 4 < 32).
 template <typename T, int N, int R, int NS>
@@ -312,12 +328,12 @@ __device__ __forceinline__ void stockham_pass_p2(cplx<T> *line, int t, const cpl
         dft_any<T, R>(v[b], tw, N, inv);
         base[b] = (j / NS) * NS * R + k;
     }
-    __syncthreads();
+    pass_sync<N>();
 #pragma unroll
     for (int b = 0; b < MB; ++b)
 #pragma unroll
         for (int r = 0; r < R; ++r) line[p2slot(base[b] + r * NS) ^ lx] = v[b][r];
-    __syncthreads();
+    pass_sync<N>();
 }
 
 template <typename T, int N, int P, int NS>
