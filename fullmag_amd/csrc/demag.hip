@@ -238,8 +238,16 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
                     if (ya + 1 < a.rows) v[e].y = src(c, z, ya + 1, x);
                 }
             }
+            if constexpr (p2_first_from_regs<N>()) {
+                // the loaded elements are the first radix-16 butterfly's inputs
+                p2_first_pass_regs<Real, N>(line, t, v, a.tw, false, lx);
+                p2_passes<Real, N, 1, 16>(line, t, a.tw, false, lx);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 16; ++e) line[L::slot(a.lp, t + e * (N / 16), lx)] = v[e];
+                for (int e = 0; e < 16; ++e) line[L::slot(a.lp, t + e * (N / 16), lx)] = v[e];
+                __syncthreads();
+                L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
+            }
         } else {
             for (int x = t; x < n; x += T) {
                 cplx<Real> v = {Real(0), Real(0)};
@@ -249,10 +257,10 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
                 }
                 line[L::slot(a.lp, x, lx)] = v;
             }
+            __syncthreads();
+            L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
         }
     }
-    __syncthreads();
-    L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
     __syncthreads();  // the unpack reads every line
     (void)LINES;
     const int hx = a.g.hx;
@@ -510,14 +518,24 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
     fetch(first);
     for (int it = 0; it < a.iters; ++it) {
         const int64_t l = first + (int64_t)it * G;
+        if constexpr (p2_first_from_regs<N>()) {
+            // the loaded elements are the first radix-16 butterfly's inputs
+            cplx<Real> v[16];
 #pragma unroll
-        for (int e = 0; e < NI; ++e) line[p2slot(t + e * T)] = buf[e];
-        if (IN_DIV > 1)
+            for (int e = 0; e < 16; ++e) v[e] = e < NI ? buf[e] : cplx<Real>{Real(0), Real(0)};
+            if (it + 1 < a.iters) fetch(l + G);
+            p2_first_pass_regs<Real, N>(line, t, v, a.tw, a.inverse != 0, 0);
+            p2_passes<Real, N, 1, 16>(line, t, a.tw, a.inverse != 0, 0);
+        } else {
 #pragma unroll
-            for (int e = NI; e < N / T; ++e) line[p2slot(t + e * T)] = cplx<Real>{Real(0), Real(0)};
-        __syncthreads();
-        if (it + 1 < a.iters) fetch(l + G);
-        p2_passes<Real, N, 0, 1>(line, t, a.tw, a.inverse != 0, 0);
+            for (int e = 0; e < NI; ++e) line[p2slot(t + e * T)] = buf[e];
+            if (IN_DIV > 1)
+#pragma unroll
+                for (int e = NI; e < N / T; ++e) line[p2slot(t + e * T)] = cplx<Real>{Real(0), Real(0)};
+            __syncthreads();
+            if (it + 1 < a.iters) fetch(l + G);
+            p2_passes<Real, N, 0, 1>(line, t, a.tw, a.inverse != 0, 0);
+        }
         if (l < a.lines) {
             cplx<Real> *g = a.data + l * a.line_pitch;
             for (int e = t; e < a.n_out; e += T) g[e] = line[p2slot(e)];

@@ -345,6 +345,25 @@ __device__ __forceinline__ void p2_passes(cplx<T> *line, int t, const cplx<T> *t
     }
 }
 
+// The first pass of a fixed schedule whose first radix is 16, fed from
+// registers: lane t holds elements t + r N/16 (r < 16), exactly the inputs
+// of its butterfly, so the line need not be written to LDS and read back
+// before it.  Continue with p2_passes<T, N, 1, 16>.
+template <int N>
+constexpr bool p2_first_from_regs() {
+    return N >= 256 && p2_radix(N, 0) == 16;
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void p2_first_pass_regs(cplx<T> *line, int t, cplx<T> v[16], const cplx<T> *tw,
+                                                   bool inv, int lx) {
+    static_assert(p2_first_from_regs<N>(), "first radix 16");
+    dft_any<T, 16>(v, tw, N, inv);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) line[p2slot(t * 16 + r) ^ lx] = v[r];
+    pass_sync<N>();
+}
+
 // Line access policy: N > 0 a fixed power-of-two schedule (N / 16 lanes per
 // line), N == 0 the runtime plan.
 template <int N>
