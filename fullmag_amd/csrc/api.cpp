@@ -203,24 +203,20 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
             return c;
         if (c->has_mask) {
             if (!ctx_alloc(*c, (void **)&c->d_mask, n, "active mask")) return c;
-            if (!hip_ok(hipMemcpy(c->d_mask, c->mask_host.data(), n, hipMemcpyHostToDevice), "hipMemcpy(mask)",
-                        c->last_error))
+            if (!ctx_upload(*c, c->d_mask, c->mask_host.data(), n, "hipMemcpy(mask)", c->last_error))
                 return c;
         }
         if (c->use_lut) {
             if (!ctx_alloc(*c, (void **)&c->d_region, n * 4, "region mask") ||
                 !ctx_alloc(*c, (void **)&c->d_lut, lut.size() * 8, "exchange table"))
                 return c;
-            if (!hip_ok(hipMemcpy(c->d_region, regions.data(), n * 4, hipMemcpyHostToDevice), "hipMemcpy(regions)",
-                        c->last_error) ||
-                !hip_ok(hipMemcpy(c->d_lut, lut.data(), lut.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(lut)",
-                        c->last_error))
+            if (!ctx_upload(*c, c->d_region, regions.data(), n * 4, "hipMemcpy(regions)", c->last_error) ||
+                !ctx_upload(*c, c->d_lut, lut.data(), lut.size() * 8, "hipMemcpy(lut)", c->last_error))
                 return c;
         }
         if (c->has_ms_cell) {
             if (!ctx_alloc(*c, (void **)&c->d_ms_cell, n * 8, "per-cell Ms")) return c;
-            if (!hip_ok(hipMemcpy(c->d_ms_cell, opt->ms_cell, n * 8, hipMemcpyHostToDevice), "hipMemcpy(Ms)",
-                        c->last_error))
+            if (!ctx_upload(*c, c->d_ms_cell, opt->ms_cell, n * 8, "hipMemcpy(Ms)", c->last_error))
                 return c;
         }
         if (!alloc_integrator_buffers(*c)) return c;

@@ -20,6 +20,14 @@ bool ctx_alloc(Context &c, void **ptr, size_t bytes, const char *what) {
     return true;
 }
 
+// Host-to-device copy ordered on the backend's stream and complete on
+// return.  The stream is non-blocking (it does not synchronise with the null
+// stream), so setup copies go through it rather than a null-stream hipMemcpy.
+bool ctx_upload(Context &c, void *dst, const void *src, size_t bytes, const char *what, std::string &err) {
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream), what, err) &&
+           hip_ok(hipStreamSynchronize(c.stream), what, err);
+}
+
 void ctx_free_all(Context &c) {
     if (c.stream) hipStreamSynchronize(c.stream);
     for (void *p : c.allocations) hipFree(p);

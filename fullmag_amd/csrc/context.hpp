@@ -174,6 +174,7 @@ struct Context {
 
 // context.cpp
 bool ctx_alloc(Context &c, void **ptr, size_t bytes, const char *what);
+bool ctx_upload(Context &c, void *dst, const void *src, size_t bytes, const char *what, std::string &err);
 void ctx_free_all(Context &c);
 bool ctx_upload_m(Context &c, const void *host, bool host_is_f64);
 bool ctx_query_device_info(Context &c);

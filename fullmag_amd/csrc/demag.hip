@@ -1718,11 +1718,11 @@ bool upload_twiddles(Context &c, int n, void **dst) {
     const size_t bytes = w.size() * sizeof(cplx<Real>);
     if (!ctx_alloc(c, dst, bytes, "twiddle table")) return false;
     if (sizeof(Real) == sizeof(double)) {
-        return hip_ok(hipMemcpy(*dst, w.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)", c.last_error);
+        return ctx_upload(c, *dst, w.data(), bytes, "hipMemcpy(twiddles)", c.last_error);
     }
     std::vector<cplx<float>> wf(w.size());
     for (size_t i = 0; i < w.size(); ++i) wf[i] = {(float)w[i].x, (float)w[i].y};
-    return hip_ok(hipMemcpy(*dst, wf.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)", c.last_error);
+    return ctx_upload(c, *dst, wf.data(), bytes, "hipMemcpy(twiddles)", c.last_error);
 }
 
 bool plan_lines(DemagPlan &dp, std::string &err) {
@@ -1787,13 +1787,11 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
     const size_t ce = sizeof(cplx<double>);
     bool ok = hip_ok(hipMalloc(&d_oct, oct.size() * sizeof(double)), "hipMalloc(newell octant)", err) &&
               hip_ok(hipMalloc(&d_spec, spec_bytes), "hipMalloc(newell spectrum)", err) &&
-              hip_ok(hipMemcpy(d_oct, oct.data(), oct.size() * sizeof(double), hipMemcpyHostToDevice),
-                     "hipMemcpy(newell octant)", err);
+              ctx_upload(c, d_oct, oct.data(), oct.size() * sizeof(double), "hipMemcpy(newell octant)", err);
     auto up = [&](int n, void **dst) {
         std::vector<cplx<double>> w = twiddle_table(n);
         return hip_ok(hipMalloc(dst, w.size() * sizeof(cplx<double>)), "hipMalloc(twiddles)", err) &&
-               hip_ok(hipMemcpy(*dst, w.data(), w.size() * sizeof(cplx<double>), hipMemcpyHostToDevice),
-                      "hipMemcpy(twiddles)", err);
+               ctx_upload(c, *dst, w.data(), w.size() * sizeof(cplx<double>), "hipMemcpy(twiddles)", err);
     };
     ok = ok && up(dp.px, &twx) && up(dp.py, &twy) && (!dp.three_d || up(dp.pz, &twz));
     int xr = 2, xls = dp.px;
@@ -1933,11 +1931,10 @@ bool upload_kern(Context &c, const std::vector<double> &kern) {
     const size_t bytes = kern.size() * sizeof(Real);
     if (!ctx_alloc(c, &dp.kern, bytes, "demag tensor")) return false;
     if (sizeof(Real) == sizeof(double))
-        return hip_ok(hipMemcpy(dp.kern, kern.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(tensor)",
-                      c.last_error);
+        return ctx_upload(c, dp.kern, kern.data(), bytes, "hipMemcpy(tensor)", c.last_error);
     std::vector<float> f(kern.size());
     for (size_t i = 0; i < kern.size(); ++i) f[i] = (float)kern[i];
-    return hip_ok(hipMemcpy(dp.kern, f.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(tensor)", c.last_error);
+    return ctx_upload(c, dp.kern, f.data(), bytes, "hipMemcpy(tensor)", c.last_error);
 }
 
 template <typename Real>
