@@ -1530,12 +1530,16 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         auto go = [&](auto PC) {
             constexpr int PP = decltype(PC)::value;
             auto launch = [&](auto kernel) {
-                // above 64 KB of dynamic LDS the kernel must opt in (once per kernel)
-                // (idempotent; a failed opt-in surfaces as the launch error)
-                static std::atomic<bool> opted{false};
-                if (!opted.load(std::memory_order_relaxed))
-                    opted = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess;
+                // above 64 KB of dynamic LDS the kernel must opt in, once per
+                // kernel and device (idempotent; a failed opt-in surfaces as
+                // the launch error)
+                static std::atomic<uint64_t> opted{0};
+                const uint64_t bit = c.device < 64 ? uint64_t(1) << c.device : 0;
+                if (!bit || !(opted.load(std::memory_order_relaxed) & bit)) {
+                    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess)
+                        opted.fetch_or(bit, std::memory_order_relaxed);
+                }
                 hipLaunchKernelGGL(kernel, dim3(dp.hx - 1), dim3(2 * PP), lds, c.stream, sa);
             };
             auto pick = [&](auto NZC) { launch(yz_slab_kernel<Real, decltype(NZC)::value, PP, 2>); };
