@@ -1080,16 +1080,46 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
     L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
     __syncthreads();  // the tensor product and the stores cross lines
     if (a.conv) {
-        for (int idx = tid; idx < a.tile * n; idx += nthr) {
-            const int j = idx % a.tile, e = idx / a.tile;
+        // one pass per octant entry oz <= nz, applied at kz = oz and at its
+        // mirror kz = pz - oz (xz and yz change sign): each tensor entry is
+        // read once, as three 2-vector loads
+        using V2 = typename std::conditional<sizeof(Real) == 4, float2, double2>::type;
+        const int nzo = a.K.nz + 1;
+        if (!a.K.k) {
+            // k-space projection fallback: the tensor is computed per kz
+            for (int idx = tid; idx < a.tile * n; idx += nthr) {
+                const int j = idx % a.tile, e = idx / a.tile;
+                const int64_t col = col0 + j;
+                if (col >= a.columns) continue;
+                const int kx = (int)(col / a.pitch), ky = (int)(col - (int64_t)kx * a.pitch);
+                if (ky >= a.py) continue;
+                Real t[6];
+                tensor_at(a.K, kx, ky, e, t);
+                const int se = L::slot(a.lp, e);
+                apply_tensor(t, lds[j * ls + se], lds[(a.tile + j) * ls + se], lds[(2 * a.tile + j) * ls + se]);
+            }
+        }
+        for (int idx = tid; a.K.k && idx < a.tile * nzo; idx += nthr) {
+            const int j = idx % a.tile, oz = idx / a.tile;
             const int64_t col = col0 + j;
             if (col >= a.columns) continue;
             const int kx = (int)(col / a.pitch), ky = (int)(col - (int64_t)kx * a.pitch);
             if (ky >= a.py) continue;
-            Real t[6];
-            tensor_at(a.K, kx, ky, e, t);
-            const int se = L::slot(a.lp, e);
+            const bool ylo = ky <= a.K.ny;
+            const int oy = ylo ? ky : a.py - ky;
+            const Real sy = ylo ? Real(1) : Real(-1);
+            const V2 *e = reinterpret_cast<const V2 *>(a.K.k +
+                                                       kern_at(a.K.kb, a.K.kodd, a.K.kcomp, a.K.ny + 1, 0, oz, kx, oy));
+            const V2 t01 = e[0], t23 = e[1], t45 = e[2];
+            Real t[6] = {t01.x, t01.y, t23.x, sy * t23.y, t45.x, sy * t45.y};
+            int se = L::slot(a.lp, oz);
             apply_tensor(t, lds[j * ls + se], lds[(a.tile + j) * ls + se], lds[(2 * a.tile + j) * ls + se]);
+            if (oz > 0 && oz < a.K.pz - oz) {
+                t[4] = -t[4];
+                t[5] = -t[5];
+                se = L::slot(a.lp, a.K.pz - oz);
+                apply_tensor(t, lds[j * ls + se], lds[(a.tile + j) * ls + se], lds[(2 * a.tile + j) * ls + se]);
+            }
         }
         __syncthreads();
         L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, true);
