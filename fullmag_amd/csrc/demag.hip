@@ -1068,13 +1068,28 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
     const int cbase = a.conv ? 0 : blockIdx.y;
     const int nslots = ncomp * a.tile;
     const int64_t col0 = (int64_t)blockIdx.x * a.tile;
-    for (int idx = tid; idx < nslots * n; idx += nthr) {
-        const int s = idx % nslots, e = idx / nslots;
-        const int c = s / a.tile;
-        const int64_t col = col0 + (s - c * a.tile);
-        cplx<Real> v = {Real(0), Real(0)};
-        if (e < a.n_in && col < a.columns) v = a.data[(cbase + c) * a.comp_stride + e * a.zstride + col];
-        lds[s * ls + L::slot(a.lp, e)] = v;
+    // batches of U loads per lane in flight before their LDS writes
+    constexpr int U = 8;
+    for (int base = tid; base < nslots * n; base += U * nthr) {
+        cplx<Real> v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * nthr;
+            const int s = idx % nslots, e = idx / nslots;
+            const int c = s / a.tile;
+            const int64_t col = col0 + (s - c * a.tile);
+            v[u] = {Real(0), Real(0)};
+            if (idx < nslots * n && e < a.n_in && col < a.columns)
+                v[u] = a.data[(cbase + c) * a.comp_stride + e * a.zstride + col];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = base + u * nthr;
+            if (idx < nslots * n) {
+                const int s = idx % nslots, e = idx / nslots;
+                lds[s * ls + L::slot(a.lp, e)] = v[u];
+            }
+        }
     }
     __syncthreads();
     L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
