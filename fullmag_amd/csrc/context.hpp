@@ -28,7 +28,7 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, z_pipe, z_wide, z_kstage, x_pipe, y_pad, zc, yz_slab).
+// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
@@ -42,6 +42,7 @@ struct DemagTuning {
     int z_pipe = 0;    // > 0: persistent staged z convolution, tiles per workgroup
     int z_wide = 0;    // 1: staged z convolution tiles of 256 columns (else 128)
     int z_kstage = 1;  // 1: staged z tile reads the tensor through per-wave LDS (FP32)
+    int z_tile = 0;    // > 0: at most this many columns per long-z (LDS line) workgroup
     int x_pipe = 0;    // row blocks per workgroup of the pipelined x kernels (0: off)
     int y_pad = 0;     // complex slots of padding after each kx row of the 3-D spectrum
     int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)

@@ -1408,10 +1408,11 @@ int pick_lpw(const LinePlan &lp, size_t ce, int nlines_per_unit = 1) {
     return lpw;
 }
 
-int pick_ztile(const LinePlan &lp, int ncomp, size_t ce) {
+// Columns per long-z workgroup: the largest power of two whose lines fit in
+// `budget` bytes of LDS and kMaxBlock lanes.
+int pick_ztile(const LinePlan &lp, int ncomp, size_t ce, size_t budget = 65536) {
     int tile = 64;
-    while (tile > 1 && ((size_t)ncomp * tile * (lp.n + kPad) * ce > lds_budget() ||
-                        ncomp * tile * lp.threads > kMaxBlock))
+    while (tile > 1 && ((size_t)ncomp * tile * (lp.n + kPad) * ce > budget || ncomp * tile * lp.threads > kMaxBlock))
         tile >>= 1;
     return tile;
 }
@@ -1812,6 +1813,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "z_pipe") t.z_pipe = v;
             else if (k == "z_wide") t.z_wide = v;
             else if (k == "z_kstage") t.z_kstage = v;
+            else if (k == "z_tile") t.z_tile = v;
             else if (k == "x_pipe") t.x_pipe = v;
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
@@ -2009,7 +2011,11 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
             return false;
         }
         if (dp.pz > z_reg_max<Real>() || dp.tune.z_lds) {
-            dp.z_tile = pick_ztile(dp.lz, 3, ce);
+            // at most 32 KB per workgroup: five or more workgroups share a CU
+            // (measured at pz = 128: 209 us with 8 columns against 265 us
+            // with 16 at 48 KB)
+            dp.z_tile = pick_ztile(dp.lz, 3, ce, 32768);
+            if (dp.tune.z_tile > 0) dp.z_tile = std::min(dp.z_tile, dp.tune.z_tile);
             if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > lds_budget()) {
                 c.last_error = "z line FFT does not fit in LDS";
                 return false;

@@ -204,10 +204,13 @@ def test_demag_field_all_shapes(fa, oracle_mod, cells, kernel):
                                    rtol=1e-12)
 
 
-@pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11)])
-def test_demag_field_long_z(fa, oracle_mod, cells):
+@pytest.mark.parametrize("tune", ["", "z_tile=2"])
+@pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11), (16, 12, 64)])
+def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
     """nz > 8: the z convolution runs as LDS line transforms (padded lengths
-    24, 18, 32 and 22, with radices 3 and 11)."""
+    24, 18, 32, 22 and 128, with radices 3 and 11), at the automatic and at
+    a narrow column tile."""
+    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
     nx, ny, nz = cells
     n = nx * ny * nz
     m0 = fa.generate_random_unit_vectors(5 + n, n)
