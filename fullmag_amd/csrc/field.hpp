@@ -290,9 +290,11 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
 #pragma unroll
         for (int c = 0; c < 3; ++c) md[c] = double(cc[c]);
         const double w = f.ms_cell ? f.ms_cell[i] : 1.0;
-        neu_add(acc[0], acc[1], w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2])));
-        neu_add(acc[2], acc[3], w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2])));
-        if (f.has_ext) neu_add(acc[4], acc[5], w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz));
+        // a lane's column (at most one z chunk) is summed plainly; lanes and
+        // workgroups merge with compensation (block_reduce, finalize)
+        acc[0] += w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2]));
+        acc[2] += w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2]));
+        if (f.has_ext) acc[4] += w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz);
         const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
         const double d2 = double(hdv[0]) * hdv[0] + double(hdv[1]) * hdv[1] + double(hdv[2]) * hdv[2];
         const double k2 = double(k[0]) * k[0] + double(k[1]) * k[1] + double(k[2]) * k[2];
