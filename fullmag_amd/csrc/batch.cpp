@@ -177,6 +177,7 @@ int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen
         std::vector<int> values;
     };
     std::vector<Knob> knobs = {{"fixed", &t.fixed, {1, 0}},         {"y_pipe", &t.y_pipe, {4, 0, 1, 2, 8}},
+                               {"y_pipe_f", &t.y_pipe_f, {2, 1, 4, 8}},
                                {"z_quad", &t.z_quad, {0, 1}},         {"z_mirror", &t.z_mirror, {0, 1}},
                                {"xcd", &t.xcd, {1, 0}},               {"zc", &t.zc, {0, 1, 2, 4}}};
     double best = fm::integrator_probe_ms(c, reps);
@@ -214,6 +215,7 @@ int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedu
     // only launch-time knobs; the planned geometry stays
     cp->dp.tune.fixed = t.fixed;
     cp->dp.tune.y_pipe = t.y_pipe;
+    cp->dp.tune.y_pipe_f = t.y_pipe_f;
     cp->dp.tune.z_quad = t.z_quad;
     cp->dp.tune.z_mirror = t.z_mirror;
     cp->dp.tune.xcd = t.xcd;

@@ -28,7 +28,7 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab).
+// fixed, z_quad, xcd, z_mirror, y_pipe, y_pipe_f, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
@@ -38,6 +38,7 @@ struct DemagTuning {
     int xcd = 1;     // 0: no per-XCD renumbering of workgroups
     int z_mirror = 0;  // 1: two-lane z kernel pairs mirrored ky columns in one workgroup
     int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
+    int y_pipe_f = 2;  // > 0: lines per workgroup of the pipelined forward y pass (half-zero input)
     int z_stage = 1;   // 1: register z convolution staged through LDS tiles
     int z_pipe = 0;    // > 0: persistent staged z convolution, tiles per workgroup
     int z_wide = 0;    // 1: staged z convolution tiles of 256 columns (else 128)

@@ -1499,7 +1499,7 @@ void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) 
             const int div = la.n_in == N ? 1 : (2 * la.n_in == N ? 2 : 0);
             if (dp.tune.y_pipe > 0 && div > 0) {
                 constexpr int T = N / 16, G = T >= 64 ? 1 : 64 / T;
-                la.iters = dp.tune.y_pipe;
+                la.iters = div == 2 && dp.tune.y_pipe_f > 0 ? dp.tune.y_pipe_f : dp.tune.y_pipe;
                 const dim3 grid((unsigned)cdiv(la.lines, (int64_t)G * la.iters)), block((unsigned)(G * T));
                 const size_t lds = (size_t)G * (N + kPad) * ce;
                 if (div == 1)
@@ -1809,6 +1809,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "xcd") t.xcd = v;
             else if (k == "z_mirror") t.z_mirror = v;
             else if (k == "y_pipe") t.y_pipe = v;
+            else if (k == "y_pipe_f") t.y_pipe_f = v;
             else if (k == "z_stage") t.z_stage = v;
             else if (k == "z_pipe") t.z_pipe = v;
             else if (k == "z_wide") t.z_wide = v;

@@ -227,7 +227,7 @@ SCHEDULES = ["", "z_lds=1", "fixed=0", "z_quad=1", "z_quad=0", "z_mirror=1", "z_
              "x_pipe=2", "x_pipe=3,x_rows=4", "x_pipe=4,x_rows=8", "y_pad=16", "y_pad=8,z_stage=0",
              "y_pad=24,z_quad=1", "y_pad=8,z_lds=1", "y_pad=8,z_mirror=1", "zc=1", "zc=3,xcd=0", "yz_slab=1",
              "yz_slab=1,z_quad=1", "z_pipe=1", "z_pipe=3", "z_pipe=8,xcd=0", "z_pipe=2,y_pad=8", "z_wide=1",
-             "z_wide=1,y_pad=8,xcd=0", "x_rows=4", "x_rows=2,y_pad=8", "z_kstage=0",
+             "z_wide=1,y_pad=8,xcd=0", "x_rows=4", "x_rows=2,y_pad=8", "y_pipe_f=1", "y_pipe_f=4,y_pipe=1", "z_kstage=0",
              "z_kstage=0,y_pad=8"]
 
 
