@@ -262,20 +262,49 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
         }
     }
     __syncthreads();  // the unpack reads every line
-    (void)LINES;
     const int hx = a.g.hx;
-    for (int idx = tid; idx < hx * a.xr; idx += nthr) {
-        const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
-        const int y = y0 + yl;
+    // lane (j, k) splits line j into rows 2j (A) and 2j + 1 (B) at kx = k;
+    // the two are adjacent in memory (one 16-byte store in FP32 when the
+    // row pitch is even)
+    if constexpr (sizeof(Real) == 8) {
+        // FP64: one 16-byte complex per lane, consecutive lanes on consecutive rows
+        for (int idx = tid; idx < hx * a.xr; idx += nthr) {
+            const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
+            const int y = y0 + yl;
+            if (y >= a.rows) continue;
+            const int j = yl >> 1, jx = L::line_xor(j);
+            const cplx<Real> zk = lds[j * ls + L::slot(a.lp, k, jx)];
+            const cplx<Real> zn = lds[j * ls + L::slot(a.lp, k == 0 ? 0 : n - k, jx)];
+            const Real h = Real(0.5);
+            cplx<Real> v;
+            if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
+            else v = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
+            spec[a.g.at(c, z, k, y)] = v;
+        }
+        return;
+    }
+    const bool pair16 = (a.g.pitch & 1) == 0;
+    for (int idx = tid; idx < hx * LINES; idx += nthr) {
+        const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+        const int y = y0 + 2 * j;
         if (y >= a.rows) continue;
-        const int j = yl >> 1, jx = L::line_xor(j);
+        const int jx = L::line_xor(j);
         const cplx<Real> zk = lds[j * ls + L::slot(a.lp, k, jx)];
         const cplx<Real> zn = lds[j * ls + L::slot(a.lp, k == 0 ? 0 : n - k, jx)];
         const Real h = Real(0.5);
-        cplx<Real> v;
-        if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
-        else v = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
-        spec[a.g.at(c, z, k, y)] = v;
+        const cplx<Real> A = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
+        const cplx<Real> B = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
+        cplx<Real> *dst = spec + a.g.at(c, z, k, y);
+        if (y + 1 < a.rows) {
+            if (pair16) {
+                *reinterpret_cast<float4 *>(dst) = float4{float(A.x), float(A.y), float(B.x), float(B.y)};
+            } else {
+                dst[0] = A;
+                dst[1] = B;
+            }
+        } else {
+            dst[0] = A;
+        }
     }
 }
 
