@@ -37,12 +37,12 @@ struct LinePlan {
 // each 16-element block.
 __host__ __device__ __forceinline__ int lslot(const LinePlan &lp, int i) { return lp.swz ? (i ^ ((i >> 4) & 15)) : i; }
 
-// Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13} and the lanes per line:
-// the fewest lanes such that no lane holds more than 16 complex values in any
-// pass, or `min_lanes` if larger (capped at the largest pass's butterfly
-// count); power-of-two radices are capped at max_radix (16, 8 or 4).  Returns
-// false (with `err`) for lengths with a prime factor above 13 or needing more
-// than 1024 lanes.
+// Chooses radices from {16, 8, 4, 2, 3, 5, 7, 11, 13}, plus any larger prime
+// factor of n as a direct-sum pass, and the lanes per line: the fewest lanes
+// such that no lane holds more than 16 complex values in any pass, or
+// `min_lanes` if larger (capped at the largest pass's butterfly count);
+// power-of-two radices are capped at max_radix (16, 8 or 4).  Returns false
+// (with `err`) for lengths needing more than kMaxPass passes or 1024 lanes.
 bool plan_line(int n, LinePlan &lp, std::string &err, int min_lanes = 1, int max_radix = 16);
 
 // w[t] = exp(-2 pi i t / n), t in [0, n), correctly rounded from long double.
@@ -239,6 +239,44 @@ __device__ __forceinline__ void stockham_pass(cplx<T> *line, const LinePlan &lp,
     __syncthreads();
 }
 
+// One Stockham pass of a prime radix R above 13, known only at run time.
+// Lane t owns the output slots o = t + m nt (m < 16; the planner gives such
+// a line at least n / 16 lanes).  Slot o = (j / ns) ns R + q ns + k of
+// butterfly j (k = j mod ns) is sum_r x[j + r n/R] w^(r e) with
+// e = k n/(ns R) + q n/R, the twiddle and the R-point DFT in one direct sum
+// over table roots (O(n R) work per pass).
+template <typename T>
+__device__ __forceinline__ void stockham_pass_prime(cplx<T> *line, const LinePlan &lp, int ns, int t, const cplx<T> *tw,
+                                                 bool inv, int R) {
+    const int n = lp.n, nt = lp.threads, nr = n / R, st = n / (ns * R);
+    cplx<T> out[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int o = t + m * nt;
+        if (o < n) {
+            const int k = o % ns, q = (o / ns) % R, jb = o / (ns * R);
+            const int j = jb * ns + k, e = k * st + q * nr;
+            cplx<T> acc = line[lslot(lp, j)];
+            int idx = e;
+            for (int r = 1; r < R; ++r) {
+                cplx<T> w = tw[idx];
+                if (inv) w.y = -w.y;
+                acc = cadd(acc, cmul(line[lslot(lp, j + r * nr)], w));
+                idx += e;
+                if (idx >= n) idx -= n;
+            }
+            out[m] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int o = t + m * nt;
+        if (o < n) line[lslot(lp, o)] = out[m];
+    }
+    __syncthreads();
+}
+
 // Full transform of one line.  Uniform across the workgroup.
 template <typename T>
 __device__ __forceinline__ void line_fft(cplx<T> *line, const LinePlan &lp, int t, const cplx<T> *tw, bool inv) {
@@ -255,7 +293,7 @@ __device__ __forceinline__ void line_fft(cplx<T> *line, const LinePlan &lp, int 
             case 11: stockham_pass<T, 11>(line, lp, ns, t, tw, inv); break;
             case 13: stockham_pass<T, 13>(line, lp, ns, t, tw, inv); break;
             case 16: stockham_pass<T, 16>(line, lp, ns, t, tw, inv); break;
-            default: break;
+            default: stockham_pass_prime<T>(line, lp, ns, t, tw, inv, R); break;
         }
         ns *= R;
     }

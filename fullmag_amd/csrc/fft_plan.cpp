@@ -65,11 +65,8 @@ bool plan_line(int n, LinePlan &lp, std::string &err, int min_lanes, int max_rad
     std::vector<int> odd;
     for (int p : {3, 5, 7, 11, 13})
         while (m % p == 0) { odd.push_back(p); m /= p; }
-    if (m != 1) {
-        err = "padded FFT length " + std::to_string(n) +
-              " has a prime factor above 13, which the line FFT does not support";
-        return false;
-    }
+    for (int p = 17; m > 1; p += 2)
+        while (m % p == 0) { odd.push_back(p); m /= p; }
     std::vector<int> rad;
     pow2_radices(k, false, rad, max_radix);
     for (int p : odd) rad.push_back(p);
@@ -84,6 +81,8 @@ bool plan_line(int n, LinePlan &lp, std::string &err, int min_lanes, int max_rad
         const int nr = n / r;
         lanes = std::max(lanes, (nr + mb - 1) / mb);
         most = std::max(most, nr);
+        // a prime pass above 13 owns up to 16 output slots per lane
+        if (r > 16) lanes = std::max(lanes, (n + 15) / 16), most = std::max(most, (n + 15) / 16);
     }
     lanes = std::max(lanes, std::min(min_lanes, most));
     if (lanes > 1024) {

@@ -204,6 +204,21 @@ def test_demag_field_all_shapes(fa, oracle_mod, cells, kernel):
                                    rtol=1e-12)
 
 
+@pytest.mark.parametrize("cells", [(17, 6, 1), (6, 19, 3), (5, 4, 17), (37, 3, 2), (3, 29, 1), (34, 17, 4)])
+@pytest.mark.parametrize("kernel", ["spectra", "device"])
+def test_demag_field_prime_factor_above_13(fa, oracle_mod, cells, kernel):
+    """Padded lengths 34, 38, 58, 68 and 74 carry a prime factor above 13,
+    transformed by the direct-sum prime pass (z at 34 through the LDS line
+    path), at FP64 against the oracle."""
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(23 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=1e6, a=1e-11, alpha=0.1,
+                         m0=m0, exchange=False, kernel=kernel)
+    hd = cpu.observe()["h_demag"]
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
+
+
 @pytest.mark.parametrize("tune", ["", "z_tile=2"])
 @pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11), (16, 12, 64)])
 def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
@@ -269,8 +284,25 @@ def test_demag_fused_slab_matches_oracle(fa, oracle_mod, monkeypatch, cells, tun
     assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12
 
 
+@pytest.mark.parametrize("cells,tune", [((13, 7, 1), ""), ((10, 9, 3), ""), ((10, 9, 3), "y_pad=3"),
+                                        ((64, 48, 8), "y_pad=1"), ((33, 17, 5), "x_rows=4"),
+                                        ((19, 17, 2), "")])
+def test_mixed_demag_odd_shapes_match_oracle(fa, oracle_mod, monkeypatch, cells, tune):
+    """FP32 demag (mixed precision) on odd grids and odd spectrum row pitches,
+    where the x forward unpack stores rows one at a time, against the FP64
+    oracle field to FP32 rounding."""
+    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(61 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=8.6e5, a=1.3e-11, alpha=0.05,
+                         m0=m0, exchange=False, kernel="device", precision="mixed")
+    hd = cpu.observe()["h_demag"]
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=2e-6 * np.abs(hd).max())
+
+
 @pytest.mark.parametrize("cells,tf", [((8, 6, 1), True), ((6, 5, 4), False), ((45, 3, 2), False),
-                                      ((7, 9, 1), False)])
+                                      ((7, 9, 1), False), ((17, 5, 1), True), ((5, 19, 3), False)])
 def test_in_library_newell_spectra_match_oracle(fa, oracle_mod, cells, tf):
     s_gpu = fa.compute_newell_kernel_spectra(*cells, 3e-9, 4e-9, 2e-9, thin_film_2d=tf)
     s_cpu = oracle_mod.newell_spectra(*cells, 3e-9, 4e-9, 2e-9, thin_film_2d=tf)
