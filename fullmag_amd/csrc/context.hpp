@@ -61,6 +61,7 @@ struct DemagPlan {
     int nx = 0, ny = 0, nz = 0;
     int px = 0, py = 0, pz = 1;
     int hx = 0;    // nx + 1 columns of the half spectrum
+    size_t lds_limit = 65536;  // LDS bytes one workgroup may allocate on this device
     int rows = 0;  // rows per plane of `spec`
     int kcomp = 0; // 6 (3-D) or 4 (2-D: xz = yz = 0 on the film plane)
     int kz_n = 1;  // octant planes in kz: nz + 1 (3-D) or 1

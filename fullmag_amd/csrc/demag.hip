@@ -586,8 +586,10 @@ struct YConvArgs {
     KernelView<Real> K;
 };
 
-template <typename Real>
-__global__ void __launch_bounds__(kMaxBlock) yconv_kernel(YConvArgs<Real> a) {
+// MAXB: kMaxBlock, or 1024 for lines too long for three of them in
+// kMaxBlock lanes.
+template <typename Real, int MAXB>
+__global__ void __launch_bounds__(MAXB) yconv_kernel(YConvArgs<Real> a) {
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
@@ -1409,8 +1411,10 @@ __global__ void extract_octant_kernel(const cplx<double> *spec, Real *kern, int 
 size_t lds_budget() { return 65536; }
 
 // Rows per x workgroup: the largest power of two <= 16 whose packed lines fit
-// in 64 KB of LDS (padded stride when that still fits) and 512 lanes.
-bool pick_x(const LinePlan &lp, size_t ce, int &xr, int &ls, int max_rows = 16) {
+// in 64 KB of LDS (padded stride when that still fits) and 512 lanes; a line
+// longer than 64 KB takes one packed line (two rows) per workgroup when it
+// fits in `limit`, the device's LDS per workgroup (160 KB on gfx950).
+bool pick_x(const LinePlan &lp, size_t ce, int &xr, int &ls, int max_rows = 16, size_t limit = 65536) {
     for (xr = max_rows; xr >= 2; xr >>= 1) {
         const int lines = xr / 2;
         if (lines * lp.threads > kMaxBlock) continue;
@@ -1420,6 +1424,13 @@ bool pick_x(const LinePlan &lp, size_t ce, int &xr, int &ls, int max_rows = 16) 
                 return true;
             }
     }
+    xr = 2;
+    if (lp.threads > kMaxBlock) return false;
+    for (int pad = kPad; pad >= 0; --pad)
+        if ((size_t)(lp.n + pad) * ce <= limit) {
+            ls = lp.n + pad;
+            return true;
+        }
     return false;
 }
 
@@ -1769,9 +1780,12 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         ya.g = g;
         ya.K = K;
         Timed t(c, "demag_y_conv");
-        hipLaunchKernelGGL((yconv_kernel<Real>), dim3((unsigned)cdiv(dp.hx, ya.kpw)),
-                           dim3((unsigned)(3 * ya.kpw * dp.ly.threads)),
-                           (size_t)3 * ya.kpw * (dp.ly.n + kPad) * ce, c.stream, ya);
+        const dim3 grid((unsigned)cdiv(dp.hx, ya.kpw)), block((unsigned)(3 * ya.kpw * dp.ly.threads));
+        const size_t lds = (size_t)3 * ya.kpw * (dp.ly.n + kPad) * ce;
+        if (block.x > (unsigned)kMaxBlock)
+            hipLaunchKernelGGL((yconv_kernel<Real, 1024>), grid, block, lds, c.stream, ya);
+        else
+            hipLaunchKernelGGL((yconv_kernel<Real, kMaxBlock>), grid, block, lds, c.stream, ya);
     }
     {
         FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
@@ -1876,7 +1890,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
     };
     ok = ok && up(dp.px, &twx) && up(dp.py, &twy) && (!dp.three_d || up(dp.pz, &twz));
     int xr = 2, xls = dp.px;
-    if (ok && !pick_x(dp.lx, ce, xr, xls)) {
+    if (ok && !pick_x(dp.lx, ce, xr, xls, 16, dp.lds_limit)) {
         err = "Newell setup: x line does not fit in LDS";
         ok = false;
     }
@@ -1907,7 +1921,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
         la.n_in = dp.py;
         la.n_out = dp.py;
         la.inverse = 0;
-        if ((size_t)la.lpw * (dp.ly.n + kPad) * ce > lds_budget()) {
+        if ((size_t)la.lpw * (dp.ly.n + kPad) * ce > dp.lds_limit) {
             err = "Newell setup: y line does not fit in LDS";
             ok = false;
         }
@@ -1929,7 +1943,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
             za.comp_stride = (int64_t)dp.pz * dp.hx * dp.py;
             za.py = dp.py;
             za.pitch = dp.py;
-            if ((size_t)za.tile * (dp.lz.n + kPad) * ce > lds_budget()) {
+            if ((size_t)za.tile * (dp.lz.n + kPad) * ce > dp.lds_limit) {
                 err = "Newell setup: z line does not fit in LDS";
                 ok = false;
             } else {
@@ -2029,14 +2043,14 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
         dp.kodd = (int)cdiv(ne * dp.kcomp, vec) * vec;
         dp.kblock = dp.kodd + (int)cdiv(no * dp.kcomp, vec) * vec;
     }
-    if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, dp.tune.x_rows > 0 ? dp.tune.x_rows : 16)) {
+    if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, dp.tune.x_rows > 0 ? dp.tune.x_rows : 16, dp.lds_limit)) {
         c.last_error = "x line FFT does not fit in one workgroup";
         return false;
     }
     if (dp.three_d) {
         dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
         if (dp.tune.y_lpw > 0) dp.y_lines_per_wg = std::min(dp.y_lines_per_wg, dp.tune.y_lpw);
-        if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > lds_budget()) {
+        if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit) {
             c.last_error = "y line FFT does not fit in LDS";
             return false;
         }
@@ -2046,15 +2060,15 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
             // with 16 at 48 KB)
             dp.z_tile = pick_ztile(dp.lz, 3, ce, 32768);
             if (dp.tune.z_tile > 0) dp.z_tile = std::min(dp.z_tile, dp.tune.z_tile);
-            if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > lds_budget()) {
+            if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > dp.lds_limit) {
                 c.last_error = "z line FFT does not fit in LDS";
                 return false;
             }
         }
     } else {
         dp.yconv_kpw = pick_lpw(dp.ly, ce, 3);
-        if ((size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > lds_budget() ||
-            3 * dp.yconv_kpw * dp.ly.threads > kMaxBlock) {
+        if ((size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+            3 * dp.yconv_kpw * dp.ly.threads > 1024) {
             c.last_error = "thin-film y convolution does not fit in one workgroup";
             return false;
         }
@@ -2116,6 +2130,13 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     dp.kcomp = dp.three_d ? 6 : 4;
     dp.kz_n = dp.three_d ? dp.nz + 1 : 1;
     dp.inv_npad = 1.0 / ((double)dp.px * dp.py * dp.pz);
+    {
+        // lines longer than 64 KB of LDS use the device's per-workgroup limit
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess &&
+            (size_t)lds > dp.lds_limit)
+            dp.lds_limit = (size_t)lds;
+    }
     parse_tuning(dp.tune);
     // measured at 512x512x8: four lanes per column are faster in FP64 (half
     // the registers per lane), the LDS-staged two-lane tile in FP32

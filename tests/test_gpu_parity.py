@@ -219,6 +219,26 @@ def test_demag_field_prime_factor_above_13(fa, oracle_mod, cells, kernel):
     np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-12 * np.abs(hd).max())
 
 
+@pytest.mark.parametrize("cells,precision", [((16, 1024, 1), "double"), ((8, 2048, 1), "mixed"),
+                                             ((2100, 2, 1), "double"), ((4, 2048, 2), "double")])
+@pytest.mark.parametrize("kernel", ["spectra", "device"])
+def test_demag_field_lines_above_64k_lds(fa, oracle_mod, cells, precision, kernel):
+    """Lines whose LDS footprint passes 64 KB (the thin-film y convolution's
+    three 2048- and 4096-point lines, a 4200-point FP64 x line, a 4096-point
+    FP64 y line and the FP64 Newell transforms) use the device's larger
+    per-workgroup LDS, against the oracle.  The FP64 bound is 1e-11 of the
+    largest field: rounding of 2048- to 4200-point transforms over up to 32K
+    cells reaches 1.5e-12, where a wrong line layout is off by O(1)."""
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(31 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=1e6, a=1e-11, alpha=0.1,
+                         m0=m0, exchange=False, kernel=kernel, precision=precision)
+    hd = cpu.observe()["h_demag"]
+    tol = 1e-11 if precision == "double" else 2e-6
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=tol * np.abs(hd).max())
+
+
 @pytest.mark.parametrize("tune", ["", "z_tile=2"])
 @pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11), (16, 12, 64)])
 def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
