@@ -1921,7 +1921,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
         la.n_in = dp.py;
         la.n_out = dp.py;
         la.inverse = 0;
-        if ((size_t)la.lpw * (dp.ly.n + kPad) * ce > dp.lds_limit) {
+        if ((size_t)la.lpw * (dp.ly.n + kPad) * ce > dp.lds_limit || la.lpw * dp.ly.threads > kMaxBlock) {
             err = "Newell setup: y line does not fit in LDS";
             ok = false;
         }
@@ -1943,7 +1943,7 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
             za.comp_stride = (int64_t)dp.pz * dp.hx * dp.py;
             za.py = dp.py;
             za.pitch = dp.py;
-            if ((size_t)za.tile * (dp.lz.n + kPad) * ce > dp.lds_limit) {
+            if ((size_t)za.tile * (dp.lz.n + kPad) * ce > dp.lds_limit || za.tile * dp.lz.threads > kMaxBlock) {
                 err = "Newell setup: z line does not fit in LDS";
                 ok = false;
             } else {
@@ -2050,8 +2050,9 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
     if (dp.three_d) {
         dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
         if (dp.tune.y_lpw > 0) dp.y_lines_per_wg = std::min(dp.y_lines_per_wg, dp.tune.y_lpw);
-        if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit) {
-            c.last_error = "y line FFT does not fit in LDS";
+        if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+            dp.y_lines_per_wg * dp.ly.threads > kMaxBlock) {
+            c.last_error = "y line FFT does not fit in LDS or 512 lanes";
             return false;
         }
         if (dp.pz > z_reg_max<Real>() || dp.tune.z_lds) {
@@ -2060,8 +2061,9 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
             // with 16 at 48 KB)
             dp.z_tile = pick_ztile(dp.lz, 3, ce, 32768);
             if (dp.tune.z_tile > 0) dp.z_tile = std::min(dp.z_tile, dp.tune.z_tile);
-            if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > dp.lds_limit) {
-                c.last_error = "z line FFT does not fit in LDS";
+            if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > dp.lds_limit ||
+                3 * dp.z_tile * dp.lz.threads > kMaxBlock) {
+                c.last_error = "z line FFT does not fit in LDS or 512 lanes";
                 return false;
             }
         }

@@ -477,6 +477,12 @@ def test_error_paths_report_last_error(fa):
                                        enable_demag=False))
     with pytest.raises(fa.RunError, match="dt must be positive"):
         b.step(-1.0)
+    # a 3-D y line of 16384 points would need 1024 lanes: rejected at set-up
+    m1 = np.tile([1.0, 0.0, 0.0], (2 * 8192 * 2, 1))
+    with pytest.raises(fa.RunError, match="y line FFT does not fit"):
+        fa.NativeFdmBackend(fa.FdmPlan(cells=(2, 8192, 2), cell_size=(1e-9,) * 3, initial_magnetization=m1,
+                                       saturation_magnetisation=8e5, exchange_stiffness=1e-11, damping=0.1,
+                                       demag_kernel="device"))
 
 
 def test_region_table_decoupled_equals_masked_neighbour(fa, oracle_mod):
