@@ -28,7 +28,8 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
 // of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe, y_pipe_f, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab).
+// fixed, z_quad, xcd, z_mirror, y_pipe, y_pipe_f, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab,
+// y_split).
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
@@ -48,6 +49,7 @@ struct DemagTuning {
     int y_pad = 0;     // complex slots of padding after each kx row of the 3-D spectrum
     int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)
     int yz_slab = 0;   // 1: fused y transform + z convolution per kx slab where eligible
+    int y_split = 0;   // 1: thin-film y convolution as separate line and tensor passes
 };
 
 // Applies "key=value,..." (unknown keys ignored) on top of t.
@@ -62,6 +64,7 @@ struct DemagPlan {
     int px = 0, py = 0, pz = 1;
     int hx = 0;    // nx + 1 columns of the half spectrum
     size_t lds_limit = 65536;  // LDS bytes one workgroup may allocate on this device
+    bool y_split = false;      // thin film: y lines, tensor and inverse y lines as three passes
     int rows = 0;  // rows per plane of `spec`
     int kcomp = 0; // 6 (3-D) or 4 (2-D: xz = yz = 0 on the film plane)
     int kz_n = 1;  // octant planes in kz: nz + 1 (3-D) or 1

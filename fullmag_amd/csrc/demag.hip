@@ -623,6 +623,22 @@ __global__ void __launch_bounds__(MAXB) yconv_kernel(YConvArgs<Real> a) {
     }
 }
 
+// Thin film, split form: the three components' ky spectra (all py entries,
+// rows `rows` apart) are multiplied by the tensor at kz = 0 in place; the y
+// transforms run before and after as line passes.  For films whose three
+// lines do not fit one workgroup.
+template <typename Real>
+__global__ void __launch_bounds__(256) film_tensor_kernel(cplx<Real> *data, KernelView<Real> K, int hx, int py,
+                                                          int64_t rows) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= (int64_t)hx * py) return;
+    const int kx = (int)(i / py), ky = (int)(i - (int64_t)kx * py);
+    Real t[6];
+    tensor_at(K, kx, ky, 0, t);
+    const int64_t cs = (int64_t)hx * rows, o = (int64_t)kx * rows + ky;
+    apply_tensor(t, data[o], data[cs + o], data[2 * cs + o]);
+}
+
 // ---------------------------------------------------------------------------
 // z passes: one lane per (kx, ky) column
 // ---------------------------------------------------------------------------
@@ -1769,6 +1785,33 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
             Timed t(c, "demag_y_inv");
             launch_lines<Real>(c, dp.ly, la, ce);
         }
+    } else if (dp.y_split) {
+        LineArgs<Real> la{};
+        la.data = spec;
+        la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
+        la.lp = dp.ly;
+        la.lines = 3ll * dp.hx;
+        la.line_pitch = dp.rows;
+        la.lpw = dp.y_lines_per_wg;
+        {
+            la.n_in = dp.ny;
+            la.n_out = dp.py;
+            la.inverse = 0;
+            Timed t(c, "demag_y_fwd");
+            launch_lines<Real>(c, dp.ly, la, ce);
+        }
+        {
+            Timed t(c, "demag_y_conv");
+            hipLaunchKernelGGL((film_tensor_kernel<Real>), dim3((unsigned)cdiv((int64_t)dp.hx * dp.py, 256)),
+                               dim3(256), 0, c.stream, spec, K, dp.hx, dp.py, (int64_t)dp.rows);
+        }
+        {
+            la.n_in = dp.py;
+            la.n_out = dp.ny;
+            la.inverse = 1;
+            Timed t(c, "demag_y_inv");
+            launch_lines<Real>(c, dp.ly, la, ce);
+        }
     } else {
         YConvArgs<Real> ya{};
         ya.data = spec;
@@ -1862,6 +1905,7 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
             else if (k == "yz_slab") t.yz_slab = v;
+            else if (k == "y_split") t.y_split = v;
         }
         pos = end + 1;
     }
@@ -2069,10 +2113,17 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
         }
     } else {
         dp.yconv_kpw = pick_lpw(dp.ly, ce, 3);
-        if ((size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > dp.lds_limit ||
-            3 * dp.yconv_kpw * dp.ly.threads > 1024) {
-            c.last_error = "thin-film y convolution does not fit in one workgroup";
-            return false;
+        dp.y_split = dp.tune.y_split != 0 || (size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+                     3 * dp.yconv_kpw * dp.ly.threads > 1024;
+        if (dp.y_split) {
+            // the spectrum keeps all py ky entries of every kx row
+            dp.rows = dp.py;
+            dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
+            if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+                dp.y_lines_per_wg * dp.ly.threads > kMaxBlock) {
+                c.last_error = "thin-film y line FFT does not fit in LDS or 512 lanes";
+                return false;
+            }
         }
     }
     if (!upload_twiddles<Real>(c, dp.px, &dp.tw_x) || !upload_twiddles<Real>(c, dp.py, &dp.tw_y)) return false;

@@ -239,6 +239,28 @@ def test_demag_field_lines_above_64k_lds(fa, oracle_mod, cells, precision, kerne
     np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=tol * np.abs(hd).max())
 
 
+@pytest.mark.parametrize("cells,precision,tune", [((12, 10, 1), "double", "y_split=1"),
+                                                  ((17, 6, 1), "double", "y_split=1"),
+                                                  ((3, 29, 1), "mixed", "y_split=1"),
+                                                  ((64, 48, 1), "mixed", "y_split=1"),
+                                                  ((64, 256, 1), "double", "y_split=1"),
+                                                  ((8, 2048, 1), "double", ""), ((4, 4096, 1), "mixed", "")])
+@pytest.mark.parametrize("kernel", ["spectra", "device"])
+def test_thin_film_split_y_matches_oracle(fa, oracle_mod, monkeypatch, cells, precision, tune, kernel):
+    """Thin films whose three y lines do not fit one workgroup (FP64 ny = 2048,
+    FP32 ny = 4096), and small films forced onto the same path, run the y
+    convolution as forward lines, a tensor pass and inverse lines."""
+    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(41 + n, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=1e6, a=1e-11, alpha=0.1,
+                         m0=m0, exchange=False, kernel=kernel, precision=precision)
+    hd = cpu.observe()["h_demag"]
+    tol = 1e-11 if precision == "double" else 2e-6
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=tol * np.abs(hd).max())
+
+
 @pytest.mark.parametrize("tune", ["", "z_tile=2"])
 @pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11), (16, 12, 64)])
 def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
