@@ -261,6 +261,31 @@ def test_thin_film_split_y_matches_oracle(fa, oracle_mod, monkeypatch, cells, pr
     np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=tol * np.abs(hd).max())
 
 
+@pytest.mark.parametrize("cells", [(12, 10, 1), (17, 6, 1), (64, 32, 1)])
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_single_layer_with_3d_spectra_matches_oracle(fa, oracle_mod, cells, precision):
+    """nz = 1 with spectra of the 3-D padded grid (pz = 2) takes the 3-D path,
+    as native/backends/fdm/src/api.cpp:184-187 routes it; the field equals
+    the thin-film field, since the padded z = 1 plane only meets padding.
+    The oracle's 2-D and 3-D spectra agree to 1e-14 (test_oracle_newell);
+    summed over 8 nx ny padded modes that leaves up to 2.4e-12 of the largest
+    field at 64x32, hence the FP64 bound of 1e-11."""
+    nx, ny, nz = cells
+    n = nx * ny
+    cell = (4e-9, 3e-9, 2e-9)
+    m0 = fa.generate_random_unit_vectors(51 + n, n)
+    spectra = oracle_mod.newell_spectra(nx, ny, 1, *cell, thin_film_2d=False)
+    assert spectra.size == 6 * 2 * (2 * nx) * (2 * ny) * 2
+    gpu = fa.NativeFdmBackend(fa.FdmPlan(cells=cells, cell_size=cell, initial_magnetization=m0,
+                                         saturation_magnetisation=1e6, exchange_stiffness=1e-11, damping=0.1,
+                                         precision=precision, enable_exchange=False, demag_kernel=spectra))
+    _, cpu = make_pair(fa, oracle_mod, cells=cells, cell=cell, ms=1e6, a=1e-11, alpha=0.1, m0=m0,
+                       exchange=False, kernel="device", precision=precision)
+    hd = cpu.observe()["h_demag"]
+    tol = 1e-11 if precision == "double" else 2e-6
+    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=tol * np.abs(hd).max())
+
+
 @pytest.mark.parametrize("tune", ["", "z_tile=2"])
 @pytest.mark.parametrize("cells", [(4, 3, 12), (6, 4, 9), (5, 4, 16), (3, 5, 11), (16, 12, 64)])
 def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
