@@ -97,10 +97,11 @@ def smooth_m0(nx, ny, nz):
     return m / np.linalg.norm(m, axis=1, keepdims=True)
 
 
-def accuracy(fa, base, dt, steps, perturb=None):
+def accuracy(fa, base, dt, steps, perturb=None, chunk=0):
     """max|dm| and relative total-energy error of mixed vs FP64 after
     `steps` steps from the same m0; with `perturb`, also the FP64 path's
-    divergence from m0 perturbed by that amplitude."""
+    divergence from m0 perturbed by that amplitude.  chunk > 0 advances in
+    step_many chunks of that many steps (stats only at chunk ends)."""
     a = fa.NativeFdmBackend(fa.FdmPlan(precision="double", **base))
     b = fa.NativeFdmBackend(fa.FdmPlan(precision="mixed", **base))
     p = None
@@ -110,11 +111,21 @@ def accuracy(fa, base, dt, steps, perturb=None):
         mp = m0 + perturb * rng.standard_normal(m0.shape)
         mp /= np.linalg.norm(mp, axis=1, keepdims=True)
         p = fa.NativeFdmBackend(fa.FdmPlan(precision="double", **dict(base, initial_magnetization=mp)))
-    for _ in range(steps):
-        sa = a.step(dt)
-        sb = b.step(dt)
-        if p is not None:
-            p.step(dt)
+    if chunk > 0:
+        done = 0
+        while done < steps:
+            k = min(chunk, steps - done)
+            sa = a.step_many(dt, k)
+            sb = b.step_many(dt, k)
+            if p is not None:
+                p.step_many(dt, k)
+            done += k
+    else:
+        for _ in range(steps):
+            sa = a.step(dt)
+            sb = b.step(dt)
+            if p is not None:
+                p.step(dt)
     ma = a.copy_m()
     out = {"steps": steps, "max_abs_dm": float(np.abs(ma - b.copy_m()).max()),
            "rel_energy_err": abs(sb.e_total - sa.e_total) / abs(sa.e_total)}
@@ -163,24 +174,59 @@ def kernel_profile(backend, dt, steps=10):
     return t
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(base, dt):
     """The oracle (FP64 C++ restatement of fullmag-engine: OpenMP per-cell
-    loops, serial FFT) timed for one Heun step of the same workload."""
+    loops, serial FFT as crates/fullmag-engine/src/lib.rs:2768-2812) on the
+    host: one Heun step of the 512x512x8 workload at nproc - 1 threads (the
+    runner's default, crates/fullmag-runner/src/lib.rs:766-770) and at nproc
+    threads, and the 64x64x1 parity workload (1000 steps) in full.  nproc is
+    the CPU share the process may use (sched_getaffinity, capped by
+    OMP_NUM_THREADS when that is set)."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    O.set_num_threads(threads)
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    if os.environ.get("OMP_NUM_THREADS"):
+        nproc = min(nproc, int(os.environ["OMP_NUM_THREADS"]))
     nx, ny, nz = base["cells"]
     dx, dy, dz = base["cell_size"]
     cfg = O.OracleConfig(nx=nx, ny=ny, nz=nz, dx=dx, dy=dy, dz=dz, ms=base["saturation_magnetisation"],
                          a=base["exchange_stiffness"], alpha=base["damping"], h_ext=base["external_field"])
     p = O.OracleProblem(cfg, base["initial_magnetization"])
+    per = {}
+    for threads in sorted({max(1, nproc - 1), nproc}):
+        O.set_num_threads(threads)
+        t0 = time.perf_counter()
+        p.step(dt)
+        per[threads] = time.perf_counter() - t0
+    best_t = min(per, key=per.get)
+    # the 64x64x1 parity workload in full: 1000 steps of 1e-13 s
+    import fullmag_amd as fa
+    O.set_num_threads(nproc)
+    c1 = O.OracleConfig(nx=64, ny=64, nz=1, dx=5e-9, dy=5e-9, dz=3e-9, ms=8e5, a=1.3e-11, alpha=0.5,
+                        h_ext=(10e-3 / MU0, 0.0, 0.0))
+    q = O.OracleProblem(c1, fa.generate_random_unit_vectors(42, 64 * 64))
     t0 = time.perf_counter()
-    p.step(dt)
-    el = time.perf_counter() - t0
-    return {"value": nx * ny * nz / el, "unit": "cell-steps/s", "cores": O.num_threads(), "kind": "port",
-            "sample": f"1 Heun step of the {nx}x{ny}x{nz} workload ({nx * ny * nz} cells, {el:.1f} s), "
-                      "FP64 oracle restating crates/fullmag-engine (setup and Newell build excluded)"}
+    for _ in range(1000):
+        q.step(1e-13)
+    el1 = time.perf_counter() - t0
+    n = nx * ny * nz
+    return {"value": n / per[best_t], "unit": "cell-steps/s", "cores": best_t, "kind": "port",
+            "sample": f"1 Heun step of the {nx}x{ny}x{nz} workload ({n} cells) per thread count, FP64 oracle "
+                      "restating crates/fullmag-engine (setup and Newell build excluded)",
+            "cpu_model": cpu_model(), "nproc": nproc,
+            "by_threads": {str(t): {"cell_steps_per_s": n / v, "seconds": v} for t, v in sorted(per.items())},
+            "config1_64x64x1_1000_steps": {"seconds": el1, "cell_steps_per_s": 64 * 64 * 1000 / el1,
+                                           "threads": nproc}}
 
 
 # ---------------------------------------------------------------------------
@@ -279,29 +325,38 @@ def dp45_block(fa, torch, args):
     base = dict(cells=(nx, ny, nz), cell_size=(2e-9, 2e-9, 2e-9), initial_magnetization=vortex_m0(nx, ny, nz),
                 saturation_magnetisation=1.4e6, exchange_stiffness=2.1e-11, damping=0.01, integrator="dp45",
                 demag_kernel="device", adaptive_max_error=1e-6)
-    res = {}
-    for prec, steps in (("mixed", args.steps), ("double", max(2, args.steps // 5))):
+    res, final = {}, {}
+    for prec in ("mixed", "double"):
         b = fa.NativeFdmBackend(fa.FdmPlan(precision=prec, **base))
         dt = 1e-13
+        s = None
         for _ in range(args.warmup):
             s = b.step(dt)
             dt = s.dt_suggested or dt
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        sim0 = s.time if args.warmup else 0.0
-        for _ in range(steps):
+        sim0 = s.time if s is not None else 0.0
+        dts = []
+        for _ in range(args.steps):
             s = b.step(dt)
+            dts.append(s.dt)
             dt = s.dt_suggested or dt
         el = time.perf_counter() - t0
-        res[prec] = {"cell_steps_per_s": n * steps / el, "ms_per_step": 1e3 * el / steps,
-                     "mean_dt": (s.time - sim0) / steps, "e_total": s.e_total}
+        res[prec] = {"cell_steps_per_s": n * args.steps / el, "ms_per_step": 1e3 * el / args.steps,
+                     "mean_dt": (s.time - sim0) / args.steps, "sim_time_s": s.time, "e_total": s.e_total}
+        final[prec] = (b.copy_m(), s, np.array(dts))
         b.close()
     m = res["mixed"]
+    (m_mx, s_mx, dt_mx), (m_64, s_64, dt_64) = final["mixed"], final["double"]
+    acc = {"accepted_steps": args.warmup + args.steps, "max_abs_dm": float(np.abs(m_mx - m_64).max()),
+           "rel_energy_err": abs(s_mx.e_total - s_64.e_total) / abs(s_64.e_total),
+           "max_rel_dt_diff": float(np.abs(dt_mx - dt_64).max() / dt_64.max()),
+           "rel_sim_time_diff": abs(s_mx.time - s_64.time) / s_64.time}
     line("DP45 accepted cell-steps/s, 256x256x64 block (mixed precision)", m["cell_steps_per_s"], "cell-steps/s",
          args.steps, args.warmup, m["ms_per_step"], "f32 fields / f64 state",
          {"workload": "3-D block 256x256x64 cells of 2 nm, Ms 1.4e6, A 2.1e-11, alpha 0.01, no Zeeman, vortex m0 "
                       "(-y, x, 0.1) + 1% noise, DP45 max error 1e-6, padded 512x512x128",
-          "cells": n, "mixed": m, "double": res["double"],
+          "cells": n, "mixed": m, "double": res["double"], "mixed_vs_fp64_same_accepted_steps": acc,
           "speedup_vs_fp64": m["cell_steps_per_s"] / res["double"]["cell_steps_per_s"]})
 
 
@@ -392,6 +447,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=100)
     ap.add_argument("--acc-steps", type=int, default=50)
     ap.add_argument("--smooth-steps", type=int, default=1000)
+    ap.add_argument("--horizon-steps", type=int, default=5000,
+                    help="the workload's configured step count: mixed vs FP64 after this many steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -421,12 +478,16 @@ def main():
     nx, ny, nz = base["cells"]
     n = nx * ny * nz
 
-    mixed = fa.NativeFdmBackend(fa.FdmPlan(precision="mixed", **base))
-    t_mixed = time_steps(mixed, args.steps, args.warmup, dt)
-    value = n * args.steps / t_mixed
-    ms_per_step = 1e3 * t_mixed / args.steps
-    prof = {} if args.no_profile else kernel_profile(mixed, dt)
-    mixed.close()
+    # accuracy of the mixed path against the FP64 GPU path from the same m0
+    # (these runs also bring the GPU to its working clocks before timing)
+    acc = accuracy(fa, base, dt, args.acc_steps, perturb=1e-12) if args.acc_steps > 0 else None
+    acc_smooth = None
+    if args.smooth_steps > 0:
+        acc_smooth = accuracy(fa, dict(base, initial_magnetization=smooth_m0(nx, ny, nz)), dt, args.smooth_steps,
+                              chunk=500)
+    acc_horizon = None
+    if args.horizon_steps > 0:
+        acc_horizon = accuracy(fa, base, dt, args.horizon_steps, perturb=1e-12, chunk=500)
 
     fp64 = fa.NativeFdmBackend(fa.FdmPlan(precision="double", **base))
     t64 = time_steps(fp64, args.fp64_steps, 5, dt)
@@ -434,11 +495,12 @@ def main():
     prof64 = {} if args.no_profile else kernel_profile(fp64, dt)
     fp64.close()
 
-    # accuracy of the mixed path against the FP64 GPU path from the same m0
-    acc = accuracy(fa, base, dt, args.acc_steps, perturb=1e-12) if args.acc_steps > 0 else None
-    acc_smooth = None
-    if args.smooth_steps > 0:
-        acc_smooth = accuracy(fa, dict(base, initial_magnetization=smooth_m0(nx, ny, nz)), dt, args.smooth_steps)
+    mixed = fa.NativeFdmBackend(fa.FdmPlan(precision="mixed", **base))
+    t_mixed = time_steps(mixed, args.steps, args.warmup, dt)
+    value = n * args.steps / t_mixed
+    ms_per_step = 1e3 * t_mixed / args.steps
+    prof = {} if args.no_profile else kernel_profile(mixed, dt)
+    mixed.close()
 
     roofline = None
     if prof:
@@ -472,6 +534,7 @@ def main():
                    "fp64_cell_steps_per_s": fp64_value, "speedup_vs_fp64": value / fp64_value,
                    "fp64_ms_per_step": 1e3 * t64 / args.fp64_steps,
                    "accuracy_vs_fp64": acc, "accuracy_vs_fp64_smooth_m0": acc_smooth,
+                   "accuracy_vs_fp64_at_horizon": acc_horizon,
                    "step_alg_bytes": sb_bytes, "step_achieved_GBs": sb_bytes / (ms_per_step / 1e3) / 1e9},
         "roofline": roofline,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(base, dt),

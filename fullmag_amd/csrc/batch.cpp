@@ -7,12 +7,17 @@
 // and join back into one origin stream during capture, so the graph holds
 // one independent branch per member; a single hipGraphLaunch then advances
 // every member by k steps, and the runtime is free to run the branches
-// concurrently.  Kernel arguments (buffer pointers, dt) are baked in at
-// capture; the Heun path swaps no buffers, so replays are exact.  The graph
-// holds no memset nodes.
+// concurrently.  Kernel arguments (buffer pointers, dt, the external field)
+// are baked in at capture; the Heun path swaps no buffers, so replays are
+// exact.  The graph holds no memset nodes.  Before each run the batch
+// re-establishes what the captured steps assume of every member: H_demag
+// of the current m in the member's demag buffer (recomputed when an upload
+// or state copy replaced m), and the captured external field (the graph is
+// recaptured when set_external_field changed it).
 #include "context.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,6 +36,7 @@ struct fullmag_fdm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> events;
+    std::vector<std::array<double, 4>> ext;  // has_ext, H_ext at capture
     std::string last_error;
 };
 
@@ -41,9 +47,19 @@ bool fail(fullmag_fdm_batch *b, const std::string &what) {
     return false;
 }
 
+std::array<double, 4> ext_of(const fm::Context &c) {
+    return {c.has_ext ? 1.0 : 0.0, c.hext[0], c.hext[1], c.hext[2]};
+}
+
 bool capture(fullmag_fdm_batch *b) {
     using fm::hip_ok;
     std::string &err = b->last_error;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->ext.clear();
+    for (fm::Context *c : b->members) b->ext.push_back(ext_of(*c));
     if (!hip_ok(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture", err))
         return false;
     hipEvent_t fork = b->events[0];
@@ -128,6 +144,18 @@ int fullmag_fdm_ext_batch_run(fullmag_fdm_batch *b, uint64_t n_steps, fullmag_fd
         b->last_error = "batch_run needs n_steps > 0";
         return FULLMAG_FDM_ERR_INVALID;
     }
+    // members changed since the capture: a new external field needs a new
+    // graph; a new m needs H_demag(m) before the first captured stage
+    bool recapture = false;
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        fm::Context &c = *b->members[i];
+        if (ext_of(c) != b->ext[i]) recapture = true;
+        if (!fm::integrator_prepare(c)) {
+            b->last_error = "batch member " + std::to_string(i) + ": " + c.last_error;
+            return FULLMAG_FDM_ERR_CUDA;
+        }
+    }
+    if (recapture && !capture(b)) return FULLMAG_FDM_ERR_CUDA;
     const uint64_t full = n_steps / (uint64_t)b->k, rem = n_steps % (uint64_t)b->k;
     for (uint64_t s = 0; s < full; ++s)
         if (!fm::hip_ok(hipGraphLaunch(b->exec, b->origin), "hipGraphLaunch", b->last_error))
@@ -162,43 +190,68 @@ void fullmag_fdm_ext_batch_destroy(fullmag_fdm_batch *b) {
     delete b;
 }
 
-// Coordinate search over the launch-time knobs of the demag transforms and
+// Coordinate search over the schedule knobs of the demag transforms and the
 // cell kernels, timing (demag + stats) at the current state; keeps the
-// fastest schedule and writes it as "key=value,..." into `chosen`.
+// fastest schedule and writes it as "key=value,..." into `chosen`.  The
+// plan-shaping knobs (radices, rows per x workgroup, y lines per workgroup,
+// long-z tile) are tried by re-planning the transforms in place; candidates
+// whose plan does not fit are skipped.
 int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen, uint64_t chosen_len) {
     fm::Context *cp = handle ? fm::backend_context(handle) : nullptr;
     if (!cp || !cp->ready) return FULLMAG_FDM_ERR_INVALID;
     fm::Context &c = *cp;
     reps = std::max(1, reps);
-    fm::DemagTuning &t = c.dp.tune;
     struct Knob {
         const char *name;
-        int *field;
+        int fm::DemagTuning::*field;
         std::vector<int> values;
+        bool plan;  // needs a re-plan
     };
-    std::vector<Knob> knobs = {{"fixed", &t.fixed, {1, 0}},         {"y_pipe", &t.y_pipe, {4, 0, 1, 2, 8}},
-                               {"y_pipe_f", &t.y_pipe_f, {2, 1, 4, 8}},
-                               {"z_quad", &t.z_quad, {0, 1}},         {"z_mirror", &t.z_mirror, {0, 1}},
-                               {"xcd", &t.xcd, {1, 0}},               {"zc", &t.zc, {0, 1, 2, 4}}};
+    using T = fm::DemagTuning;
+    std::vector<Knob> knobs = {{"x_radix", &T::x_radix, {16, 8, 4}, true},
+                               {"x_rows", &T::x_rows, {0, 8, 4, 2}, true},
+                               {"y_radix", &T::y_radix, {16, 8, 4}, true},
+                               {"y_lpw", &T::y_lpw, {0, 1, 2}, true},
+                               {"fixed", &T::fixed, {1, 0}, false},
+                               {"y_pipe", &T::y_pipe, {4, 0, 1, 2, 8}, false},
+                               {"y_pipe_f", &T::y_pipe_f, {2, 1, 4, 8}, false},
+                               {"z_quad", &T::z_quad, {0, 1}, false},
+                               {"xcd", &T::xcd, {1, 0}, false},
+                               {"zc", &T::zc, {0, 1, 2, 4}, false}};
+    if (c.demag && c.dp.three_d && c.dp.pz > 16) knobs.push_back({"z_tile", &T::z_tile, {0, 16, 8, 4, 2}, true});
+    std::string err;
     double best = fm::integrator_probe_ms(c, reps);
     for (auto &k : knobs) {
-        const int keep = *k.field;
+        fm::DemagTuning cur = c.dp.tune;
+        const int keep = cur.*k.field;
         int best_v = keep;
         for (int v : k.values) {
             if (v == keep) continue;
-            *k.field = v;
+            fm::DemagTuning t = cur;
+            t.*k.field = v;
+            if (k.plan) {
+                if (!fm::demag_replan(c, t, err)) continue;
+            } else {
+                c.dp.tune = t;
+            }
             const double ms = fm::integrator_probe_ms(c, reps);
             if (ms < best * 0.98) {
                 best = ms;
                 best_v = v;
             }
         }
-        *k.field = best_v;
+        fm::DemagTuning t = cur;
+        t.*k.field = best_v;
+        if (!fm::demag_replan(c, t, err)) {
+            c.last_error = "autotune: " + err;
+            return FULLMAG_FDM_ERR_INTERNAL;
+        }
     }
     if (!fm::hip_ok(hipGetLastError(), "autotune", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
     if (chosen && chosen_len) {
         std::string s;
-        for (auto &k : knobs) s += std::string(s.empty() ? "" : ",") + k.name + "=" + std::to_string(*k.field);
+        for (auto &k : knobs)
+            s += std::string(s.empty() ? "" : ",") + k.name + "=" + std::to_string(c.dp.tune.*k.field);
         std::strncpy(chosen, s.c_str(), chosen_len - 1);
         chosen[chosen_len - 1] = 0;
     }
@@ -206,23 +259,19 @@ int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen
 }
 
 // Applies a schedule string as written by fullmag_fdm_ext_autotune (or the
-// FULLMAG_FDM_TUNE syntax) to the launch-time knobs of one backend.
+// FULLMAG_FDM_TUNE syntax) to one backend, re-planning its transforms.
 int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedule) {
     fm::Context *cp = handle ? fm::backend_context(handle) : nullptr;
     if (!cp || !cp->ready || !schedule) return FULLMAG_FDM_ERR_INVALID;
     fm::DemagTuning t = cp->dp.tune;
     fm::parse_tuning_string(t, schedule);
-    // only launch-time knobs; the planned geometry stays
-    cp->dp.tune.fixed = t.fixed;
-    cp->dp.tune.y_pipe = t.y_pipe;
-    cp->dp.tune.y_pipe_f = t.y_pipe_f;
-    cp->dp.tune.z_quad = t.z_quad;
-    cp->dp.tune.z_mirror = t.z_mirror;
-    cp->dp.tune.xcd = t.xcd;
-    cp->dp.tune.zc = t.zc;
-    cp->dp.tune.z_pipe = t.z_pipe;
-    cp->dp.tune.z_wide = t.z_wide;
-    cp->dp.tune.z_kstage = t.z_kstage;
+    t.y_split = cp->dp.tune.y_split;  // the spectrum layout stays
+    std::string err;
+    if (!fm::demag_replan(*cp, t, err)) {
+        cp->last_error = "set_schedule: " + err;
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    cp->pre_valid = false;  // the stored predictor came from the old schedule's rounding
     return FULLMAG_FDM_OK;
 }
 

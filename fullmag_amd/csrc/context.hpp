@@ -27,28 +27,20 @@ enum Mode { MODE_SINGLE = 1, MODE_DOUBLE = 2, MODE_MIXED = 3 };
 
 // Launch-schedule knobs of the demag transforms (0 = automatic).  Settable
 // per backend from the FULLMAG_FDM_TUNE environment variable as a comma list
-// of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, z_lds,
-// fixed, z_quad, xcd, z_mirror, y_pipe, y_pipe_f, z_stage, z_pipe, z_wide, z_kstage, z_tile, x_pipe, y_pad, zc, yz_slab,
-// y_split).
+// of key=value (x_radix, x_lanes, x_rows, y_radix, y_lanes, y_lpw, fixed,
+// z_quad, xcd, y_pipe, y_pipe_f, z_tile, zc, y_split).  x_radix, x_lanes,
+// x_rows, y_radix, y_lanes, y_lpw, z_tile and y_split shape the plan made at
+// set-up; the others are read at every launch.
 struct DemagTuning {
     int x_radix = 16, x_lanes = 0, x_rows = 0;
     int y_radix = 16, y_lanes = 0, y_lpw = 0;
-    int z_lds = 0;   // 1: LDS z kernel even when the register path applies
     int fixed = 1;   // 0: runtime-planned line kernels even for fixed schedules
     int z_quad = -1;  // 1: four lanes per z column instead of two (even nz); -1: FP64 only
     int xcd = 1;     // 0: no per-XCD renumbering of workgroups
-    int z_mirror = 0;  // 1: two-lane z kernel pairs mirrored ky columns in one workgroup
     int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
     int y_pipe_f = 2;  // > 0: lines per workgroup of the pipelined forward y pass (half-zero input)
-    int z_stage = 1;   // 1: register z convolution staged through LDS tiles
-    int z_pipe = 0;    // > 0: persistent staged z convolution, tiles per workgroup
-    int z_wide = 0;    // 1: staged z convolution tiles of 256 columns (else 128)
-    int z_kstage = 1;  // 1: staged z tile reads the tensor through per-wave LDS (FP32)
     int z_tile = 0;    // > 0: at most this many columns per long-z (LDS line) workgroup
-    int x_pipe = 0;    // row blocks per workgroup of the pipelined x kernels (0: off)
-    int y_pad = 0;     // complex slots of padding after each kx row of the 3-D spectrum
     int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)
-    int yz_slab = 0;   // 1: fused y transform + z convolution per kx slab where eligible
     int y_split = 0;   // 1: thin-film y convolution as separate line and tensor passes
 };
 
@@ -76,7 +68,7 @@ struct DemagPlan {
     int yconv_kpw = 1;           // thin-film y convolution: kx columns per workgroup
     int z_tile = 1;              // long z lines: columns per workgroup
     // device buffers (element type depends on Real)
-    void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr, *tw_p = nullptr;  // tw_p: length py / 4
+    void *tw_x = nullptr, *tw_y = nullptr, *tw_z = nullptr;
     void *spec = nullptr;
     void *kern = nullptr;
     uint64_t spec_elems = 0;  // complex elements
@@ -195,6 +187,10 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
 // `out` (Real): -N * (Ms src), or the bare convolution -N * src when
 // weight_ms is false.  Inactive cells get zero.
 void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true);
+// Re-plans the demag lines and launch geometry under the schedule `t`
+// (radices, lanes, rows per workgroup, tiles) without reallocating; on
+// failure the backend keeps its plan and `err` says why.
+bool demag_replan(Context &c, const DemagTuning &t, std::string &err);
 bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, double dy, double dz, bool thin_film_2d,
                          double *out, std::string &err);
 

@@ -152,8 +152,6 @@ struct XArgs {
     int n_out;        // inverse: outputs stored per row
     Real scale;       // inverse: 1/Npad
     SpecGeom g;
-    int iters;        // pipelined kernels: row blocks per workgroup
-    int yblocks;      // row blocks per (c, z) plane: ceil(rows / xr)
 };
 
 // Ms m at (c, z, y, x) of the state; zero on inactive cells.
@@ -361,125 +359,6 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
     }
 }
 
-// Software-pipelined x passes for fixed power-of-two lengths N: a workgroup
-// walks `iters` consecutive row blocks of one (c, z) plane; the global loads
-// of the next block are issued into registers before the current block's
-// LDS transform, so memory traffic overlaps the FFT.  Per block the work is
-// exactly that of xfwd_kernel / xinv_kernel.
-template <typename Real, class Src, int N>
-__global__ void __launch_bounds__(kMaxBlock) xfwd_pipe_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
-    using L = LineOps<N>;
-    constexpr int T = N / 16, NS = N / T, NI = (N / 2 + T - 1) / T;  // slots / loaded slots per lane
-    extern __shared__ unsigned char smem_raw[];
-    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int ls = a.ls, tid = threadIdx.x, nthr = blockDim.x;
-    const int jl = tid / T, t = tid - jl * T, lx = L::line_xor(jl);
-    cplx<Real> *line = lds + jl * ls;
-    const int z = blockIdx.y, c = blockIdx.z, b0 = blockIdx.x * a.iters;
-    cplx<Real> buf[NI];
-    auto fetch = [&](int yb) {
-        const int ya = yb * a.xr + 2 * jl;
-        const bool live = yb < a.yblocks;
-#pragma unroll
-        for (int e = 0; e < NI; ++e) {
-            const int x = t + e * T;
-            cplx<Real> v = {Real(0), Real(0)};
-            if (live && x < a.n_in) {
-                if (ya < a.rows) v.x = src(c, z, ya, x);
-                if (ya + 1 < a.rows) v.y = src(c, z, ya + 1, x);
-            }
-            buf[e] = v;
-        }
-    };
-    fetch(b0);
-    const int hx = a.g.hx;
-    for (int it = 0; it < a.iters; ++it) {
-        const int yb = b0 + it;
-#pragma unroll
-        for (int e = 0; e < NS; ++e)
-            line[L::slot(a.lp, t + e * T, lx)] = e < NI ? buf[e] : cplx<Real>{Real(0), Real(0)};
-        __syncthreads();
-        if (it + 1 < a.iters) fetch(yb + 1);
-        L::template fft<Real>(line, a.lp, t, a.tw, false, lx);
-        __syncthreads();  // the unpack reads every line
-        if (yb < a.yblocks) {
-            const int y0 = yb * a.xr;
-            for (int idx = tid; idx < hx * a.xr; idx += nthr) {
-                const int yl = idx & (a.xr - 1), k = idx >> a.xr_log2;
-                const int y = y0 + yl;
-                if (y >= a.rows) continue;
-                const int j = yl >> 1, jx = L::line_xor(j);
-                const cplx<Real> zk = lds[j * ls + L::slot(a.lp, k, jx)];
-                const cplx<Real> zn = lds[j * ls + L::slot(a.lp, k == 0 ? 0 : N - k, jx)];
-                const Real h = Real(0.5);
-                cplx<Real> v;
-                if ((yl & 1) == 0) v = {h * (zk.x + zn.x), h * (zk.y - zn.y)};
-                else v = {h * (zk.y + zn.y), h * (zn.x - zk.x)};
-                spec[a.g.at(c, z, k, y)] = v;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-template <typename Real, class Sink, int N>
-__global__ void __launch_bounds__(kMaxBlock) xinv_pipe_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
-    using L = LineOps<N>;
-    constexpr int T = N / 16;
-    constexpr int MI = (N / 2 + 1 + T - 1) / T;  // (kx, line) pairs per lane: ceil(hx / T)
-    extern __shared__ unsigned char smem_raw[];
-    cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
-    const int LINES = a.xr >> 1;
-    const int ls = a.ls, tid = threadIdx.x, nthr = blockDim.x;
-    const int z = blockIdx.y, c = blockIdx.z, b0 = blockIdx.x * a.iters;
-    const int hx = a.g.hx;
-    cplx<Real> bA[MI], bB[MI];
-    auto fetch = [&](int yb) {
-        const bool live = yb < a.yblocks;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-            const int idx = tid + i * nthr;
-            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
-            const int ya = yb * a.xr + 2 * j;
-            cplx<Real> A = {Real(0), Real(0)}, B = {Real(0), Real(0)};
-            if (live && idx < hx * LINES) {
-                if (ya < a.rows) A = spec[a.g.at(c, z, k, ya)];
-                if (ya + 1 < a.rows) B = spec[a.g.at(c, z, k, ya + 1)];
-            }
-            bA[i] = A;
-            bB[i] = B;
-        }
-    };
-    fetch(b0);
-    const int jl = tid / T, t = tid - jl * T, lx = L::line_xor(jl);
-    const cplx<Real> *line = lds + jl * ls;
-    for (int it = 0; it < a.iters; ++it) {
-        const int yb = b0 + it;
-#pragma unroll
-        for (int i = 0; i < MI; ++i) {
-            const int idx = tid + i * nthr;
-            if (idx < hx * LINES) {
-                const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1), jx = L::line_xor(j);
-                const cplx<Real> A = bA[i], B = bB[i];
-                lds[j * ls + L::slot(a.lp, k, jx)] = {A.x - B.y, A.y + B.x};
-                if (k > 0 && k < N - k) lds[j * ls + L::slot(a.lp, N - k, jx)] = {A.x + B.y, B.x - A.y};
-            }
-        }
-        __syncthreads();
-        if (it + 1 < a.iters) fetch(yb + 1);
-        L::template fft<Real>(lds + jl * ls, a.lp, t, a.tw, true, lx);
-        if (yb < a.yblocks) {
-            const int ya = yb * a.xr + 2 * jl;
-            for (int x = t; x < a.n_out; x += T) {
-                const cplx<Real> v = line[L::slot(a.lp, x, lx)];
-                if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
-                if (ya + 1 < a.rows) sink(c, z, ya + 1, x, v.y * a.scale);
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Contiguous line passes (y; and the Newell setup's z when long)
 // ---------------------------------------------------------------------------
@@ -652,8 +531,6 @@ struct ZArgs {
     int64_t comp_stride;   // nz * hx * pitch
     int pitch;             // distance between kx rows of the spectrum (>= py)
     int xcd;               // renumber workgroups per XCD
-    int half, groups;      // py / 2; groups of 16 direct ky (0..half) per kx
-    int kstage;            // staged tile: tensor halves through per-wave LDS
     KernelView<Real> K;
 };
 
@@ -712,38 +589,18 @@ __device__ __forceinline__ void half_dft(cplx<Real> *v, const cplx<Real> *tw_p, 
 // NZ non-zero inputs exist); tensor at its own kz; inverse:
 // y_z = IDFT_NZ(Y_even)[z] + w_pz^(-z) IDFT_NZ(Y_odd)[z], z < NZ, summed
 // across the lane pair with a shuffle.  All three components in registers.
-// Lane layout (groups > 0): workgroup g of kx row blk / groups holds the 64
-// direct columns ky = 64 g + 32 (wave & 1) + lane / 2 (ky <= py/2) in waves
-// 0-1 and their mirrors py - ky in waves 2-3; a column and its mirror read
-// the same tensor octant entries.  groups = 0: consecutive lane pairs take
-// consecutive columns.
+// Consecutive lane pairs take consecutive columns.  Used where the staged
+// tile does not apply (the k-space projection fallback).
 template <typename Real, int NZ>
 __global__ void __launch_bounds__(256) zconv_pair_kernel(ZArgs<Real> a) {
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int p = (int)(gid & 1);
-    bool valid;  // both lanes of a pair stay for the shuffle
-    int kx, ky;
-    int64_t col;
-    if (a.groups > 0) {
-        // workgroup = one kx and 64 direct ky (waves 0-1) plus their mirrors
-        // (waves 2-3): full 128-byte tensor lines per instruction, and the
-        // mirrored waves re-read them from the workgroup's L1
-        const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-        const int w = (int)(threadIdx.x >> 6), i = (int)((threadIdx.x & 63) >> 1);
-        const int mir = w >> 1;
-        kx = (int)(blk / (uint32_t)a.groups);
-        const int kd = 64 * (int)(blk - (uint32_t)kx * a.groups) + 32 * (w & 1) + i;
-        valid = kx < a.hx && kd <= a.half && (!mir || (kd >= 1 && kd < a.half));
-        ky = valid ? (mir ? a.py - kd : kd) : 0;
-        col = valid ? (int64_t)kx * a.pitch + ky : 0;
-    } else {
-        // consecutive lane pairs take consecutive columns
-        col = gid >> 1;
-        valid = col < a.columns;
-        kx = valid ? (int)(col / a.pitch) : 0;
-        ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
-        valid = valid && ky < a.py;
-    }
+    // both lanes of a pair stay for the shuffle
+    const int64_t col = gid >> 1;
+    bool valid = col < a.columns;
+    const int kx = valid ? (int)(col / a.pitch) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+    valid = valid && ky < a.py;
     cplx<Real> v[3][NZ];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -996,34 +853,9 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
         in.put(tile, t);
     }
     __syncthreads();
-    ztile_convolve<Real, NZ, TW>(a, tile, col0, t, S::ON && TW == kZTile && a.kstage ? kstage : nullptr);
+    ztile_convolve<Real, NZ, TW>(a, tile, col0, t, S::ON && TW == kZTile ? kstage : nullptr);
     __syncthreads();
     ztile_store<Real, NZ, TW>(a, tile, col0, t);
-}
-
-// Persistent form: each workgroup convolves `iters` consecutive tiles and
-// holds the next tile's input in registers while the current one is
-// transformed, so its loads overlap the arithmetic and the stores.
-template <typename Real, int NZ>
-__global__ void __launch_bounds__(256, 2) zconv_tile_pipe_kernel(ZArgs<Real> a, int iters) {
-    using TL = ZTile<Real, NZ>;
-    __shared__ cplx<Real> tile[TL::ROWS][kZTile + TL::V];
-    const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int t = (int)threadIdx.x;
-    const int64_t first = (int64_t)blk * iters * kZTile;
-    TL buf;
-    buf.fetch(a, first, t);
-    for (int it = 0; it < iters; ++it) {
-        const int64_t col0 = first + (int64_t)it * kZTile;
-        if (col0 >= a.columns) break;  // uniform over the workgroup
-        buf.put(tile, t);
-        __syncthreads();
-        if (it + 1 < iters) buf.fetch(a, col0 + kZTile, t);
-        ztile_convolve<Real, NZ>(a, tile, col0, t);
-        __syncthreads();
-        ztile_store<Real, NZ>(a, tile, col0, t);
-        __syncthreads();
-    }
 }
 
 // z convolution with four lanes per (kx, ky) column (pz = 2 NZ = 4 M, NZ
@@ -1195,214 +1027,6 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// Fused y transform + z convolution + inverse y transform of one kx slab
-// ---------------------------------------------------------------------------
-// With py = 2 ny and the y input zero beyond ny, the ky = S k + s outputs of
-// the length-py transform are a length-P = py / S transform of the folded
-// input u_s[n] = sum_{b < S/2} x[n + b P] w^(s (n + b P)) (w = exp(-2 pi i /
-// py)), and the first ny outputs of the inverse are
-//   y[n] = sum_s w^(-s n) IDFT_P(Y_s)[n mod P].
-// So one workgroup owns a whole kx slab (3 components x nz planes x ny) and
-// walks the S classes: fold from memory into LDS, forward P-point transforms
-// of the 3 nz rows, the register z convolution of each of the P columns (two
-// lanes per column, as zconv_pair_kernel), inverse transforms, and
-// accumulation of w^(-s n) IDFT_P into an LDS accumulator; the slab is
-// written back once.  Per demag evaluation this replaces the separate
-// y-forward, z and y-inverse passes (three round trips of the 2 ny spectrum)
-// with one write and four (mostly L2) reads of the ny spectrum and one read
-// of the tensor.  At one 144 KB workgroup per CU its phases do not overlap
-// each other's memory latency, which is why it is not the default schedule.
-template <typename Real>
-struct SlabArgs {
-    cplx<Real> *data;          // spec[c][z][kx][pitch], first ny used
-    const cplx<Real> *tw_y;    // roots of length py
-    const cplx<Real> *tw_p;    // roots of length P
-    const cplx<Real> *tw_z;    // roots of length pz
-    int hx, ny, py, pitch;
-    int64_t zstride;           // hx * pitch
-    int64_t comp_stride;       // nz * hx * pitch
-    KernelView<Real> K;
-};
-
-// Fixed power-of-two line transform where only lanes with `active` own a
-// line; every lane of the workgroup reaches the barriers.
-template <typename T, int N, int R, int NS>
-__device__ __forceinline__ void guarded_pass(cplx<T> *line, int t, const cplx<T> *tw, bool inv, bool active, int lx) {
-    constexpr int MB = lane_butterflies<R>();
-    constexpr int NT = N / 16, NR = N / R, ST = N / (NS * R);
-    cplx<T> v[MB][R];
-    int base[MB];
-    if (active) {
-#pragma unroll
-        for (int b = 0; b < MB; ++b) {
-            const int j = t + b * NT;
-            const int k = j % NS;
-#pragma unroll
-            for (int r = 0; r < R; ++r) v[b][r] = line[p2slot(j + r * NR) ^ lx];
-            if (NS > 1 && k != 0) twiddle<T, R>(v[b], tw, k * ST, inv);
-            dft_any<T, R>(v[b], tw, N, inv);
-            base[b] = (j / NS) * NS * R + k;
-        }
-    }
-    __syncthreads();
-    if (active) {
-#pragma unroll
-        for (int b = 0; b < MB; ++b)
-#pragma unroll
-            for (int r = 0; r < R; ++r) line[p2slot(base[b] + r * NS) ^ lx] = v[b][r];
-    }
-    __syncthreads();
-}
-
-template <typename T, int N, int P, int NS>
-__device__ __forceinline__ void guarded_passes(cplx<T> *line, int t, const cplx<T> *tw, bool inv, bool active,
-                                               int lx) {
-    if constexpr (P < p2_npass(N)) {
-        constexpr int R = p2_radix(N, P);
-        guarded_pass<T, N, R, NS>(line, t, tw, inv, active, lx);
-        guarded_passes<T, N, P + 1, NS * R>(line, t, tw, inv, active, lx);
-    }
-}
-
-// Bank rotation of row r of the slab class buffer (rows are P apart, a
-// multiple of the bank count): XOR the slot with 4 (r mod 8).
-template <int P>
-__device__ __forceinline__ int row_xor(int r) {
-    return P >= 32 ? ((r & 7) << 2) : 0;
-}
-
-template <typename Real, int NZ, int P, int W>
-__global__ void __launch_bounds__(W * P) yz_slab_kernel(SlabArgs<Real> a) {
-    using C = cplx<Real>;
-    constexpr int S = 4;                 // ky classes
-    constexpr int ROWS = 3 * NZ;         // (component, z) rows
-    constexpr int NT = W * P;            // threads: one lane pair per class column
-    constexpr int NY = 2 * P;            // ny = S P / 2
-    constexpr int T = P / 16;            // lanes per row transform
-    static_assert(W == 2, "one lane pair per class column");
-    static_assert(ROWS * T <= NT, "every row has its lanes");
-    extern __shared__ unsigned char smem_raw[];
-    C *acc = reinterpret_cast<C *>(smem_raw);  // [ROWS][NY] output accumulator
-    C *cls = acc + ROWS * NY;                  // [ROWS][P] one ky class
-    const int t = (int)threadIdx.x;
-    const int frow = t / T, ft = t - frow * T;  // row transform lanes
-    const bool factive = frow < ROWS;
-    C *fline = cls + (factive ? frow : 0) * P;
-    const int flx = row_xor<P>(factive ? frow : 0);
-    const int col = t >> 1, p = t & 1;          // z convolution: lane pair per column
-    // The grid has hx - 1 workgroups; workgroup 0 also takes the last slab
-    // (kx = hx - 1), so an even hx - 1 fills whole rounds of the CUs.
-    const int nslab = blockIdx.x == 0 ? 2 : 1;
-#pragma unroll 1
-    for (int sl = 0; sl < nslab; ++sl) {
-        const int kx = sl == 0 ? (int)blockIdx.x : a.hx - 1;
-        const int64_t colbase = (int64_t)kx * a.pitch;
-        auto gaddr = [&](int r, int n) -> int64_t {
-            const int c = r / NZ, z = r - c * NZ;
-            return c * a.comp_stride + z * a.zstride + colbase + n;
-        };
-#pragma unroll 1
-        for (int s = 0; s < S; ++s) {
-            // fold class s from the slab in memory: u[n] = sum_b x[n + b P] w^(s (n + b P));
-            // thread t folds position n = t mod P of rows t / P + (NT / P) i
-            {
-                const int n = t & (P - 1);
-                const C w0 = a.tw_y[(s * n) & (a.py - 1)], w1 = a.tw_y[(s * (n + P)) & (a.py - 1)];
-                // rows in batches of FB, each batch's loads in flight together;
-                // for P >= 64 a wave's lanes share one row (a scalar offset)
-                constexpr int RI = (ROWS + NT / P - 1) / (NT / P), FB = RI < 6 ? RI : 6;
-                const int rb = P >= 64 ? __builtin_amdgcn_readfirstlane(t / P) : t / P;
-#pragma unroll
-                for (int i0 = 0; i0 < RI; i0 += FB) {
-                    C u[FB], x1[FB];
-#pragma unroll
-                    for (int i = 0; i < FB; ++i) {
-                        const int r = rb + (i0 + i) * (NT / P);
-                        u[i] = x1[i] = C{Real(0), Real(0)};
-                        if (i0 + i < RI && r < ROWS) {
-                            u[i] = a.data[gaddr(r, n)];
-                            x1[i] = a.data[gaddr(r, n + P)];
-                        }
-                    }
-#pragma unroll
-                    for (int i = 0; i < FB; ++i) {
-                        const int r = rb + (i0 + i) * (NT / P);
-                        if (i0 + i < RI && r < ROWS) {
-                            C v0 = u[i], v1 = x1[i];
-                            if (s) {
-                                v0 = cmul(v0, w0);
-                                v1 = cmul(v1, w1);
-                            }
-                            cls[r * P + (p2slot(n) ^ row_xor<P>(r))] = cadd(v0, v1);
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-            guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, false, factive, flx);
-            __syncthreads();
-            {
-                // two lanes per column (zconv_pair_kernel); the column is read
-                // and written only by its own lane pair
-                C v[3][NZ];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                    for (int z = 0; z < NZ; ++z) {
-                        const int r = c * NZ + z;
-                        C x = cls[r * P + (p2slot(col) ^ row_xor<P>(r))];
-                        if (p && z) x = cmul(x, a.tw_z[z]);
-                        v[c][z] = x;
-                    }
-                    half_dft<Real, NZ>(v[c], a.tw_z, false);
-                }
-#pragma unroll
-                for (int k = 0; k < NZ; ++k) {
-                    Real tt[6];
-                    tensor_at(a.K, kx, S * col + s, 2 * k + p, tt);
-                    apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    half_dft<Real, NZ>(v[c], a.tw_z, true);
-#pragma unroll
-                    for (int z = 0; z < NZ; ++z) {
-                        C y = v[c][z];
-                        if (p && z) y = cmul(y, cconj(a.tw_z[z]));
-                        const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
-                        const int r = c * NZ + z;
-                        if ((z & 1) == p) cls[r * P + (p2slot(col) ^ row_xor<P>(r))] = cadd(y, o);
-                    }
-                }
-            }
-            __syncthreads();
-            guarded_passes<Real, P, 0, 1>(fline, ft, a.tw_p, true, factive, flx);
-            __syncthreads();
-            // accumulate w^(-s n) IDFT_P(Y_s)[n mod P]: thread t owns position
-            // n = t mod NY of rows t / NY + (NT / NY) i
-            {
-                const int n = t & (NY - 1);
-                const C w = cconj(a.tw_y[(s * n) & (a.py - 1)]);
-                const int sl2 = p2slot(n & (P - 1));
-#pragma unroll 4
-                for (int r = t / NY; r < ROWS; r += NT / NY) {
-                    C y = cls[r * P + (sl2 ^ row_xor<P>(r))];
-                    if (s) y = cmul(y, w);
-                    acc[r * NY + n] = s ? cadd(acc[r * NY + n], y) : y;
-                }
-            }
-            __syncthreads();
-        }
-        {
-            const int n = t & (NY - 1);
-#pragma unroll 4
-            for (int r = t / NY; r < ROWS; r += NT / NY) a.data[gaddr(r, n)] = acc[r * NY + n];
-        }
-        __syncthreads();  // the accumulator is rewritten by the next slab
-    }
-}
-
 // Octant extraction after the setup transform of layout spec[c][z][kx][ky]
 // (z < pz, ky < py): entry (c, oz, kx, oy) of the tensor = Re spec[c][oz][kx][oy],
 // stored at kern_at.  `total` counts kcomp * kz_n * hx * (ny + 1) entries.
@@ -1473,6 +1097,16 @@ int pick_ztile(const LinePlan &lp, int ncomp, size_t ce, size_t budget = 65536) 
     return tile;
 }
 
+// LDS bytes one workgroup may allocate on `device`: lines longer than 64 KB
+// use the device's per-workgroup limit (160 KB on gfx950).
+size_t device_lds_limit(int device) {
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        (size_t)lds > 65536)
+        return (size_t)lds;
+    return 65536;
+}
+
 template <typename Real>
 KernelView<Real> kview(const Context &c) {
     const DemagPlan &dp = c.dp;
@@ -1529,21 +1163,6 @@ void with_line(const LinePlan &lp, bool allow_fixed, F &&f) {
     f(std::integral_constant<int, 0>{});
 }
 
-// The fused slab path (yz_slab_kernel): 3-D, nz <= 8, py a power of two with
-// P = py / 4 in 32..256 and the slab's LDS within 64 KB.
-size_t slab_lds(const DemagPlan &dp, size_t ce) { return (size_t)3 * dp.nz * (3 * (dp.py / 4)) * ce; }  // acc + class
-
-bool slab_geometry(const DemagPlan &dp, size_t ce) {
-    if (!dp.three_d || dp.nz > 8 || dp.projection) return false;
-    if (dp.py & (dp.py - 1)) return false;
-    const int P = dp.py / 4;
-    return P >= 16 && P <= 256 && slab_lds(dp, ce) <= 160 * 1024;
-}
-
-bool slab_eligible(const DemagPlan &dp, size_t ce) {
-    return dp.tune.yz_slab && dp.tune.fixed && dp.tw_p && slab_geometry(dp, ce);
-}
-
 // One contiguous-line pass (the 3-D y transforms): the pipelined kernel for
 // fixed power-of-two schedules with n_in = N or N / 2, else line_kernel.
 template <typename Real>
@@ -1591,79 +1210,18 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
     const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
     const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
-    xa.yblocks = (int)cdiv(dp.ny, dp.xr);
-    xa.iters = std::max(1, dp.tune.x_pipe);
-    const dim3 xpgrid((unsigned)cdiv(xa.yblocks, xa.iters), (unsigned)dp.nz, 3);
-    const bool xpipe = dp.tune.x_pipe > 0;
     {
         MagSource<Real, S> ms{src, c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
                               dp.nx, dp.ny, dp.nz, (int64_t)c.n};
         Timed t(c, "demag_x_r2c");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            if constexpr (N >= 32) {
-                if (xpipe) {
-                    hipLaunchKernelGGL((xfwd_pipe_kernel<Real, MagSource<Real, S>, N>), xpgrid, xblock, xlds,
-                                       c.stream, xa, ms, spec);
-                    return;
-                }
-            }
             hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, N>), xgrid, xblock, xlds, c.stream, xa, ms,
                                spec);
         });
     }
     const KernelView<Real> K = kview<Real>(c);
-    if (slab_eligible(dp, ce)) {
-        SlabArgs<Real> sa{};
-        sa.data = spec;
-        sa.tw_y = static_cast<const cplx<Real> *>(dp.tw_y);
-        sa.tw_p = static_cast<const cplx<Real> *>(dp.tw_p);
-        sa.tw_z = static_cast<const cplx<Real> *>(dp.tw_z);
-        sa.hx = dp.hx;
-        sa.ny = dp.ny;
-        sa.py = dp.py;
-        sa.pitch = dp.rows;
-        sa.zstride = (int64_t)dp.hx * dp.rows;
-        sa.comp_stride = (int64_t)dp.nz * dp.hx * dp.rows;
-        sa.K = K;
-        const int P = dp.py / 4;
-        const size_t lds = slab_lds(dp, ce);
-        Timed t(c, "demag_yz_slab");
-        auto go = [&](auto PC) {
-            constexpr int PP = decltype(PC)::value;
-            auto launch = [&](auto kernel) {
-                // above 64 KB of dynamic LDS the kernel must opt in, once per
-                // kernel and device (idempotent; a failed opt-in surfaces as
-                // the launch error)
-                static std::atomic<uint64_t> opted{0};
-                const uint64_t bit = c.device < 64 ? uint64_t(1) << c.device : 0;
-                if (!bit || !(opted.load(std::memory_order_relaxed) & bit)) {
-                    if (hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess)
-                        opted.fetch_or(bit, std::memory_order_relaxed);
-                }
-                hipLaunchKernelGGL(kernel, dim3(dp.hx - 1), dim3(2 * PP), lds, c.stream, sa);
-            };
-            auto pick = [&](auto NZC) { launch(yz_slab_kernel<Real, decltype(NZC)::value, PP, 2>); };
-            switch (dp.nz) {
-                case 1: pick(std::integral_constant<int, 1>{}); break;
-                case 2: pick(std::integral_constant<int, 2>{}); break;
-                case 3: pick(std::integral_constant<int, 3>{}); break;
-                case 4: pick(std::integral_constant<int, 4>{}); break;
-                case 5: pick(std::integral_constant<int, 5>{}); break;
-                case 6: pick(std::integral_constant<int, 6>{}); break;
-                case 7: pick(std::integral_constant<int, 7>{}); break;
-                default: pick(std::integral_constant<int, 8>{}); break;
-            }
-        };
-        switch (P) {
-            case 16: go(std::integral_constant<int, 16>{}); break;
-            case 32: go(std::integral_constant<int, 32>{}); break;
-            case 64: go(std::integral_constant<int, 64>{}); break;
-            case 128: go(std::integral_constant<int, 128>{}); break;
-            default: go(std::integral_constant<int, 256>{}); break;
-        }
-    } else if (dp.three_d) {
+    if (dp.three_d) {
         LineArgs<Real> la{};
         la.data = spec;
         la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
@@ -1680,7 +1238,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         }
         {
             Timed t(c, "demag_z_conv");
-            if (dp.pz <= z_reg_max<Real>() && !dp.tune.z_lds) {
+            if (dp.pz <= z_reg_max<Real>()) {
                 ZArgs<Real> za{};
                 za.data = spec;
                 za.tw = static_cast<const cplx<Real> *>(dp.tw_z);
@@ -1693,67 +1251,39 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.comp_stride = (int64_t)dp.nz * dp.hx * dp.rows;
                 za.K = K;
                 za.xcd = dp.tune.xcd;
-                za.half = dp.py / 2;
-                za.kstage = dp.tune.z_kstage;
-                za.groups = dp.tune.z_mirror ? (int)cdiv(za.half + 1, 64) : 0;
-                const dim3 zg((unsigned)(za.groups ? (uint64_t)dp.hx * za.groups : cdiv(2 * za.columns, 256))),
-                    zb(256);
-                const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
+                const dim3 zb(256);
                 const bool quad = dp.tune.z_quad && (dp.nz == 4 || dp.nz == 6 || dp.nz == 8);
                 // the staged tiles read the tensor blocks directly (no projection fallback)
-                const bool staged = dp.tune.z_stage && !quad && !dp.projection;
-                if (staged && dp.tune.z_pipe > 0) {
-                    const int it = dp.tune.z_pipe;
-                    const dim3 zt((unsigned)cdiv(cdiv(za.columns, kZTile), it));
+                const bool staged = !quad && !dp.projection;
+                auto by_nz = [&](auto launch) {
                     switch (dp.nz) {
-                        case 1: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 1>), zt, zb, 0, c.stream, za, it); break;
-                        case 2: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 2>), zt, zb, 0, c.stream, za, it); break;
-                        case 3: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 3>), zt, zb, 0, c.stream, za, it); break;
-                        case 4: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 4>), zt, zb, 0, c.stream, za, it); break;
-                        case 5: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 5>), zt, zb, 0, c.stream, za, it); break;
-                        case 6: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 6>), zt, zb, 0, c.stream, za, it); break;
-                        case 7: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 7>), zt, zb, 0, c.stream, za, it); break;
-                        default: hipLaunchKernelGGL((zconv_tile_pipe_kernel<Real, 8>), zt, zb, 0, c.stream, za, it); break;
+                        case 1: launch(std::integral_constant<int, 1>{}); break;
+                        case 2: launch(std::integral_constant<int, 2>{}); break;
+                        case 3: launch(std::integral_constant<int, 3>{}); break;
+                        case 4: launch(std::integral_constant<int, 4>{}); break;
+                        case 5: launch(std::integral_constant<int, 5>{}); break;
+                        case 6: launch(std::integral_constant<int, 6>{}); break;
+                        case 7: launch(std::integral_constant<int, 7>{}); break;
+                        default: launch(std::integral_constant<int, 8>{}); break;
                     }
-                } else if (staged && dp.tune.z_wide) {
-                    const dim3 zt((unsigned)cdiv(za.columns, 2 * kZTile));
-                    switch (dp.nz) {
-                        case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 2: hipLaunchKernelGGL((zconv_tile_kernel<Real, 2, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 3: hipLaunchKernelGGL((zconv_tile_kernel<Real, 3, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 4: hipLaunchKernelGGL((zconv_tile_kernel<Real, 4, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 5: hipLaunchKernelGGL((zconv_tile_kernel<Real, 5, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 6: hipLaunchKernelGGL((zconv_tile_kernel<Real, 6, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        case 7: hipLaunchKernelGGL((zconv_tile_kernel<Real, 7, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                        default: hipLaunchKernelGGL((zconv_tile_kernel<Real, 8, 2 * kZTile>), zt, zb, 0, c.stream, za); break;
-                    }
-                } else if (staged) {
+                };
+                if (staged) {
                     const dim3 zt((unsigned)cdiv(za.columns, kZTile));
-                    switch (dp.nz) {
-                        case 1: hipLaunchKernelGGL((zconv_tile_kernel<Real, 1>), zt, zb, 0, c.stream, za); break;
-                        case 2: hipLaunchKernelGGL((zconv_tile_kernel<Real, 2>), zt, zb, 0, c.stream, za); break;
-                        case 3: hipLaunchKernelGGL((zconv_tile_kernel<Real, 3>), zt, zb, 0, c.stream, za); break;
-                        case 4: hipLaunchKernelGGL((zconv_tile_kernel<Real, 4>), zt, zb, 0, c.stream, za); break;
-                        case 5: hipLaunchKernelGGL((zconv_tile_kernel<Real, 5>), zt, zb, 0, c.stream, za); break;
-                        case 6: hipLaunchKernelGGL((zconv_tile_kernel<Real, 6>), zt, zb, 0, c.stream, za); break;
-                        case 7: hipLaunchKernelGGL((zconv_tile_kernel<Real, 7>), zt, zb, 0, c.stream, za); break;
-                        default: hipLaunchKernelGGL((zconv_tile_kernel<Real, 8>), zt, zb, 0, c.stream, za); break;
-                    }
+                    by_nz([&](auto NZC) {
+                        hipLaunchKernelGGL((zconv_tile_kernel<Real, decltype(NZC)::value>), zt, zb, 0, c.stream, za);
+                    });
                 } else if (quad) {
+                    const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
                     switch (dp.nz) {
                         case 4: hipLaunchKernelGGL((zconv_quad_kernel<Real, 4>), zq, zb, 0, c.stream, za); break;
                         case 6: hipLaunchKernelGGL((zconv_quad_kernel<Real, 6>), zq, zb, 0, c.stream, za); break;
                         default: hipLaunchKernelGGL((zconv_quad_kernel<Real, 8>), zq, zb, 0, c.stream, za); break;
                     }
-                } else switch (dp.nz) {
-                    case 4: hipLaunchKernelGGL((zconv_pair_kernel<Real, 4>), zg, zb, 0, c.stream, za); break;
-                    case 6: hipLaunchKernelGGL((zconv_pair_kernel<Real, 6>), zg, zb, 0, c.stream, za); break;
-                    case 1: hipLaunchKernelGGL((zconv_pair_kernel<Real, 1>), zg, zb, 0, c.stream, za); break;
-                    case 2: hipLaunchKernelGGL((zconv_pair_kernel<Real, 2>), zg, zb, 0, c.stream, za); break;
-                    case 3: hipLaunchKernelGGL((zconv_pair_kernel<Real, 3>), zg, zb, 0, c.stream, za); break;
-                    case 5: hipLaunchKernelGGL((zconv_pair_kernel<Real, 5>), zg, zb, 0, c.stream, za); break;
-                    case 7: hipLaunchKernelGGL((zconv_pair_kernel<Real, 7>), zg, zb, 0, c.stream, za); break;
-                    default: hipLaunchKernelGGL((zconv_pair_kernel<Real, 8>), zg, zb, 0, c.stream, za); break;
+                } else {
+                    const dim3 zg((unsigned)cdiv(2 * za.columns, 256));
+                    by_nz([&](auto NZC) {
+                        hipLaunchKernelGGL((zconv_pair_kernel<Real, decltype(NZC)::value>), zg, zb, 0, c.stream, za);
+                    });
                 }
             } else {
                 ZLdsArgs<Real> za{};
@@ -1835,13 +1365,6 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         Timed t(c, "demag_x_c2r");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            if constexpr (N >= 32) {
-                if (xpipe) {
-                    hipLaunchKernelGGL((xinv_pipe_kernel<Real, FieldSink<Real>, N>), xpgrid, xblock, xlds,
-                                       c.stream, xa, spec, sink);
-                    return;
-                }
-            }
             hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream, xa, spec,
                                sink);
         });
@@ -1889,22 +1412,13 @@ void parse_tuning_string(DemagTuning &t, const char *text) {
             else if (k == "y_radix") t.y_radix = v;
             else if (k == "y_lanes") t.y_lanes = v;
             else if (k == "y_lpw") t.y_lpw = v;
-            else if (k == "z_lds") t.z_lds = v;
             else if (k == "fixed") t.fixed = v;
             else if (k == "z_quad") t.z_quad = v;
             else if (k == "xcd") t.xcd = v;
-            else if (k == "z_mirror") t.z_mirror = v;
             else if (k == "y_pipe") t.y_pipe = v;
             else if (k == "y_pipe_f") t.y_pipe_f = v;
-            else if (k == "z_stage") t.z_stage = v;
-            else if (k == "z_pipe") t.z_pipe = v;
-            else if (k == "z_wide") t.z_wide = v;
-            else if (k == "z_kstage") t.z_kstage = v;
             else if (k == "z_tile") t.z_tile = v;
-            else if (k == "x_pipe") t.x_pipe = v;
-            else if (k == "y_pad") t.y_pad = v;
             else if (k == "zc") t.zc = v;
-            else if (k == "yz_slab") t.yz_slab = v;
             else if (k == "y_split") t.y_split = v;
         }
         pos = end + 1;
@@ -2076,9 +1590,12 @@ bool upload_kern(Context &c, const std::vector<double> &kern) {
     return ctx_upload(c, dp.kern, f.data(), bytes, "hipMemcpy(tensor)", c.last_error);
 }
 
+// Launch geometry of the planned lines (no allocation): x rows per
+// workgroup, y lines per workgroup, long-z tile, thin-film y convolution
+// form.  `keep_split` >= 0 requires the thin-film form to stay as it is (a
+// re-plan may not change the spectrum's row count).
 template <typename Real>
-bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
-    DemagPlan &dp = c.dp;
+bool plan_geometry(DemagPlan &dp, std::string &err, int keep_split = -1) {
     const size_t ce = sizeof(cplx<Real>);
     {
         // tensor blocks: even-oz half, then odd-oz half, each padded to 16 bytes
@@ -2088,7 +1605,7 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
         dp.kblock = dp.kodd + (int)cdiv(no * dp.kcomp, vec) * vec;
     }
     if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, dp.tune.x_rows > 0 ? dp.tune.x_rows : 16, dp.lds_limit)) {
-        c.last_error = "x line FFT does not fit in one workgroup";
+        err = "x line FFT does not fit in one workgroup";
         return false;
     }
     if (dp.three_d) {
@@ -2096,10 +1613,10 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
         if (dp.tune.y_lpw > 0) dp.y_lines_per_wg = std::min(dp.y_lines_per_wg, dp.tune.y_lpw);
         if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit ||
             dp.y_lines_per_wg * dp.ly.threads > kMaxBlock) {
-            c.last_error = "y line FFT does not fit in LDS or 512 lanes";
+            err = "y line FFT does not fit in LDS or 512 lanes";
             return false;
         }
-        if (dp.pz > z_reg_max<Real>() || dp.tune.z_lds) {
+        if (dp.pz > z_reg_max<Real>()) {
             // at most 32 KB per workgroup: five or more workgroups share a CU
             // (measured at pz = 128: 209 us with 8 columns against 265 us
             // with 16 at 48 KB)
@@ -2107,28 +1624,40 @@ bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
             if (dp.tune.z_tile > 0) dp.z_tile = std::min(dp.z_tile, dp.tune.z_tile);
             if ((size_t)3 * dp.z_tile * (dp.lz.n + kPad) * ce > dp.lds_limit ||
                 3 * dp.z_tile * dp.lz.threads > kMaxBlock) {
-                c.last_error = "z line FFT does not fit in LDS or 512 lanes";
+                err = "z line FFT does not fit in LDS or 512 lanes";
                 return false;
             }
         }
     } else {
         dp.yconv_kpw = pick_lpw(dp.ly, ce, 3);
-        dp.y_split = dp.tune.y_split != 0 || (size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > dp.lds_limit ||
-                     3 * dp.yconv_kpw * dp.ly.threads > 1024;
+        const bool split = dp.tune.y_split != 0 || (size_t)3 * dp.yconv_kpw * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+                           3 * dp.yconv_kpw * dp.ly.threads > 1024;
+        if (keep_split >= 0 && split != (keep_split != 0)) {
+            err = "schedule changes the thin-film y convolution form";
+            return false;
+        }
+        dp.y_split = split;
         if (dp.y_split) {
             // the spectrum keeps all py ky entries of every kx row
             dp.rows = dp.py;
             dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
             if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit ||
                 dp.y_lines_per_wg * dp.ly.threads > kMaxBlock) {
-                c.last_error = "thin-film y line FFT does not fit in LDS or 512 lanes";
+                err = "thin-film y line FFT does not fit in LDS or 512 lanes";
                 return false;
             }
         }
     }
+    return true;
+}
+
+template <typename Real>
+bool setup_t(Context &c, const fullmag_fdm_plan_desc &plan) {
+    DemagPlan &dp = c.dp;
+    const size_t ce = sizeof(cplx<Real>);
+    if (!plan_geometry<Real>(dp, c.last_error)) return false;
     if (!upload_twiddles<Real>(c, dp.px, &dp.tw_x) || !upload_twiddles<Real>(c, dp.py, &dp.tw_y)) return false;
     if (dp.three_d && !upload_twiddles<Real>(c, dp.pz, &dp.tw_z)) return false;
-    if (slab_geometry(dp, sizeof(cplx<Real>)) && !upload_twiddles<Real>(c, dp.py / 4, &dp.tw_p)) return false;
     dp.spec_elems = 3ull * (dp.three_d ? dp.nz : 1) * dp.hx * dp.rows;
     if (!ctx_alloc(c, &dp.spec, dp.spec_elems * ce, "demag spectrum")) return false;
     if (dp.projection) return true;
@@ -2177,19 +1706,11 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     dp.py = 2 * c.ny;
     dp.pz = dp.three_d ? 2 * c.nz : 1;
     dp.hx = c.nx + 1;
-    // 3-D: kx rows of the spectrum are padded by y_pad complex slots so the
-    // x passes' transposed accesses are not a power-of-two stride apart
-    dp.rows = dp.three_d ? dp.py + std::max(0, dp.tune.y_pad) : dp.ny;
+    dp.rows = dp.three_d ? dp.py : dp.ny;
     dp.kcomp = dp.three_d ? 6 : 4;
     dp.kz_n = dp.three_d ? dp.nz + 1 : 1;
     dp.inv_npad = 1.0 / ((double)dp.px * dp.py * dp.pz);
-    {
-        // lines longer than 64 KB of LDS use the device's per-workgroup limit
-        int lds = 0;
-        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c.device) == hipSuccess &&
-            (size_t)lds > dp.lds_limit)
-            dp.lds_limit = (size_t)lds;
-    }
+    dp.lds_limit = device_lds_limit(c.device);
     parse_tuning(dp.tune);
     // measured at 512x512x8: four lanes per column are faster in FP64 (half
     // the registers per lane), the LDS-staged two-lane tile in FP32
@@ -2197,6 +1718,22 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
     if (!plan_lines(dp, c.last_error)) return false;
     if (c.mode == MODE_DOUBLE) return setup_t<double>(c, plan);
     return setup_t<float>(c, plan);
+}
+
+bool demag_replan(Context &c, const DemagTuning &t, std::string &err) {
+    if (!c.demag) {
+        c.dp.tune = t;
+        return true;
+    }
+    DemagPlan dp = c.dp;
+    dp.tune = t;
+    if (dp.tune.z_quad < 0) dp.tune.z_quad = c.mode == MODE_DOUBLE ? 1 : 0;
+    if (!plan_lines(dp, err)) return false;
+    const bool ok = c.mode == MODE_DOUBLE ? plan_geometry<double>(dp, err, c.dp.y_split ? 1 : 0)
+                                          : plan_geometry<float>(dp, err, c.dp.y_split ? 1 : 0);
+    if (!ok) return false;
+    c.dp = dp;  // same buffers: their sizes do not depend on the schedule
+    return true;
 }
 
 void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms) {
@@ -2214,8 +1751,10 @@ bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, doubl
     Context c;
     c.nx = nx; c.ny = ny; c.nz = nz; c.dx = dx; c.dy = dy; c.dz = dz;
     c.mode = MODE_DOUBLE;
+    if (!hip_ok(hipGetDevice(&c.device), "hipGetDevice", err)) return false;
     if (!hip_ok(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate", err)) return false;
     DemagPlan dp;
+    dp.lds_limit = device_lds_limit(c.device);
     dp.nx = nx; dp.ny = ny; dp.nz = nz;
     dp.three_d = !thin_film_2d;
     dp.px = 2 * nx; dp.py = 2 * ny; dp.pz = dp.three_d ? 2 * nz : 1;

@@ -97,7 +97,9 @@ int fullmag_fdm_ext_set_state_device(fullmag_fdm_backend *handle, const double *
  * branches of one hipGraph; batch_run(n) replays it n / steps_per_graph
  * times, runs the remainder eagerly and fills one step_stats per member
  * (stats of the final state).  Members must not be stepped or destroyed
- * outside the batch while it exists.  A failed create returns a handle whose
+ * outside the batch while it exists; uploads, state copies and new external
+ * fields between runs are honoured (the next run recomputes H_demag of the
+ * new m and recaptures the graph for a new field).  A failed create returns a handle whose
  * batch_last_error is set.  No counterpart in the reference ABI; for
  * parameter sweeps that run many small problems on one GPU. */
 typedef struct fullmag_fdm_batch fullmag_fdm_batch;
@@ -107,16 +109,19 @@ int fullmag_fdm_ext_batch_run(fullmag_fdm_batch *batch, uint64_t n_steps, fullma
 const char *fullmag_fdm_ext_batch_last_error(fullmag_fdm_batch *batch);
 void fullmag_fdm_ext_batch_destroy(fullmag_fdm_batch *batch);
 
-/* Launch-schedule autotuner: times (demag + stats) evaluations of the
- * current state under candidate schedules of the demag transforms and cell
- * kernels (coordinate search over the FULLMAG_FDM_TUNE launch-time keys),
- * keeps the fastest and writes it as "key=value,..." into chosen.  Results
- * of every schedule agree with the oracle; only speed differs. */
+/* Schedule autotuner: times (demag + stats) evaluations of the current
+ * state under candidate schedules (coordinate search over the FULLMAG_FDM_TUNE
+ * keys: FFT radices and rows / lines per workgroup, re-planned in place, and
+ * the launch-time keys), keeps the fastest and writes it as "key=value,..."
+ * into chosen.  Results of every schedule agree with the oracle; only speed
+ * differs.  For callers that run many steps of one grid. */
 int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen, uint64_t chosen_len);
 
 /* Applies a schedule string (as autotune writes it; FULLMAG_FDM_TUNE syntax)
- * to the launch-time knobs of a backend, e.g. to reuse one tuning across
- * backends of the same grid. */
+ * to a backend, re-planning its transforms without reallocation, e.g. to
+ * reuse one tuning across backends of the same grid.  ERR_INVALID (with
+ * last_error) when the schedule does not fit the grid; the backend then keeps
+ * its schedule. */
 int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedule);
 
 /* Bytes of device memory the backend holds. */

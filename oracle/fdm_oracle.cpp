@@ -505,6 +505,10 @@ struct Problem {
     bool precession, exchange, demag, has_ext;
     V3 hext;
     std::vector<uint8_t> mask;  // empty = all active
+    // per-pair exchange table (native/backends/fdm/src/exchange_fp64.cu:80-114):
+    // region id per cell and a 256 x 256 table of A_ij; empty = uniform A
+    std::vector<uint32_t> region;
+    std::vector<double> lut;
     double max_error, dt_min, dt_max, headroom;
     // state
     std::vector<V3> m;
@@ -522,6 +526,9 @@ struct Problem {
     std::vector<cd> bm[3], bh[3];
 
     bool active(size_t i) const { return mask.empty() || mask[i] != 0; }
+    bool use_lut() const { return !region.empty(); }
+    double a_pair(size_t i, size_t j) const { return lut[(size_t)region[i] * kRegions + region[j]]; }
+    static constexpr size_t kRegions = 256;  // FULLMAG_FDM_MAX_EXCHANGE_REGIONS
     double ms_at(size_t i) const { return ms_cell.empty() ? ms : ms_cell[i]; }
 };
 
@@ -539,7 +546,34 @@ void build_workspace(Problem &p) {
     }
 }
 
-// lib.rs:2247-2299 (per-cell Ms generalises the prefactor 2A/(mu0 Ms))
+// Per-pair exchange sum at cell f with the region table,
+// native/backends/fdm/src/exchange_fp64.cu:80-114:
+//   sum over the six faces of A(r_f, r_j) (q_j - q_f) / d^2,
+// where a face at the grid edge or towards an inactive cell takes j = f (its
+// difference is zero whatever A is), and A is read from row r_f of the table.
+// The engine (crates/fullmag-engine) has no region table; this follows the
+// reference GPU kernel that the runner drives whenever region_mask is set.
+V3 pair_sum(const Problem &p, const std::vector<V3> &q, size_t f) {
+    const size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
+    auto nbi = [&](size_t a, size_t b, size_t cc) -> size_t {
+        const size_t id = a + p.nx * (b + p.ny * cc);
+        return p.active(id) ? id : f;
+    };
+    const size_t xm = nbi(x > 0 ? x - 1 : x, y, z), xp = nbi(x + 1 < p.nx ? x + 1 : x, y, z);
+    const size_t ym = nbi(x, y > 0 ? y - 1 : y, z), yp = nbi(x, y + 1 < p.ny ? y + 1 : y, z);
+    const size_t zm = nbi(x, y, z > 0 ? z - 1 : z), zp = nbi(x, y, z + 1 < p.nz ? z + 1 : z);
+    const double ix = 1.0 / (p.dx * p.dx), iy = 1.0 / (p.dy * p.dy), iz = 1.0 / (p.dz * p.dz);
+    const V3 c = q[f];
+    V3 s;
+    for (int k = 0; k < 3; ++k)
+        s[k] = p.a_pair(f, xp) * (q[xp][k] - c[k]) * ix + p.a_pair(f, xm) * (q[xm][k] - c[k]) * ix +
+               p.a_pair(f, yp) * (q[yp][k] - c[k]) * iy + p.a_pair(f, ym) * (q[ym][k] - c[k]) * iy +
+               p.a_pair(f, zp) * (q[zp][k] - c[k]) * iz + p.a_pair(f, zm) * (q[zm][k] - c[k]) * iz;
+    return s;
+}
+
+// lib.rs:2247-2299 (per-cell Ms generalises the prefactor 2A/(mu0 Ms)); with a
+// region table, H_f = 2 / (mu0 Ms_f) pair_sum(m, f).
 std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m) {
     std::vector<V3> h(p.n);
     const double dx2 = p.dx * p.dx, dy2 = p.dy * p.dy, dz2 = p.dz * p.dz;
@@ -557,6 +591,10 @@ std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m) {
         V3 xm = nb(x > 0 ? x - 1 : 0, y, z), xp = nb(std::min(x + 1, p.nx - 1), y, z);
         V3 ym = nb(x, y > 0 ? y - 1 : 0, z), yp = nb(x, std::min(y + 1, p.ny - 1), z);
         V3 zm = nb(x, y, z > 0 ? z - 1 : 0), zp = nb(x, y, std::min(z + 1, p.nz - 1));
+        if (p.use_lut()) {
+            h[f] = scale(pair_sum(p, m, f), 2.0 / (MU0 * p.ms_at(f)));
+            continue;
+        }
         V3 lap;
         for (int k = 0; k < 3; ++k)
             lap[k] = (xp[k] - 2.0 * c[k] + xm[k]) / dx2 + (yp[k] - 2.0 * c[k] + ym[k]) / dy2 +
@@ -609,8 +647,9 @@ std::vector<V3> demag_apply(Problem &p, const std::vector<V3> &m, bool weight_ms
 std::vector<V3> demag_field(Problem &p, const std::vector<V3> &m) { return demag_apply(p, m, true); }
 
 // Discrete Laplacian sum_j w_ij (q_j - q_i) with the exchange stencil's
-// clamped and masked neighbours (exchange_field = (2A / (mu0 Ms)) lap(m)).
-// Symmetric in i, j, so it is its own transpose.
+// clamped and masked neighbours (exchange_field = (2A / (mu0 Ms)) lap(m)), or
+// with w_ij = A_ij from the region table (exchange_field = 2 / (mu0 Ms) lap(m)).
+// Symmetric in i, j (for a symmetric table), so it is its own transpose.
 std::vector<V3> lap_field(const Problem &p, const std::vector<V3> &q) {
     std::vector<V3> h(p.n);
     const double dx2 = p.dx * p.dx, dy2 = p.dy * p.dy, dz2 = p.dz * p.dz;
@@ -618,6 +657,11 @@ std::vector<V3> lap_field(const Problem &p, const std::vector<V3> &q) {
     for (long fl = 0; fl < (long)p.n; ++fl) {
         size_t f = (size_t)fl;
         if (!p.active(f)) { h[f] = {0, 0, 0}; continue; }
+        if (p.use_lut()) {
+            // weighted by A_ij (symmetric tables give a symmetric operator)
+            h[f] = pair_sum(p, q, f);
+            continue;
+        }
         size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
         const V3 c = q[f];
         auto nb = [&](size_t a, size_t b, size_t cc) -> V3 {
@@ -740,39 +784,50 @@ double max_error_norm(const std::vector<std::pair<const std::vector<V3> *, doubl
     return mx;
 }
 
-// lib.rs:1332-1407
+// lib.rs:1332-1407, one attempt of size dt from m0: the candidate state in
+// st and the returned error norm (Bogacki-Shampine 3(2), no FSAL).
+double rk23_attempt(Problem &p, const std::vector<V3> &m0, double dt, std::vector<V3> &st) {
+    const size_t n = p.n;
+    st.assign(n, V3{0, 0, 0});
+    auto k1 = rhs(p, m0);
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], 0.5 * dt)));
+    auto k2 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k2[i], 0.75 * dt)));
+    auto k3 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i) {
+        V3 d = scale(add(add(scale(k1[i], 2.0 / 9.0), scale(k2[i], 1.0 / 3.0)), scale(k3[i], 4.0 / 9.0)), dt);
+        st[i] = normalized(add(m0[i], d));
+    }
+    auto k4 = rhs(p, st);
+    return max_error_norm({{&k1, -5.0 / 72.0}, {&k2, 1.0 / 12.0}, {&k3, 1.0 / 9.0}, {&k4, -1.0 / 8.0}}, dt, n);
+}
+
+// lib.rs:1395-1404: dt for the next step (or the retry) from the error norm.
+double next_dt(const Problem &p, double dt, double err, double order_inv) {
+    return std::min(std::max(p.headroom * dt * std::pow(p.max_error / std::max(err, 1e-30), order_inv), p.dt_min),
+                    p.dt_max);
+}
+
 Report rk23_step(Problem &p, double dt) {
     dt = std::max(std::min(dt, p.dt_max), p.dt_min);
-    const size_t n = p.n;
-    std::vector<V3> m0 = p.m, st(n);
+    std::vector<V3> m0 = p.m, st;
     for (;;) {
-        auto k1 = rhs(p, m0);
-        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], 0.5 * dt)));
-        auto k2 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k2[i], 0.75 * dt)));
-        auto k3 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i) {
-            V3 d = scale(add(add(scale(k1[i], 2.0 / 9.0), scale(k2[i], 1.0 / 3.0)), scale(k3[i], 4.0 / 9.0)), dt);
-            st[i] = normalized(add(m0[i], d));
-        }
-        auto k4 = rhs(p, st);
-        double err = max_error_norm({{&k1, -5.0 / 72.0}, {&k2, 1.0 / 12.0}, {&k3, 1.0 / 9.0}, {&k4, -1.0 / 8.0}}, dt, n);
+        const double err = rk23_attempt(p, m0, dt, st);
         if (err <= p.max_error || dt <= p.dt_min) {
             p.m = st;
             p.time += dt;
-            double dt_next = std::min(std::max(p.headroom * dt * std::pow(p.max_error / std::max(err, 1e-30), 1.0 / 3.0),
-                                               p.dt_min),
-                                      p.dt_max);
-            return finish(p, dt, dt_next);
+            return finish(p, dt, next_dt(p, dt, err, 1.0 / 3.0));
         }
         double dt_new = p.headroom * dt * std::pow(p.max_error / err, 1.0 / 3.0);
         dt = std::min(std::max(dt_new, p.dt_min), p.dt_max);
     }
 }
 
-// lib.rs:1412-1568
-Report dp45_step(Problem &p, double dt) {
-    dt = std::max(std::min(dt, p.dt_max), p.dt_min);
+// lib.rs:1412-1568, one attempt of size dt from m0 with first slope k1: the
+// candidate in st, the last slope (the next step's FSAL slope) in k7 and the
+// returned error norm (Dormand-Prince 5(4)).
+double dp45_attempt(Problem &p, const std::vector<V3> &m0, const std::vector<V3> &k1, double dt,
+                    std::vector<V3> &st, std::vector<V3> &k7) {
     const size_t n = p.n;
     const double A21 = 1.0 / 5.0;
     const double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
@@ -783,44 +838,47 @@ Report dp45_step(Problem &p, double dt) {
     const double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0, B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
     const double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0, E5 = -17253.0 / 339200.0,
                  E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
-    std::vector<V3> m0 = p.m, st(n);
+    st.assign(n, V3{0, 0, 0});
+    for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], A21 * dt)));
+    auto k2 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        st[i] = normalized(add(m0[i], scale(add(scale(k1[i], A31), scale(k2[i], A32)), dt)));
+    auto k3 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A41), scale(k2[i], A42)), scale(k3[i], A43)), dt)));
+    auto k4 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A51), scale(k2[i], A52)),
+                                                add(scale(k3[i], A53), scale(k4[i], A54))),
+                                            dt)));
+    auto k5 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], A61), scale(k2[i], A62)), scale(k3[i], A63)),
+                                                add(scale(k4[i], A64), scale(k5[i], A65))),
+                                            dt)));
+    auto k6 = rhs(p, st);
+    for (size_t i = 0; i < n; ++i)
+        st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], B1), scale(k3[i], B3)), scale(k4[i], B4)),
+                                                add(scale(k5[i], B5), scale(k6[i], B6))),
+                                            dt)));
+    k7 = rhs(p, st);
+    return max_error_norm({{&k1, E1}, {&k3, E3}, {&k4, E4}, {&k5, E5}, {&k6, E6}, {&k7, E7}}, dt, n);
+}
+
+Report dp45_step(Problem &p, double dt) {
+    dt = std::max(std::min(dt, p.dt_max), p.dt_min);
+    std::vector<V3> m0 = p.m, st, k7;
     for (;;) {
         std::vector<V3> k1;
         if (p.has_fsal) { k1 = p.k_fsal; p.has_fsal = false; p.k_fsal.clear(); }
         else k1 = rhs(p, m0);
-        for (size_t i = 0; i < n; ++i) st[i] = normalized(add(m0[i], scale(k1[i], A21 * dt)));
-        auto k2 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i)
-            st[i] = normalized(add(m0[i], scale(add(scale(k1[i], A31), scale(k2[i], A32)), dt)));
-        auto k3 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i)
-            st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A41), scale(k2[i], A42)), scale(k3[i], A43)), dt)));
-        auto k4 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i)
-            st[i] = normalized(add(m0[i], scale(add(add(scale(k1[i], A51), scale(k2[i], A52)),
-                                                    add(scale(k3[i], A53), scale(k4[i], A54))),
-                                                dt)));
-        auto k5 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i)
-            st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], A61), scale(k2[i], A62)), scale(k3[i], A63)),
-                                                    add(scale(k4[i], A64), scale(k5[i], A65))),
-                                                dt)));
-        auto k6 = rhs(p, st);
-        for (size_t i = 0; i < n; ++i)
-            st[i] = normalized(add(m0[i], scale(add(add(add(scale(k1[i], B1), scale(k3[i], B3)), scale(k4[i], B4)),
-                                                    add(scale(k5[i], B5), scale(k6[i], B6))),
-                                                dt)));
-        auto k7 = rhs(p, st);
-        double err = max_error_norm({{&k1, E1}, {&k3, E3}, {&k4, E4}, {&k5, E5}, {&k6, E6}, {&k7, E7}}, dt, n);
+        const double err = dp45_attempt(p, m0, k1, dt, st, k7);
         if (err <= p.max_error || dt <= p.dt_min) {
             p.m = st;
             p.time += dt;
             p.k_fsal = k7;
             p.has_fsal = true;
-            double dt_next = std::min(std::max(p.headroom * dt * std::pow(p.max_error / std::max(err, 1e-30), 0.2),
-                                               p.dt_min),
-                                      p.dt_max);
-            return finish(p, dt, dt_next);
+            return finish(p, dt, next_dt(p, dt, err, 0.2));
         }
         double dt_new = p.headroom * dt * std::pow(p.max_error / err, 0.2);
         dt = std::min(std::max(dt_new, p.dt_min), p.dt_max);
@@ -921,7 +979,7 @@ void field_adjoint(Problem &p, const std::vector<V3> &m, const std::vector<V3> &
     if (p.exchange) {
         std::vector<V3> q(n);
         for (size_t i = 0; i < n; ++i)
-            q[i] = p.active(i) ? scale(g[i], 2.0 * p.a / (MU0 * p.ms_at(i))) : V3{0, 0, 0};
+            q[i] = p.active(i) ? scale(g[i], (p.use_lut() ? 2.0 : 2.0 * p.a) / (MU0 * p.ms_at(i))) : V3{0, 0, 0};
         std::vector<V3> lq = lap_field(p, q);
         for (size_t i = 0; i < n; ++i) {
             lam[i] = add(lam[i], lq[i]);
@@ -1116,6 +1174,38 @@ int oracle_problem_step(void *h, double dt, double *report) {
     }
     report_to(r, report);
     return 0;
+}
+
+// Region ids (one per cell, < 256) and the 256 x 256 exchange table A_ij.
+// lut NULL: the default table of native/backends/fdm/src/api.cpp:215-231,
+// A on the diagonal and 0 elsewhere.  region NULL: back to uniform exchange.
+int oracle_problem_set_regions(void *h, const uint32_t *region, const double *lut) {
+    auto *p = static_cast<Problem *>(h);
+    const size_t R = Problem::kRegions;
+    if (!region) {
+        p->region.clear();
+        p->lut.clear();
+        return 0;
+    }
+    for (size_t i = 0; i < p->n; ++i)
+        if (region[i] >= R) return -1;
+    p->region.assign(region, region + p->n);
+    p->lut.assign(R * R, 0.0);
+    if (lut) std::memcpy(p->lut.data(), lut, R * R * sizeof(double));
+    else
+        for (size_t r = 0; r < R; ++r) p->lut[r * R + r] = p->a;
+    return 0;
+}
+
+// Error norm (lib.rs:1653-1674) of one attempted RK23 or DP45 step of size
+// dt from the current state, which is left unchanged.  The first slope is
+// evaluated at the state.  Returns -1 for other integrators.
+double oracle_problem_trial_error(void *h, double dt) {
+    auto *p = static_cast<Problem *>(h);
+    std::vector<V3> st, k7;
+    if (p->integrator == RK23) return rk23_attempt(*p, p->m, dt, st);
+    if (p->integrator == DP45) return dp45_attempt(*p, p->m, rhs(*p, p->m), dt, st, k7);
+    return -1.0;
 }
 
 void oracle_problem_get_m(void *h, double *out) {

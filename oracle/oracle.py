@@ -67,6 +67,10 @@ def lib():
         L.oracle_problem_time.restype = d
         L.oracle_problem_time.argtypes = [ctypes.c_void_p]
         L.oracle_problem_observe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_problem_set_regions.restype = i
+        L.oracle_problem_set_regions.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_problem_trial_error.restype = d
+        L.oracle_problem_trial_error.argtypes = [ctypes.c_void_p, d]
         L.oracle_num_threads.restype = i
         L.oracle_set_num_threads.argtypes = [i]
         _lib = L
@@ -154,6 +158,8 @@ class OracleConfig:
     h_ext: Optional[tuple] = None  # A/m
     active_mask: Optional[np.ndarray] = None
     ms_cell: Optional[np.ndarray] = None  # extension: per-cell Ms
+    region_mask: Optional[np.ndarray] = None  # region id per cell (< 256)
+    exchange_lut: Optional[np.ndarray] = None  # 256 x 256 A_ij; None: A on the diagonal
     max_error: float = 0.0
     dt_min: float = 0.0
     dt_max: float = 0.0
@@ -188,6 +194,16 @@ class OracleProblem:
             None if self._mask is None else self._mask.ctypes.data_as(ctypes.c_void_p),
             None if self._ms is None else self._ms.ctypes.data_as(ctypes.c_void_p),
             _ptr(m0))
+        self._region = self._lut = None
+        if cfg.region_mask is not None:
+            self._region = np.ascontiguousarray(cfg.region_mask, dtype=np.uint32).ravel()
+            if cfg.exchange_lut is not None:
+                self._lut = np.ascontiguousarray(cfg.exchange_lut, dtype=np.float64).reshape(256 * 256)
+            rc = lib().oracle_problem_set_regions(
+                self._h, self._region.ctypes.data_as(ctypes.c_void_p),
+                None if self._lut is None else self._lut.ctypes.data_as(ctypes.c_void_p))
+            if rc != 0:
+                raise ValueError("region ids must be below 256")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -230,6 +246,11 @@ class OracleProblem:
         lib().oracle_problem_heun_adjoint(self._h, dt, _ptr(m0), _ptr(lam1), _ptr(lam0),
                                           gms.ctypes.data_as(ctypes.c_void_p), gh.ctypes.data_as(ctypes.c_void_p))
         return lam0, gms, gh
+
+    def trial_error(self, dt: float) -> float:
+        """Error norm of one attempted RK23 / DP45 step of size dt from the
+        current state (state unchanged)."""
+        return lib().oracle_problem_trial_error(self._h, dt)
 
     @property
     def time(self) -> float:

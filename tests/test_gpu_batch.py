@@ -74,3 +74,59 @@ def test_autotuned_schedule_matches_oracle(fa, oracle_mod, cells):
         cpu.step(5e-14)
     assert float(np.abs(b.copy_m() - cpu.m).max()) <= 1e-12, chosen
     b.close()
+
+
+def test_batch_follows_member_changes_between_runs(fa):
+    """An upload of m and a new external field on a member between two runs:
+    the batch recomputes H_demag at the uploaded m and recaptures the graph
+    with the new field, so members stay bit-identical to eager stepping."""
+    specs = MEMBERS[:3]
+    batch_members = [member(fa, c, p, 60 + i) for i, (c, _, p) in enumerate(specs)]
+    eager = [member(fa, c, p, 60 + i) for i, (c, _, p) in enumerate(specs)]
+    dts = [dt for _, dt, _ in specs]
+    run = fa.BatchRun(batch_members, dts, steps_per_graph=3)
+
+    def advance(n):
+        run.run(n)
+        for e, dt in zip(eager, dts):
+            for _ in range(n):
+                e.step(dt)
+        for b, e in zip(batch_members, eager):
+            np.testing.assert_array_equal(b.copy_m(), e.copy_m())
+
+    advance(6)
+    n0 = specs[0][0][0] * specs[0][0][1] * specs[0][0][2]
+    m_new = fa.generate_random_unit_vectors(999, n0)
+    for x in (batch_members[0], eager[0]):
+        x.upload_magnetization(m_new)
+    for x in (batch_members[1], eager[1]):
+        x.set_external_field((-3e4, 1e4, 5e3))
+    advance(7)
+    np.testing.assert_array_equal(batch_members[1].copy_h_ext(), eager[1].copy_h_ext())
+    run.close()
+    for b in batch_members + eager:
+        b.close()
+
+
+def test_schedule_replan_knobs_match_oracle(fa, oracle_mod):
+    """set_schedule with plan-shaping knobs (radices, rows per x workgroup,
+    y lines per workgroup) re-plans the transforms in place; the field stays
+    the oracle's, and the autotuner reports those knobs."""
+    nx, ny, nz = 48, 40, 4
+    n = nx * ny * nz
+    m0 = fa.generate_random_unit_vectors(12, n)
+    b = fa.NativeFdmBackend(fa.FdmPlan(cells=(nx, ny, nz), cell_size=(4e-9, 4e-9, 2e-9), initial_magnetization=m0,
+                                       saturation_magnetisation=8.6e5, exchange_stiffness=1.3e-11, damping=0.05,
+                                       precision="double", demag_kernel="device"))
+    cfg = oracle_mod.OracleConfig(nx=nx, ny=ny, nz=nz, dx=4e-9, dy=4e-9, dz=2e-9, ms=8.6e5, a=1.3e-11, alpha=0.05)
+    hd = oracle_mod.OracleProblem(cfg, m0).observe()["h_demag"]
+    for sched in ("x_radix=4,y_radix=8", "x_rows=2,y_lpw=1", "x_radix=8,fixed=0", "x_radix=16,y_radix=16"):
+        b.set_schedule(sched)
+        b.refresh_observables()
+        np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max(), err_msg=sched)
+    chosen = b.autotune(reps=1)
+    for key in ("x_radix=", "y_radix=", "x_rows=", "y_lpw="):
+        assert key in chosen
+    b.refresh_observables()
+    np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max(), err_msg=chosen)
+    b.close()

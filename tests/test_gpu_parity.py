@@ -23,7 +23,7 @@ MU0 = 4.0 * math.pi * 1e-7
 
 def make_pair(fa, O, *, cells, cell, ms, a, alpha, m0, h_ext=None, exchange=True, demag=True,
               integrator="heun", precision="double", mask=None, kernel="spectra", disable_precession=False,
-              adaptive=None, ms_cell=None):
+              adaptive=None, ms_cell=None, region=None, pairs=()):
     nx, ny, nz = cells
     spectra = None
     if demag and kernel == "spectra":
@@ -33,14 +33,21 @@ def make_pair(fa, O, *, cells, cell, ms, a, alpha, m0, h_ext=None, exchange=True
                       exchange_stiffness=a, damping=alpha, precision=precision, integrator=integrator,
                       enable_exchange=exchange, enable_demag=demag, external_field=h_ext, active_mask=mask,
                       demag_kernel=spectra if spectra is not None else kernel,
-                      disable_precession=disable_precession, ms_cell=ms_cell, **adaptive)
+                      disable_precession=disable_precession, ms_cell=ms_cell, region_mask=region,
+                      inter_region_exchange=list(pairs), **adaptive)
     gpu = fa.NativeFdmBackend(plan)
+    lut = None
+    if region is not None:
+        lut = np.zeros((256, 256))
+        lut[np.arange(256), np.arange(256)] = a
+        for ri, rj, aij in pairs:
+            lut[ri, rj] = lut[rj, ri] = aij
     cfg = O.OracleConfig(nx=nx, ny=ny, nz=nz, dx=cell[0], dy=cell[1], dz=cell[2], ms=ms, a=a, alpha=alpha,
                          integrator=integrator, exchange=exchange, demag=demag, h_ext=h_ext,
                          active_mask=mask, precession=not disable_precession, ms_cell=ms_cell,
                          max_error=adaptive.get("adaptive_max_error", 0.0),
                          dt_min=adaptive.get("adaptive_dt_min", 0.0), dt_max=adaptive.get("adaptive_dt_max", 0.0),
-                         headroom=adaptive.get("adaptive_headroom", 0.0))
+                         headroom=adaptive.get("adaptive_headroom", 0.0), region_mask=region, exchange_lut=lut)
     cpu = O.OracleProblem(cfg, m0)
     return gpu, cpu
 
@@ -304,13 +311,9 @@ def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
 
 # Launch-schedule knobs change only how the transforms are scheduled; every
 # schedule must give the oracle's field and trajectory.
-SCHEDULES = ["", "z_lds=1", "fixed=0", "z_quad=1", "z_quad=0", "z_mirror=1", "z_stage=0", "x_rows=4,y_lpw=1",
-             "xcd=0", "x_radix=4,y_radix=8", "x_lanes=128,y_lanes=64", "y_pipe=0", "y_pipe=3",
-             "x_pipe=2", "x_pipe=3,x_rows=4", "x_pipe=4,x_rows=8", "y_pad=16", "y_pad=8,z_stage=0",
-             "y_pad=24,z_quad=1", "y_pad=8,z_lds=1", "y_pad=8,z_mirror=1", "zc=1", "zc=3,xcd=0", "yz_slab=1",
-             "yz_slab=1,z_quad=1", "z_pipe=1", "z_pipe=3", "z_pipe=8,xcd=0", "z_pipe=2,y_pad=8", "z_wide=1",
-             "z_wide=1,y_pad=8,xcd=0", "x_rows=4", "x_rows=2,y_pad=8", "y_pipe_f=1", "y_pipe_f=4,y_pipe=1", "z_kstage=0",
-             "z_kstage=0,y_pad=8"]
+SCHEDULES = ["", "fixed=0", "z_quad=1", "z_quad=0", "x_rows=4,y_lpw=1", "xcd=0", "x_radix=4,y_radix=8",
+             "x_lanes=128,y_lanes=64", "y_pipe=0", "y_pipe=3", "zc=1", "zc=3,xcd=0", "x_rows=4", "x_rows=2",
+             "y_pipe_f=1", "y_pipe_f=4,y_pipe=1", "x_radix=8", "y_radix=4,fixed=0"]
 
 
 @pytest.mark.parametrize("cells", [(32, 16, 4), (64, 32, 8), (16, 16, 6)])
@@ -332,32 +335,13 @@ def test_demag_schedule_variants_match_oracle(fa, oracle_mod, monkeypatch, cells
     assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12
 
 
-@pytest.mark.parametrize("cells", [(64, 32, 8), (16, 128, 3), (20, 256, 5), (8, 512, 2), (24, 64, 7), (12, 64, 1)])
-@pytest.mark.parametrize("tune", ["yz_slab=1", "yz_slab=1,y_pad=8"])
-def test_demag_fused_slab_matches_oracle(fa, oracle_mod, monkeypatch, cells, tune):
-    """The fused y transform + z convolution per kx slab (ky split into four
-    classes of py/4 = 16..256 points, odd and even nz)."""
-    monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
-    nx, ny, nz = cells
-    n = nx * ny * nz
-    m0 = fa.generate_random_unit_vectors(29 + n, n)
-    gpu, cpu = make_pair(fa, oracle_mod, cells=cells, cell=(4e-9, 3e-9, 2e-9), ms=8.6e5, a=1.3e-11, alpha=0.05,
-                         m0=m0, h_ext=(1e4, 0.0, 2e3), kernel="device")
-    hd = cpu.observe()["h_demag"]
-    np.testing.assert_allclose(gpu.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max())
-    for _ in range(3):
-        gpu.step(5e-14)
-        cpu.step(5e-14)
-    assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12
-
-
-@pytest.mark.parametrize("cells,tune", [((13, 7, 1), ""), ((10, 9, 3), ""), ((10, 9, 3), "y_pad=3"),
-                                        ((64, 48, 8), "y_pad=1"), ((33, 17, 5), "x_rows=4"),
-                                        ((19, 17, 2), "")])
+@pytest.mark.parametrize("cells,tune", [((13, 7, 1), ""), ((10, 9, 3), ""), ((10, 9, 3), "x_rows=2"),
+                                        ((64, 48, 8), "z_quad=1"), ((33, 17, 5), "x_rows=4"),
+                                        ((19, 17, 2), ""), ((7, 5, 1), "")])
 def test_mixed_demag_odd_shapes_match_oracle(fa, oracle_mod, monkeypatch, cells, tune):
-    """FP32 demag (mixed precision) on odd grids and odd spectrum row pitches,
-    where the x forward unpack stores rows one at a time, against the FP64
-    oracle field to FP32 rounding."""
+    """FP32 demag (mixed precision) on odd grids, including odd thin-film row
+    pitches (ny odd), where the x forward unpack stores rows one at a time,
+    against the FP64 oracle field to FP32 rounding."""
     monkeypatch.setenv("FULLMAG_FDM_TUNE", tune)
     nx, ny, nz = cells
     n = nx * ny * nz
@@ -551,6 +535,42 @@ def test_region_table_decoupled_equals_masked_neighbour(fa, oracle_mod):
     c = fa.NativeFdmBackend(fa.FdmPlan(region_mask=np.zeros(n, np.uint32), **common))
     d = fa.NativeFdmBackend(fa.FdmPlan(**common))
     np.testing.assert_array_equal(c.copy_h_ex(), d.copy_h_ex())
+
+
+@pytest.mark.parametrize("precision,masked", [("double", False), ("double", True), ("mixed", False)])
+def test_region_table_coupled_regions_match_oracle(fa, oracle_mod, precision, masked):
+    """Three regions with A_01 = 0.3 A, A_12 = 1.7 A and A_02 = 0 (per-pair
+    table of native/backends/fdm/src/exchange_fp64.cu:80-114) against the
+    oracle's table exchange: fields, energies and 20 Heun steps with demag."""
+    nx, ny, nz = 12, 9, 3
+    n = nx * ny * nz
+    a = 1.3e-11
+    region = np.zeros(n, np.uint32)
+    r3 = region.reshape(nz, ny, nx)
+    r3[:, :, 4:8] = 1
+    r3[:, :, 8:] = 2
+    r3[2, :, :] = np.where(r3[2] == 0, 2, r3[2])
+    mask = None
+    if masked:
+        mask = np.ones(n, np.uint8)
+        mask[np.random.default_rng(4).choice(n, 30, replace=False)] = 0
+    m0 = fa.generate_random_unit_vectors(314, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=(nx, ny, nz), cell=(3e-9, 2e-9, 4e-9), ms=8e5, a=a, alpha=0.1,
+                         m0=m0, h_ext=(5e3, 0.0, 1e4), precision=precision, mask=mask, region=region,
+                         pairs=[(0, 1, 0.3 * a), (1, 2, 1.7 * a), (0, 2, 0.0)])
+    o = cpu.observe()
+    tol_f = 1e-11 if precision == "double" else 2e-6
+    scale = np.abs(o["h_ex"]).max()
+    np.testing.assert_allclose(gpu.copy_h_ex(), o["h_ex"], rtol=0, atol=tol_f * scale)
+    for _ in range(20):
+        g = gpu.step(1e-13)
+        r = cpu.step(1e-13)
+    dm = float(np.abs(gpu.copy_m() - cpu.m).max())
+    print(f"region table {precision} masked={masked}: max|dm|={dm:.3e}")
+    assert dm <= (1e-10 if precision == "double" else 1e-5)
+    tol_e = 1e-9 if precision == "double" else 1e-5
+    assert rel(g.e_ex, r["e_ex"]) <= tol_e
+    assert rel(g.e_total, r["e_total"]) <= tol_e
 
 
 def test_config2_full_size_properties(fa):
