@@ -726,134 +726,127 @@ __device__ __forceinline__ void ztile_store(const ZArgs<Real> &a, cplx<Real> (*t
     }
 }
 
-// The register convolution of the tile's columns (two lanes per column, as
-// zconv_pair_kernel), in place in LDS; lane pair t/2 takes columns t/2,
-// t/2 + 128, ...  A column is read and written only by its own lane pair,
-// which sits in one wave, so no barrier separates the reads from the writes.
-// Per-wave LDS staging of the tensor halves (FP32): a column's block is
-// stored compactly (KS reals: NE even-oz then NO odd-oz entries) for the 32
-// columns of each wave.
+// The lane's half of its column's octant tensor block (the entries of its
+// kz parity, six components each), in registers.  Issued at the start of the
+// tile kernel together with the tile's loads, so one memory wait covers both.
+// Pad columns and the projection fallback keep zeros.
+// FP64 keeps only the block's address (30 doubles would double the
+// registers) and reads each entry where it is applied.
 template <typename Real, int NZ>
-struct ZTensorStage {
-    static constexpr int NE = NZ / 2 + 1, NO = (NZ + 1) / 2, KS = 6 * (NE + NO);
-    static constexpr bool ON = sizeof(Real) == 4;
-    static constexpr int ELEMS = ON ? 4 * 32 * KS : 1;  // four waves of 32 columns
+struct ZTensorRegs {
+    static constexpr int NE = NZ / 2 + 1, NO = (NZ + 1) / 2;
+    static constexpr bool REGS = sizeof(Real) == 4;
+    static constexpr int NV = (6 * NE + 3) / 4;  // 16-byte loads of the longer (even) half
+    Real k[REGS ? NV * 4 : 1];
+    const Real *blk = nullptr;
+    bool valid = false;
+    __device__ __forceinline__ void load(const ZArgs<Real> &a, bool col_valid, int kx, int ky, int p) {
+        valid = col_valid && a.K.k;
+        const int oy = ky <= a.K.ny ? ky : a.py - ky;
+        blk = valid ? a.K.k + ((int64_t)kx * (a.K.ny + 1) + oy) * a.K.kb + (p ? a.K.kodd : 0) : a.K.k;
+        if constexpr (REGS) {
+#pragma unroll
+            for (int e = 0; e < NV * 4; ++e) k[e] = Real(0);
+            if (!valid) return;
+            const float4 *src = reinterpret_cast<const float4 *>(blk);
+            const int nv = p ? (6 * NO + 3) / 4 : NV;
+#pragma unroll
+            for (int e = 0; e < NV; ++e) {
+                if (e < nv) {
+                    const float4 q = src[e];
+                    k[4 * e] = q.x;
+                    k[4 * e + 1] = q.y;
+                    k[4 * e + 2] = q.z;
+                    k[4 * e + 3] = q.w;
+                }
+            }
+        }
+    }
 };
 
+// The register convolution of one column of the tile (two lanes per column,
+// as zconv_pair_kernel), in place in LDS: lane pair t/2 takes column t/2.  A
+// column is read and written only by its own lane pair, which sits in one
+// wave, so no barrier separates the reads from the writes.
 template <typename Real, int NZ, int TW = kZTile>
 __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> (*tile)[TW + ZTile<Real, NZ, TW>::V],
-                                               int64_t col0, int t, Real *kstage = nullptr) {
+                                               int j, int p, int ky, const ZTensorRegs<Real, NZ> &kr) {
     using C = cplx<Real>;
-    const int p = t & 1;
-#pragma unroll 1
-    for (int j = t >> 1; j < TW; j += 128) {
-        const int64_t col = col0 + j;
-        bool valid = col < a.columns;
-        const int kx = valid ? (int)(col / a.pitch) : 0;
-        const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
-        valid = valid && ky < a.py;  // pad columns: transformed with a zero tensor, never read
-        C v[3][NZ];
+    C v[3][NZ];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 3; ++c) {
 #pragma unroll
-            for (int z = 0; z < NZ; ++z) {
-                C x = tile[c * NZ + z][j];
-                if (p && z) x = cmul(x, a.tw[z]);
-                v[c][z] = x;
-            }
-            half_dft<Real, NZ>(v[c], a.tw, false);
+        for (int z = 0; z < NZ; ++z) {
+            C x = tile[c * NZ + z][j];
+            if (p && z) x = cmul(x, a.tw[z]);
+            v[c][z] = x;
         }
-        // The lane's half of the column's tensor block (its kz parity): entry
-        // (kz <= nz ? kz : pz - kz) / 2, six components in three 8-byte (FP32)
-        // or 16-byte (FP64) loads.
-        constexpr int VEC = 16 / (int)sizeof(Real), NE = NZ / 2 + 1;
-        constexpr int B0 = (NE * 6 + VEC - 1) / VEC * VEC;
-        using V2 = typename std::conditional<sizeof(Real) == 4, float2, double2>::type;
-        const int oy = ky <= a.K.ny ? ky : a.py - ky;
-        const Real *blk = a.K.k + (valid ? ((int64_t)kx * (a.K.ny + 1) + oy) * a.K.kb + (p ? B0 : 0) : 0);
-        const Real sy = ky <= a.K.ny ? Real(1) : Real(-1);
-        // staged: the lane copies its half of the block into this wave's
-        // LDS region with 16-byte loads, then reads entries by offset
-        using S = ZTensorStage<Real, NZ>;
-        Real *kw = nullptr;
-        if constexpr (S::ON) {
-            if (kstage) {
-                kw = kstage + ((t >> 6) * 32 + ((t >> 1) & 31)) * S::KS + (p ? 6 * S::NE : 0);
-                if (valid) {
-                    const float4 *src = reinterpret_cast<const float4 *>(blk);
-                    const int nv = p ? (6 * S::NO + 3) / 4 : (6 * S::NE + 3) / 4;
-                    const int nreal = p ? 6 * S::NO : 6 * S::NE;
+        half_dft<Real, NZ>(v[c], a.tw, false);
+    }
+    const Real sy = ky <= a.K.ny ? Real(1) : Real(-1);
 #pragma unroll
-                    for (int e = 0; e < (6 * S::NE + 3) / 4; ++e) {
-                        if (e < nv) {
-                            const float4 q = src[e];
-                            // compact copy: drop the half's padding (8-byte stores)
-                            if (4 * e + 1 < nreal) reinterpret_cast<float2 *>(kw)[2 * e] = float2{q.x, q.y};
-                            if (4 * e + 3 < nreal) reinterpret_cast<float2 *>(kw)[2 * e + 1] = float2{q.z, q.w};
-                        }
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
+    for (int k = 0; k < NZ; ++k) {
+        // lane parity p owns kz = 2k + p; its octant entry is (kz or pz - kz) / 2
+        const int kz0 = 2 * k, kz1 = 2 * k + 1;
+        const int i0 = (kz0 <= NZ ? kz0 : 2 * NZ - kz0) >> 1, i1 = (kz1 <= NZ ? kz1 : 2 * NZ - kz1) >> 1;
+        const Real s0 = kz0 <= NZ ? Real(1) : Real(-1), s1 = kz1 <= NZ ? Real(1) : Real(-1);
+        const Real sz = p ? s1 : s0;
+        Real tt[6];
+        if constexpr (sizeof(Real) == 4) {
+            // select between two fixed registers with a bit mask (an index
+            // chosen at run time would move the array to scratch)
+            const uint32_t msk = 0u - (uint32_t)p;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const uint32_t b0 = __float_as_uint(kr.k[6 * i0 + q]), b1 = __float_as_uint(kr.k[6 * i1 + q]);
+                tt[q] = __uint_as_float((b0 & ~msk) | (b1 & msk));
             }
+        } else {
+            const Real *e = kr.blk + 6 * (p ? i1 : i0);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) tt[q] = kr.valid ? e[q] : Real(0);
         }
+        tt[3] = sy * tt[3];
+        tt[4] = sz * tt[4];
+        tt[5] = sy * sz * tt[5];
+        apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
+    }
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) {
-            Real tt[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
-            if (valid) {
-                const int kz0 = 2 * k, kz1 = 2 * k + 1;
-                const int i0 = (kz0 <= NZ ? kz0 : 2 * NZ - kz0) >> 1, i1 = (kz1 <= NZ ? kz1 : 2 * NZ - kz1) >> 1;
-                const Real sz = p ? (kz1 <= NZ ? Real(1) : Real(-1)) : (kz0 <= NZ ? Real(1) : Real(-1));
-                V2 t01, t23, t45;
-                if (kstage) {
-                    const V2 *e = reinterpret_cast<const V2 *>(kw + 6 * (p ? i1 : i0));
-                    t01 = e[0];
-                    t23 = e[1];
-                    t45 = e[2];
-                } else {
-                    const V2 *e = reinterpret_cast<const V2 *>(blk + 6 * (p ? i1 : i0));
-                    t01 = e[0];
-                    t23 = e[1];
-                    t45 = e[2];
-                }
-                tt[0] = t01.x;
-                tt[1] = t01.y;
-                tt[2] = t23.x;
-                tt[3] = sy * t23.y;
-                tt[4] = sz * t45.x;
-                tt[5] = sy * sz * t45.y;
-            }
-            apply_tensor(tt, v[0][k], v[1][k], v[2][k]);
-        }
+    for (int c = 0; c < 3; ++c) {
+        half_dft<Real, NZ>(v[c], a.tw, true);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            half_dft<Real, NZ>(v[c], a.tw, true);
-#pragma unroll
-            for (int z = 0; z < NZ; ++z) {
-                C y = v[c][z];
-                if (p && z) y = cmul(y, cconj(a.tw[z]));
-                const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
-                if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
-            }
+        for (int z = 0; z < NZ; ++z) {
+            C y = v[c][z];
+            if (p && z) y = cmul(y, cconj(a.tw[z]));
+            const C o = {__shfl_xor(y.x, 1), __shfl_xor(y.y, 1)};
+            if ((z & 1) == p) tile[c * NZ + z][j] = cadd(y, o);
         }
     }
 }
 
 template <typename Real, int NZ, int TW = kZTile>
 __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+    static_assert(TW == 128, "one column per lane pair of a 256-lane workgroup");
     using TL = ZTile<Real, NZ, TW>;
-    using S = ZTensorStage<Real, NZ>;
     __shared__ cplx<Real> tile[TL::ROWS][TW + TL::V];
-    __shared__ __align__(16) Real kstage[S::ELEMS];
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const int64_t col0 = (int64_t)blk * TW;
     const int t = (int)threadIdx.x;
+    const int j = t >> 1, p = t & 1;
+    const int64_t col = col0 + j;
+    bool valid = col < a.columns;
+    const int kx = valid ? (int)(col / a.pitch) : 0;
+    const int ky = valid ? (int)(col - (int64_t)kx * a.pitch) : 0;
+    valid = valid && ky < a.py;  // pad columns: transformed with a zero tensor, never read
+    ZTensorRegs<Real, NZ> kr;
+    kr.load(a, valid, kx, ky, p);
     {
         TL in;
         in.fetch(a, col0, t);
         in.put(tile, t);
     }
     __syncthreads();
-    ztile_convolve<Real, NZ, TW>(a, tile, col0, t, S::ON && TW == kZTile ? kstage : nullptr);
+    ztile_convolve<Real, NZ, TW>(a, tile, j, p, ky, kr);
     __syncthreads();
     ztile_store<Real, NZ, TW>(a, tile, col0, t);
 }
