@@ -213,8 +213,8 @@ int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen
                                {"y_radix", &T::y_radix, {16, 8, 4}, true},
                                {"y_lpw", &T::y_lpw, {0, 1, 2}, true},
                                {"fixed", &T::fixed, {1, 0}, false},
-                               {"y_pipe", &T::y_pipe, {4, 0, 1, 2, 8}, false},
-                               {"y_pipe_f", &T::y_pipe_f, {2, 1, 4, 8}, false},
+                               {"y_pipe", &T::y_pipe, {1, 0, 2, 4, 8}, false},
+                               {"y_pipe_f", &T::y_pipe_f, {1, 2, 4, 8}, false},
                                {"z_quad", &T::z_quad, {0, 1}, false},
                                {"xcd", &T::xcd, {1, 0}, false},
                                {"zc", &T::zc, {0, 1, 2, 4}, false}};

@@ -37,8 +37,8 @@ struct DemagTuning {
     int fixed = 1;   // 0: runtime-planned line kernels even for fixed schedules
     int z_quad = -1;  // 1: four lanes per z column instead of two (even nz); -1: FP64 only
     int xcd = 1;     // 0: no per-XCD renumbering of workgroups
-    int y_pipe = 4;    // line groups per workgroup of the pipelined y kernel (0: off)
-    int y_pipe_f = 2;  // > 0: lines per workgroup of the pipelined forward y pass (half-zero input)
+    int y_pipe = 1;    // line groups per workgroup of the y line kernel; > 1 prefetches the next (0: off)
+    int y_pipe_f = 1;  // > 0: the same for the forward y pass (half-zero input)
     int z_tile = 0;    // > 0: at most this many columns per long-z (LDS line) workgroup
     int zc = 0;        // z planes per workgroup of the cell kernels (0: automatic)
     int y_split = 0;   // 1: thin-film y convolution as separate line and tensor passes

@@ -319,30 +319,65 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
     const int hx = a.g.hx;
-    // consecutive lanes read consecutive rows of one kx column; batches of
-    // four loads per lane are in flight before their LDS writes
-    constexpr int U = 4;
-    for (int base = tid; base < hx * LINES; base += U * nthr) {
+    // consecutive lanes read consecutive rows of one kx column
+    auto put = [&](int idx, cplx<Real> A, cplx<Real> B) {
+        const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+        const int jx = L::line_xor(j);
+        lds[j * ls + L::slot(a.lp, k, jx)] = {A.x - B.y, A.y + B.x};
+        if (k > 0 && k < n - k) lds[j * ls + L::slot(a.lp, n - k, jx)] = {A.x + B.y, B.x - A.y};
+    };
+    if constexpr (N > 0) {
+        // fixed length: every load of the lane (hx / T <= U row pairs) in
+        // flight before the LDS writes; a row pair is one 16-byte load in
+        // FP32 when the row pitch is even
+        constexpr int U = (N / 2 + 1 + N / 16 - 1) / (N / 16);
+        const bool pair16 = sizeof(Real) == 4 && (a.g.pitch & 1) == 0;
         cplx<Real> A[U], B[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int idx = base + u * nthr;
+            const int idx = tid + u * nthr;
             const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
             const int ya = y0 + 2 * j;
             A[u] = B[u] = {Real(0), Real(0)};
-            if (idx < hx * LINES) {
-                if (ya < a.rows) A[u] = spec[a.g.at(c, z, k, ya)];
-                if (ya + 1 < a.rows) B[u] = spec[a.g.at(c, z, k, ya + 1)];
+            if (k < hx && ya < a.rows) {
+                const cplx<Real> *src = spec + a.g.at(c, z, k, ya);
+                if (pair16 && ya + 1 < a.rows) {
+                    const float4 q = *reinterpret_cast<const float4 *>(src);
+                    A[u] = {Real(q.x), Real(q.y)};
+                    B[u] = {Real(q.z), Real(q.w)};
+                } else {
+                    A[u] = src[0];
+                    if (ya + 1 < a.rows) B[u] = src[1];
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int idx = base + u * nthr;
-            if (idx >= hx * LINES) break;
-            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
-            const int jx = L::line_xor(j);
-            lds[j * ls + L::slot(a.lp, k, jx)] = {A[u].x - B[u].y, A[u].y + B[u].x};
-            if (k > 0 && k < n - k) lds[j * ls + L::slot(a.lp, n - k, jx)] = {A[u].x + B[u].y, B[u].x - A[u].y};
+            const int idx = tid + u * nthr;
+            if (idx < hx * LINES) put(idx, A[u], B[u]);
+        }
+    } else {
+        // batches of four loads per lane in flight before their LDS writes
+        constexpr int U = 4;
+        for (int base = tid; base < hx * LINES; base += U * nthr) {
+            cplx<Real> A[U], B[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + u * nthr;
+                const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+                const int ya = y0 + 2 * j;
+                A[u] = B[u] = {Real(0), Real(0)};
+                if (idx < hx * LINES) {
+                    if (ya < a.rows) A[u] = spec[a.g.at(c, z, k, ya)];
+                    if (ya + 1 < a.rows) B[u] = spec[a.g.at(c, z, k, ya + 1)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = base + u * nthr;
+                if (idx >= hx * LINES) break;
+                put(idx, A[u], B[u]);
+            }
         }
     }
     __syncthreads();
@@ -398,13 +433,14 @@ __global__ void __launch_bounds__(kMaxBlock) line_kernel(LineArgs<Real> a) {
         for (int e = t; e < a.n_out; e += T) g[e] = line[L::slot(a.lp, e)];
 }
 
-// Software-pipelined contiguous line pass for a fixed power-of-two length N:
-// a workgroup holds G = max(1, 64 / T) lines at a time (T = N / 16 lanes per
-// line) and walks `iters` groups of lines; the global loads of the next group
-// are issued into registers before the current group's LDS transform, so
-// memory traffic overlaps the FFT.  Inputs: N / IN_DIV per line (the rest
-// zero); outputs: a.n_out per line.
-template <typename Real, int N, int IN_DIV>
+// Contiguous line pass for a fixed power-of-two length N: a workgroup holds
+// G = max(1, 64 / T) lines at a time (T = N / 16 lanes per line) and walks
+// `iters` groups of lines.  With PIPE the global loads of the next group are
+// issued into registers before the current group's LDS transform, so memory
+// traffic overlaps the FFT; without it (one group per workgroup) the kernel
+// holds no prefetch registers and more waves share each SIMD instead.
+// Inputs: N / IN_DIV per line (the rest zero); outputs: a.n_out per line.
+template <typename Real, int N, int IN_DIV, bool PIPE>
 __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
     constexpr int T = N / 16;
     constexpr int G = T >= 64 ? 1 : 64 / T;
@@ -415,7 +451,7 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int j = (int)threadIdx.x / T, t = (int)threadIdx.x - j * T;
     cplx<Real> *line = lds + j * (N + kPad);
-    const int64_t first = (int64_t)blockIdx.x * a.iters * G + j;
+    const int64_t first = (int64_t)blockIdx.x * (PIPE ? a.iters : 1) * G + j;
     cplx<Real> buf[NI];
     auto fetch = [&](int64_t l) {
         const bool live = l < a.lines;
@@ -424,14 +460,15 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
         for (int e = 0; e < NI; ++e) buf[e] = live ? g[t + e * T] : cplx<Real>{Real(0), Real(0)};
     };
     fetch(first);
-    for (int it = 0; it < a.iters; ++it) {
+    const int iters = PIPE ? a.iters : 1;
+    for (int it = 0; it < iters; ++it) {
         const int64_t l = first + (int64_t)it * G;
         if constexpr (p2_first_from_regs<N>()) {
             // the loaded elements are the first radix-16 butterfly's inputs
             cplx<Real> v[16];
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = e < NI ? buf[e] : cplx<Real>{Real(0), Real(0)};
-            if (it + 1 < a.iters) fetch(l + G);
+            if (PIPE && it + 1 < iters) fetch(l + G);
             p2_first_pass_regs<Real, N>(line, t, v, a.tw, a.inverse != 0, 0);
             p2_passes<Real, N, 1, 16>(line, t, a.tw, a.inverse != 0, 0);
         } else {
@@ -441,14 +478,14 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
 #pragma unroll
                 for (int e = NI; e < N / T; ++e) line[p2slot(t + e * T)] = cplx<Real>{Real(0), Real(0)};
             __syncthreads();
-            if (it + 1 < a.iters) fetch(l + G);
+            if (PIPE && it + 1 < iters) fetch(l + G);
             p2_passes<Real, N, 0, 1>(line, t, a.tw, a.inverse != 0, 0);
         }
         if (l < a.lines) {
             cplx<Real> *g = a.data + l * a.line_pitch;
             for (int e = t; e < a.n_out; e += T) g[e] = line[p2slot(e)];
         }
-        __syncthreads();
+        if (PIPE) __syncthreads();
     }
 }
 
@@ -1170,10 +1207,16 @@ void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) 
                 la.iters = div == 2 && dp.tune.y_pipe_f > 0 ? dp.tune.y_pipe_f : dp.tune.y_pipe;
                 const dim3 grid((unsigned)cdiv(la.lines, (int64_t)G * la.iters)), block((unsigned)(G * T));
                 const size_t lds = (size_t)G * (N + kPad) * ce;
-                if (div == 1)
-                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1>), grid, block, lds, c.stream, la);
-                else
-                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2>), grid, block, lds, c.stream, la);
+                if (la.iters == 1) {
+                    if (div == 1)
+                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, false>), grid, block, lds, c.stream, la);
+                    else
+                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, false>), grid, block, lds, c.stream, la);
+                } else if (div == 1) {
+                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, true>), grid, block, lds, c.stream, la);
+                } else {
+                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, true>), grid, block, lds, c.stream, la);
+                }
                 return;
             }
         }

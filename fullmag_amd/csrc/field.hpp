@@ -358,6 +358,133 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
     block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * (MODE == STAGE_STATS ? kNStat : 1));
 }
 
+// Uniform-material form of stage_kernel: no active mask, no region table,
+// no per-cell Ms, demag and exchange on, at most one earlier slope (the Heun
+// stages and the stats pass; RK4 / RK23 / DP45 / ABM3 stages with more
+// slopes use stage_kernel).  The per-cell arithmetic is stage_cell's, in the
+// same order, so results are bit-identical; what changes is that every load
+// of a plane (the z neighbour ahead, the four x/y neighbours, H_d, the base
+// point, the earlier slope) is issued in one branch-free batch before any
+// arithmetic, where stage_kernel's optional-input branches split a plane's
+// loads into several serial waits.  M0_IS_SRC: the base point is src itself
+// (predictor stages and the stats pass), so it is not loaded again.
+template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
+          bool HAS_OUT>
+__global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, Real, Src, Out> a) {
+    static_assert(MODE != STAGE_ERROR && NPREV <= 1, "plain and stats stages with at most one earlier slope");
+    static_assert(!M0_IS_SRC || sizeof(Src) == sizeof(State), "a base point read from src is a state");
+    const FieldArgs &f = a.f;
+    const int64_t n = f.n;
+    double acc[kNStat];
+#pragma unroll
+    for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
+    const uint32_t tile = f.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int chunk = (int)(tile % (uint32_t)f.nchunks);
+    const uint32_t t2 = tile / (uint32_t)f.nchunks;
+    const int txi = (int)(t2 % (uint32_t)f.tiles_x), tyi = (int)(t2 / (uint32_t)f.tiles_x);
+    const int TX = 1 << f.tx_log2;
+    const int x = txi * TX + (int)(threadIdx.x & (TX - 1));
+    const int y = tyi * (kBlock >> f.tx_log2) + (int)(threadIdx.x >> f.tx_log2);
+    const int z0 = chunk * f.zc, z1 = min(f.nz, z0 + f.zc);
+    if (x < f.nx && y < f.ny && z0 < z1) {
+        const int64_t nxy = (int64_t)f.nx * f.ny;
+        // clamped (Neumann) neighbours as offsets: 0 at the grid edge
+        const int64_t oxm = x > 0 ? -1 : 0, oxp = x < f.nx - 1 ? 1 : 0;
+        const int64_t oym = y > 0 ? -(int64_t)f.nx : 0, oyp = y < f.ny - 1 ? (int64_t)f.nx : 0;
+        const Src *__restrict__ src = a.src;
+        const Real *__restrict__ hd = a.hd;
+        const State *__restrict__ m0p = a.m0;
+        const Real *__restrict__ kp = a.kp[0];
+        int64_t i = ((int64_t)z0 * f.ny + y) * f.nx + x;
+        Src prev[3], cur[3];
+        const int64_t ib = z0 > 0 ? i - nxy : i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cur[c] = src[c * n + i];
+            prev[c] = src[c * n + ib];
+        }
+        const Src ix = Src(f.idx2), iy = Src(f.idy2), iz = Src(f.idz2), pref = Src(f.pref);
+        const Real he[3] = {f.has_ext ? Real(f.hx) : Real(0), f.has_ext ? Real(f.hy) : Real(0),
+                            f.has_ext ? Real(f.hz) : Real(0)};
+        for (int z = z0; z < z1; ++z, i += nxy) {
+            const int64_t ia = z + 1 < f.nz ? i + nxy : i;
+            Src nxt[3], xm[3], xp[3], ym[3], yp[3];
+            Real hdv[3], kpv[3];
+            State m0v[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const Src *sc = src + c * n;
+                nxt[c] = sc[ia];
+                xm[c] = sc[i + oxm];
+                xp[c] = sc[i + oxp];
+                ym[c] = sc[i + oym];
+                yp[c] = sc[i + oyp];
+                hdv[c] = hd[c * n + i];
+                if (HAS_OUT && !M0_IS_SRC) m0v[c] = m0p[c * n + i];
+                if (HAS_OUT && NPREV == 1) kpv[c] = kp[c * n + i];
+            }
+            Real m[3], hex[3], h[3], k[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                m[c] = Real(cur[c]);
+                // exchange_col's arithmetic (z neighbours clamped at the grid edge)
+                const Src cc = cur[c];
+                const Src zmv = z > 0 ? prev[c] : cc, zpv = z < f.nz - 1 ? nxt[c] : cc;
+                const Src lap = (xp[c] - Src(2) * cc + xm[c]) * ix + (yp[c] - Src(2) * cc + ym[c]) * iy +
+                                (zpv - Src(2) * cc + zmv) * iz;
+                hex[c] = Real(pref * lap);
+                h[c] = (hex[c] + hdv[c]) + he[c];
+            }
+            torque(f, m, h, k);
+            if (a.kout) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) a.kout[c * n + i] = k[c];
+            }
+            if constexpr (HAS_OUT) {
+                State v[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    State s = State(a.anew) * State(k[c]);
+                    if constexpr (NPREV == 1) s += State(a.ap[0]) * State(kpv[c]);
+                    const State base = M0_IS_SRC ? State(cur[c]) : m0v[c];
+                    v[c] = base + State(a.dt) * s;
+                }
+                const State nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+                const State inv = nrm > State(0) ? State(1) / nrm : State(0);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) a.out[c * n + i] = Out(v[c] * inv);
+            }
+            if constexpr (MODE == STAGE_STATS) {
+                double md[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) md[c] = double(cur[c]);
+                const double w = 1.0;
+                acc[0] += w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2]));
+                acc[2] += w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2]));
+                if (f.has_ext) acc[4] += w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz);
+                const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
+                const double d2 = double(hdv[0]) * hdv[0] + double(hdv[1]) * hdv[1] + double(hdv[2]) * hdv[2];
+                const double k2 = double(k[0]) * k[0] + double(k[1]) * k[1] + double(k[2]) * k[2];
+                acc[6] = fmax(acc[6], h2);
+                acc[7] = fmax(acc[7], d2);
+                acc[8] = fmax(acc[8], k2);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                prev[c] = cur[c];
+                cur[c] = nxt[c];
+            }
+        }
+    }
+    if constexpr (MODE == STAGE_PLAIN) return;
+    block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * kNStat);
+}
+
+// Whether a stage with these arguments may run as stage_uniform_kernel.
+inline bool stage_uniform_ok(const FieldArgs &f, const void *hd, int nprev) {
+    return !f.mask && !f.region && !f.ms_cell && hd && f.exchange && nprev <= 1;
+}
+
 // Column tiling of the stage kernels: tile width TX = 64 (or the smallest
 // power of two >= nx), TY = kBlock / TX rows; z is split into chunks only
 // when the tiles alone give fewer than about 1024 workgroups (longer chunks

@@ -167,8 +167,24 @@ struct Driver {
         a.anew = anew;
         a.dt = dt;
         Timed t(c, "llg_stage");
-        hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), dim3(grid_stage()), dim3(kBlock), 0,
-                           c.stream, a);
+        const dim3 g(grid_stage()), b(kBlock);
+        if (out && stage_uniform_ok(a.f, a.hd, a.nprev)) {
+            constexpr bool src_is_state = sizeof(Src) == sizeof(State);
+            const bool m0_is_src = src_is_state && static_cast<const void *>(src) == static_cast<const void *>(a.m0);
+            if (a.nprev == 1) {
+                hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 1, false, true>), g, b,
+                                   0, c.stream, a);
+            } else if (m0_is_src) {
+                hipLaunchKernelGGL(
+                    (stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 0, src_is_state, true>), g, b, 0,
+                    c.stream, a);
+            } else {
+                hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 0, false, true>), g, b,
+                                   0, c.stream, a);
+            }
+            return;
+        }
+        hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), g, b, 0, c.stream, a);
     }
 
     template <typename Out>
@@ -233,8 +249,17 @@ struct Driver {
         const unsigned g = grid_reduce();
         {
             Timed t(c, "llg_stats");
-            hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS>), dim3(g), dim3(kBlock), 0,
-                               c.stream, a);
+            if (stage_uniform_ok(a.f, a.hd, 0)) {
+                if (out)
+                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, true>),
+                                       dim3(g), dim3(kBlock), 0, c.stream, a);
+                else
+                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false>),
+                                       dim3(g), dim3(kBlock), 0, c.stream, a);
+            } else {
+                hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS>), dim3(g), dim3(kBlock), 0,
+                                   c.stream, a);
+            }
         }
         hipLaunchKernelGGL((finalize_kernel<STAGE_STATS>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
                            c.d_result);

@@ -313,7 +313,7 @@ def test_demag_field_long_z(fa, oracle_mod, monkeypatch, cells, tune):
 # schedule must give the oracle's field and trajectory.
 SCHEDULES = ["", "fixed=0", "z_quad=1", "z_quad=0", "x_rows=4,y_lpw=1", "xcd=0", "x_radix=4,y_radix=8",
              "x_lanes=128,y_lanes=64", "y_pipe=0", "y_pipe=3", "zc=1", "zc=3,xcd=0", "x_rows=4", "x_rows=2",
-             "y_pipe_f=1", "y_pipe_f=4,y_pipe=1", "x_radix=8", "y_radix=4,fixed=0"]
+             "y_pipe_f=1", "y_pipe_f=4,y_pipe=1", "y_pipe=4,y_pipe_f=2", "x_radix=8", "y_radix=4,fixed=0"]
 
 
 @pytest.mark.parametrize("cells", [(32, 16, 4), (64, 32, 8), (16, 16, 6)])
