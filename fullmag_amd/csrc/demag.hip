@@ -169,6 +169,12 @@ struct MagSource {
         const double w = ms_cell ? ms_cell[i] : ms;
         return Real(double(m[(int64_t)c * n + i]) * w);
     }
+    // No mask and no per-cell Ms: the value is one unconditional load, so a
+    // lane's loads can all be issued before any of them is used.
+    __device__ bool uniform() const { return !mask && !ms_cell; }
+    __device__ Real uniform_at(int c, int z, int y, int x) const {
+        return Real(double(m[(int64_t)c * n + ((int64_t)z * ny + y) * nx + x]) * ms);
+    }
 };
 
 // Padded real-space Newell kernel at (c, z, y, x) of the (px, py, pz) grid,
@@ -178,6 +184,8 @@ struct MagSource {
 struct NewellSource {
     const double *oct;  // [6][nz][ny][nx]
     int nx, ny, nz, px, py, pz;
+    __device__ bool uniform() const { return false; }
+    __device__ double uniform_at(int, int, int, int) const { return 0.0; }
     __device__ double operator()(int c, int zz, int yy, int x) const {
         if (x == nx || yy == ny || (pz > 1 && zz == nz)) return 0.0;
         const int xi = x < nx ? x : px - x;
@@ -227,13 +235,31 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
         if constexpr (N > 0) {
             // fixed length: all 16 loads of a lane in flight before the LDS writes
             cplx<Real> v[16];
+            if (src.uniform() && 2 * a.n_in <= N) {
+                // branch-free: clamped addresses, every load issued before
+                // any is used; elements t + e N/16 with e >= 8 lie in the
+                // zero padding (n_in <= N/2) and are not loaded
+                const bool oka = ya < a.rows, okb = ya + 1 < a.rows;
+                const int yA = oka ? ya : 0, yB = okb ? ya + 1 : 0;
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int x = t + e * (N / 16);
-                v[e] = {Real(0), Real(0)};
-                if (x < a.n_in) {
-                    if (ya < a.rows) v[e].x = src(c, z, ya, x);
-                    if (ya + 1 < a.rows) v[e].y = src(c, z, ya + 1, x);
+                for (int e = 0; e < 8; ++e) {
+                    const int x = t + e * (N / 16);
+                    const bool okx = x < a.n_in;
+                    const int xs = okx ? x : 0;
+                    const Real va = src.uniform_at(c, z, yA, xs), vb = src.uniform_at(c, z, yB, xs);
+                    v[e] = {(okx && oka) ? va : Real(0), (okx && okb) ? vb : Real(0)};
+                }
+#pragma unroll
+                for (int e = 8; e < 16; ++e) v[e] = {Real(0), Real(0)};
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int x = t + e * (N / 16);
+                    v[e] = {Real(0), Real(0)};
+                    if (x < a.n_in) {
+                        if (ya < a.rows) v[e].x = src(c, z, ya, x);
+                        if (ya + 1 < a.rows) v[e].y = src(c, z, ya + 1, x);
+                    }
                 }
             }
             if constexpr (p2_first_from_regs<N>()) {
@@ -333,22 +359,30 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
         constexpr int U = (N / 2 + 1 + N / 16 - 1) / (N / 16);
         const bool pair16 = sizeof(Real) == 4 && (a.g.pitch & 1) == 0;
         cplx<Real> A[U], B[U];
+        // branch-free: clamped addresses, selects after the loads
+        if (pair16) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int idx = tid + u * nthr;
-            const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
-            const int ya = y0 + 2 * j;
-            A[u] = B[u] = {Real(0), Real(0)};
-            if (k < hx && ya < a.rows) {
-                const cplx<Real> *src = spec + a.g.at(c, z, k, ya);
-                if (pair16 && ya + 1 < a.rows) {
-                    const float4 q = *reinterpret_cast<const float4 *>(src);
-                    A[u] = {Real(q.x), Real(q.y)};
-                    B[u] = {Real(q.z), Real(q.w)};
-                } else {
-                    A[u] = src[0];
-                    if (ya + 1 < a.rows) B[u] = src[1];
-                }
+            for (int u = 0; u < U; ++u) {
+                const int idx = tid + u * nthr;
+                const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+                const int ya = y0 + 2 * j;
+                const bool oka = k < hx && ya < a.rows, okb = oka && ya + 1 < a.rows;
+                // an even row pitch and an even ya keep the pair 16-byte aligned
+                const float4 q = *reinterpret_cast<const float4 *>(spec + (oka ? a.g.at(c, z, k, ya) : 0));
+                A[u] = oka ? cplx<Real>{Real(q.x), Real(q.y)} : cplx<Real>{Real(0), Real(0)};
+                B[u] = okb ? cplx<Real>{Real(q.z), Real(q.w)} : cplx<Real>{Real(0), Real(0)};
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int idx = tid + u * nthr;
+                const int j = idx & (LINES - 1), k = idx >> (a.xr_log2 - 1);
+                const int ya = y0 + 2 * j;
+                const bool oka = k < hx && ya < a.rows, okb = oka && ya + 1 < a.rows;
+                const cplx<Real> qa = spec[oka ? a.g.at(c, z, k, ya) : 0];
+                const cplx<Real> qb = spec[okb ? a.g.at(c, z, k, ya + 1) : 0];
+                A[u] = oka ? qa : cplx<Real>{Real(0), Real(0)};
+                B[u] = okb ? qb : cplx<Real>{Real(0), Real(0)};
             }
         }
 #pragma unroll
