@@ -488,10 +488,13 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
     const int64_t first = (int64_t)blockIdx.x * (PIPE ? a.iters : 1) * G + j;
     cplx<Real> buf[NI];
     auto fetch = [&](int64_t l) {
-        const bool live = l < a.lines;
-        const cplx<Real> *g = a.data + (live ? l : 0) * a.line_pitch;
+        // a line past the end loads line 0 instead: its transform is never
+        // stored, and unconditional loads issue together (a load guarded by
+        // a select becomes a branch, and the join waits for the loads
+        // issued before it)
+        const cplx<Real> *g = a.data + (l < a.lines ? l : 0) * a.line_pitch;
 #pragma unroll
-        for (int e = 0; e < NI; ++e) buf[e] = live ? g[t + e * T] : cplx<Real>{Real(0), Real(0)};
+        for (int e = 0; e < NI; ++e) buf[e] = g[t + e * T];
     };
     fetch(first);
     const int iters = PIPE ? a.iters : 1;

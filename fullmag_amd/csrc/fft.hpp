@@ -173,11 +173,14 @@ template <typename T, int R>
 __device__ __forceinline__ void twiddle(cplx<T> *v, const cplx<T> *tw, int st, bool inv) {
     if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
         constexpr int LB = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+        // conjugation for the inverse as a product with -1 (exact), so the
+        // table loads stay out of any branch
+        const T sg = inv ? T(-1) : T(1);
         cplx<T> pw[LB];
 #pragma unroll
         for (int b = 0; b < LB; ++b) {
             pw[b] = tw[st << b];
-            if (inv) pw[b].y = -pw[b].y;
+            pw[b].y *= sg;
         }
         cplx<T> w[R];
 #pragma unroll
@@ -188,10 +191,11 @@ __device__ __forceinline__ void twiddle(cplx<T> *v, const cplx<T> *tw, int st, b
             v[r] = cmul(v[r], w[r]);
         }
     } else {
+        const T sg = inv ? T(-1) : T(1);
 #pragma unroll
         for (int r = 1; r < R; ++r) {
             cplx<T> w = tw[r * st];
-            if (inv) w.y = -w.y;
+            w.y *= sg;
             v[r] = cmul(v[r], w);
         }
     }
@@ -362,7 +366,9 @@ __device__ __forceinline__ void stockham_pass_p2(cplx<T> *line, int t, const cpl
         const int k = j % NS;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[b][r] = line[p2slot(j + r * NR) ^ lx];
-        if (NS > 1 && k != 0) twiddle<T, R>(v[b], tw, k * ST, inv);
+        // k = 0 multiplies by the table's exact w^0 = 1 (no branch: lanes of
+        // one wave differ in k, and a branch would split the table loads)
+        if (NS > 1) twiddle<T, R>(v[b], tw, k * ST, inv);
         dft_any<T, R>(v[b], tw, N, inv);
         base[b] = (j / NS) * NS * R + k;
     }
