@@ -733,6 +733,27 @@ struct ZTile {
     C v[PER][V];
 
     __device__ __forceinline__ void fetch(const ZArgs<Real> &a, int64_t col0, int t) {
+        if (col0 + TW <= a.columns) {
+            // the whole tile lies inside the spectrum (every tile but possibly
+            // the last): unconditional 16-byte loads, all issued before any
+            // wait; a lane's unit past the tile re-reads the tile's last unit
+            // and put() drops it
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int idx = min(t + 256 * i, ROWS * UNITS - 1);
+                const int r = idx / UNITS, u = idx - r * UNITS;
+                const int c = r / NZ, z = r - c * NZ;
+                const C *src = a.data + c * a.comp_stride + z * a.zstride + col0 + (int64_t)u * V;
+                if constexpr (V == 2) {
+                    const float4 q = *reinterpret_cast<const float4 *>(src);
+                    v[i][0] = C{Real(q.x), Real(q.y)};
+                    v[i][1] = C{Real(q.z), Real(q.w)};
+                } else {
+                    v[i][0] = *src;
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = t + 256 * i;
