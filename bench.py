@@ -452,7 +452,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run one 256 MiB device-to-device copy kernel (reference for HBM counters)")
+                    help="first run two device-to-device copies, 256 MiB and 1 GiB (references for the HBM "
+                         "counters; the second exceeds the 256 MB Infinity Cache)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this "
@@ -465,11 +466,12 @@ def main():
 
     torch.cuda.init()
     if args.copy_calibration:
-        src = torch.ones(64 * 2**20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        for mib in (256, 1024):
+            src = torch.ones(mib * 2**18, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
     if args.workload != "film512":
         {"sweep96": sweep96, "dp45-block": dp45_block, "adjoint-film": adjoint_film}[args.workload](fa, torch, args)
         return
