@@ -177,7 +177,12 @@ SIGNATURES = {
     "fullmag_fdm_ext_autotune": (_i32, [ctypes.c_void_p, _i32, ctypes.c_char_p, _u64]),
     "fullmag_fdm_ext_set_schedule": (_i32, [ctypes.c_void_p, ctypes.c_char_p]),
     "fullmag_fdm_ext_set_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
+    "fullmag_fdm_ext_copy_field_preview_noalloc": (_i32, [ctypes.c_void_p, _i32, _u32, _u32, _u32, _u32, _u32,
+                                                          _i32, ctypes.c_void_p, _u64]),
 }
+
+# fullmag_fdm_ext.h: FULLMAG_FDM_EXT_NOALLOC_SCRATCH_BYTES
+NOALLOC_SCRATCH_BYTES = 4194304
 
 _lib = None
 

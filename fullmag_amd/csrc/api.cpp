@@ -11,6 +11,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <optional>
 
@@ -317,6 +318,54 @@ int copy_field(fullmag_fdm_backend *h, fullmag_fdm_observable obs, T *out, uint6
     return download_soa<T>(c, buf, elem, out) ? FULLMAG_FDM_OK : FULLMAG_FDM_ERR_CUDA;
 }
 
+// Box-average preview of the device field `buf` staged through `stage`
+// (3 pnx pny pnz elements of T), then copied to the host.
+template <typename T>
+int preview_staged(Context &c, const void *buf, size_t elem, uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0,
+                   uint32_t zs, void *stage, T *out, uint64_t len) {
+    if (!preview_field(c, buf, elem == 8, pnx, pny, pnz, z0, zs, stage, sizeof(T) == 8)) return FULLMAG_FDM_ERR_CUDA;
+    if (!hip_ok(hipMemcpyAsync(out, stage, len * sizeof(T), hipMemcpyDeviceToHost, c.stream), "hipMemcpy(preview)",
+                c.last_error) ||
+        !hip_ok(hipStreamSynchronize(c.stream), "hipMemcpy(preview)", c.last_error))
+        return FULLMAG_FDM_ERR_CUDA;
+    return FULLMAG_FDM_OK;
+}
+
+// The external field's box average, computed on the host (context.cu:935-989:
+// the uniform field scaled by the active fraction of each bin).
+template <typename T>
+void preview_h_ext(const Context &c, uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, T *out) {
+    for (uint32_t bz = 0; bz < pnz; ++bz) {
+        uint32_t za = z0 + bz * zs;
+        if (za >= c.nz) za = c.nz - 1;
+        uint32_t zb = z0 + (bz + 1) * zs;
+        if (zb <= za) zb = za + 1;
+        if (zb > c.nz) zb = c.nz;
+        for (uint32_t by = 0; by < pny; ++by) {
+            uint32_t y0 = (uint32_t)((uint64_t)by * c.ny / pny), y1 = (uint32_t)((uint64_t)(by + 1) * c.ny / pny);
+            if (y1 <= y0) y1 = y0 + 1;
+            if (y1 > c.ny) y1 = c.ny;
+            for (uint32_t bx = 0; bx < pnx; ++bx) {
+                uint32_t x0 = (uint32_t)((uint64_t)bx * c.nx / pnx),
+                         x1 = (uint32_t)((uint64_t)(bx + 1) * c.nx / pnx);
+                if (x1 <= x0) x1 = x0 + 1;
+                if (x1 > c.nx) x1 = c.nx;
+                double act = 0, cnt = 0;
+                for (uint32_t z = za; z < zb; ++z)
+                    for (uint32_t y = y0; y < y1; ++y)
+                        for (uint32_t x = x0; x < x1; ++x) {
+                            const uint64_t i = ((uint64_t)z * c.ny + y) * c.nx + x;
+                            act += (!c.has_mask || c.mask_host[i]) ? 1.0 : 0.0;
+                            cnt += 1.0;
+                        }
+                const uint64_t pi = ((uint64_t)bz * pny + by) * pnx + bx;
+                const double s = (c.has_ext && cnt > 0) ? act / cnt : 0.0;
+                for (int k = 0; k < 3; ++k) out[pi * 3 + k] = (T)(c.hext[k] * s);
+            }
+        }
+    }
+}
+
 template <typename T>
 int copy_preview(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny, uint32_t pnz,
                  uint32_t z0, uint32_t zs, T *out, uint64_t len) {
@@ -331,36 +380,7 @@ int copy_preview(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pn
     if (z0 >= c.nz) return FULLMAG_FDM_ERR_CUDA;  // reference returns false -> ERR_CUDA
     if (pnx == c.nx && pny == c.ny && pnz == c.nz && z0 == 0 && zs == 1) return copy_field<T>(h, obs, out, len);
     if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
-        // context.cu:935-989: external field scaled by the active fraction of each bin
-        for (uint32_t bz = 0; bz < pnz; ++bz) {
-            uint32_t za = z0 + bz * zs;
-            if (za >= c.nz) za = c.nz - 1;
-            uint32_t zb = z0 + (bz + 1) * zs;
-            if (zb <= za) zb = za + 1;
-            if (zb > c.nz) zb = c.nz;
-            for (uint32_t by = 0; by < pny; ++by) {
-                uint32_t y0 = (uint32_t)((uint64_t)by * c.ny / pny), y1 = (uint32_t)((uint64_t)(by + 1) * c.ny / pny);
-                if (y1 <= y0) y1 = y0 + 1;
-                if (y1 > c.ny) y1 = c.ny;
-                for (uint32_t bx = 0; bx < pnx; ++bx) {
-                    uint32_t x0 = (uint32_t)((uint64_t)bx * c.nx / pnx),
-                             x1 = (uint32_t)((uint64_t)(bx + 1) * c.nx / pnx);
-                    if (x1 <= x0) x1 = x0 + 1;
-                    if (x1 > c.nx) x1 = c.nx;
-                    double act = 0, cnt = 0;
-                    for (uint32_t z = za; z < zb; ++z)
-                        for (uint32_t y = y0; y < y1; ++y)
-                            for (uint32_t x = x0; x < x1; ++x) {
-                                const uint64_t i = ((uint64_t)z * c.ny + y) * c.nx + x;
-                                act += (!c.has_mask || c.mask_host[i]) ? 1.0 : 0.0;
-                                cnt += 1.0;
-                            }
-                    const uint64_t pi = ((uint64_t)bz * pny + by) * pnx + bx;
-                    const double s = (c.has_ext && cnt > 0) ? act / cnt : 0.0;
-                    for (int k = 0; k < 3; ++k) out[pi * 3 + k] = (T)(c.hext[k] * s);
-                }
-            }
-        }
+        preview_h_ext<T>(c, pnx, pny, pnz, z0, zs, out);
         return FULLMAG_FDM_OK;
     }
     const void *buf = nullptr;
@@ -375,13 +395,7 @@ int copy_preview(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pn
         if (!ctx_alloc(c, &c.d_preview, bytes, "preview scratch")) return FULLMAG_FDM_ERR_CUDA;
         c.preview_bytes = bytes;
     }
-    if (!preview_field(c, buf, elem == 8, pnx, pny, pnz, z0, zs, c.d_preview, sizeof(T) == 8))
-        return FULLMAG_FDM_ERR_CUDA;
-    if (!hip_ok(hipMemcpyAsync(out, c.d_preview, bytes, hipMemcpyDeviceToHost, c.stream), "hipMemcpy(preview)",
-                c.last_error) ||
-        !hip_ok(hipStreamSynchronize(c.stream), "hipMemcpy(preview)", c.last_error))
-        return FULLMAG_FDM_ERR_CUDA;
-    return FULLMAG_FDM_OK;
+    return preview_staged<T>(c, buf, elem, pnx, pny, pnz, z0, zs, c.d_preview, out, len);
 }
 
 struct Snapshot {
@@ -426,6 +440,88 @@ int upload_impl(fullmag_fdm_backend *h, const void *m, uint64_t len, bool f64) {
     c.abm_startup = 0;
     c.abm_last_dt = 0.0;
     return FULLMAG_FDM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free preview: staging in one fixed device array per GPU.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocScratchBytes = FULLMAG_FDM_EXT_NOALLOC_SCRATCH_BYTES;
+static_assert(kNoallocScratchBytes % sizeof(double) == 0, "whole doubles");
+
+// Each GPU that loads the library holds one copy until the process ends.  In
+// host code this name is only a handle for hipGetSymbolAddress: the device
+// address of a GPU's copy is `noalloc_scratch(device)`.
+static __device__ __align__(256) double scratch[kNoallocScratchBytes / sizeof(double)];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device`, resolved once per device with that
+// device current (the caller's current device is restored).
+void *noalloc_scratch(int device, std::string &err) {
+    if (device < 0 || device >= kMaxScratchDevices) {
+        err = "no-alloc preview: device index out of range";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int prev = -1;
+        if (!hip_ok(hipGetDevice(&prev), "hipGetDevice", err)) return nullptr;
+        if (prev != device && !hip_ok(hipSetDevice(device), "hipSetDevice", err)) return nullptr;
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (prev != device) hipSetDevice(prev);
+        if (!hip_ok(e, "hipGetSymbolAddress(scratch)", err)) return nullptr;
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+// copy_preview without any device allocation: the preview kernel writes into
+// the GPU's `scratch` (every element of the output, so nothing is cleared),
+// also for a full-resolution request.  H_ex / H_eff need the observable
+// buffers that the first copy of either allocates.
+template <typename T>
+int copy_preview_noalloc(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uint32_t pnx, uint32_t pny,
+                         uint32_t pnz, uint32_t z0, uint32_t zs, T *out, uint64_t len) {
+    if (!h || !out || pnx == 0 || pny == 0 || pnz == 0 || zs == 0) return FULLMAG_FDM_ERR_INVALID;
+    Context &c = *as_ctx(h);
+    if (!c.ready) return FULLMAG_FDM_ERR_INVALID;
+    const uint64_t count = (uint64_t)pnx * pny * pnz;
+    if (len != 3 * count) {
+        c.last_error = "preview out_len mismatch";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    if (len > kNoallocScratchBytes / sizeof(T)) {
+        c.last_error = "no-alloc preview: 3 * preview cells * element size exceeds the " +
+                       std::to_string(kNoallocScratchBytes) + "-byte scratch array";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    if (z0 >= c.nz) return FULLMAG_FDM_ERR_CUDA;  // as copy_preview
+    if (obs == FULLMAG_FDM_OBSERVABLE_H_EXT) {
+        preview_h_ext<T>(c, pnx, pny, pnz, z0, zs, out);
+        return FULLMAG_FDM_OK;
+    }
+    if ((obs == FULLMAG_FDM_OBSERVABLE_H_EX || obs == FULLMAG_FDM_OBSERVABLE_H_EFF) &&
+        (!c.h_ex_obs || !c.h_eff_obs)) {
+        c.last_error = "no-alloc preview: the H_ex / H_eff observable buffers are not allocated yet (one "
+                       "fullmag_fdm_backend_copy_field or copy_field_preview of H_ex or H_eff allocates them)";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    const void *buf = nullptr;
+    size_t elem = 8;
+    if (!field_source(c, obs, &buf, &elem)) {
+        if (c.last_error.empty()) c.last_error = "unknown observable";
+        return FULLMAG_FDM_ERR_CUDA;
+    }
+    if (!buf) {
+        std::fill(out, out + len, T(0));
+        return FULLMAG_FDM_OK;
+    }
+    void *stage = noalloc_scratch(c.device, c.last_error);
+    if (!stage) return FULLMAG_FDM_ERR_CUDA;
+    return preview_staged<T>(c, buf, elem, pnx, pny, pnz, z0, zs, stage, out, len);
 }
 
 }  // namespace
@@ -508,6 +604,17 @@ int fullmag_fdm_backend_copy_field_preview_f32(fullmag_fdm_backend *handle, full
                                                uint64_t out_len) {
     return copy_preview<float>(handle, observable, preview_nx, preview_ny, preview_nz, z_origin, z_stride, out_xyz,
                                out_len);
+}
+
+int fullmag_fdm_ext_copy_field_preview_noalloc(fullmag_fdm_backend *handle, fullmag_fdm_observable observable,
+                                               uint32_t preview_nx, uint32_t preview_ny, uint32_t preview_nz,
+                                               uint32_t z_origin, uint32_t z_stride, int out_f64, void *out_xyz,
+                                               uint64_t out_len) {
+    if (out_f64)
+        return copy_preview_noalloc<double>(handle, observable, preview_nx, preview_ny, preview_nz, z_origin,
+                                            z_stride, static_cast<double *>(out_xyz), out_len);
+    return copy_preview_noalloc<float>(handle, observable, preview_nx, preview_ny, preview_nz, z_origin, z_stride,
+                                       static_cast<float *>(out_xyz), out_len);
 }
 
 fullmag_fdm_field_snapshot *fullmag_fdm_backend_begin_field_snapshot(fullmag_fdm_backend *handle,

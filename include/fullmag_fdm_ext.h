@@ -127,6 +127,28 @@ int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedu
 /* Bytes of device memory the backend holds. */
 uint64_t fullmag_fdm_ext_device_bytes(fullmag_fdm_backend *handle);
 
+/* Size of the fixed device array (one per GPU, held until the process ends)
+ * through which fullmag_fdm_ext_copy_field_preview_noalloc stages its output. */
+#define FULLMAG_FDM_EXT_NOALLOC_SCRATCH_BYTES 4194304
+
+/* fullmag_fdm_backend_copy_field_preview_f64 (out_f64 != 0) or _f32 (out_f64
+ * == 0) without any device allocation: the box averages are staged in the
+ * library's fixed per-GPU array instead of a buffer it allocates, with the
+ * same results.  No hipMalloc, hipFree or allocating library routine is called
+ * on any path.  Limits: 3 * preview_nx * preview_ny * preview_nz * element
+ * size must not exceed FULLMAG_FDM_EXT_NOALLOC_SCRATCH_BYTES (ERR_INVALID with
+ * last_error otherwise, also for a full-resolution request); H_ex and H_eff
+ * need the observable buffers that the first copy_field / copy_field_preview
+ * of either allocates (ERR_INVALID before that).  All calls on one GPU share
+ * its array and every call overwrites the part it uses: calls that overlap on
+ * one GPU (from several threads or backends) race.  No counterpart in the
+ * reference ABI; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving or UI loops, harnesses that forbid it). */
+int fullmag_fdm_ext_copy_field_preview_noalloc(fullmag_fdm_backend *handle, fullmag_fdm_observable observable,
+                                               uint32_t preview_nx, uint32_t preview_ny, uint32_t preview_nz,
+                                               uint32_t z_origin, uint32_t z_stride, int out_f64, void *out_xyz,
+                                               uint64_t out_len);
+
 #ifdef __cplusplus
 }
 #endif
