@@ -28,6 +28,16 @@ namespace fm {
 Context *backend_context(fullmag_fdm_backend *h);
 }
 
+// Members whose recorded Heun step has the same launches (kernels, launch
+// geometry, argument sizes) form a class: each pass of the step is then one
+// launch over all of them, instance blockIdx.z / zdiv reading its arguments
+// from a device table.
+struct BatchClass {
+    std::vector<size_t> members;           // indices into fullmag_fdm_batch::members
+    std::vector<fm::LaunchRec> proto;      // the launches of one instance
+    std::vector<size_t> table_off;         // byte offset of each launch's table
+};
+
 struct fullmag_fdm_batch {
     std::vector<fm::Context *> members;
     std::vector<double> dts;
@@ -37,6 +47,10 @@ struct fullmag_fdm_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<hipEvent_t> events;
     std::vector<std::array<double, 4>> ext;  // has_ext, H_ext at capture
+    std::vector<BatchClass> classes;         // multi-instance launch groups
+    std::vector<size_t> singles;             // members stepped by their own launches
+    void *d_tables = nullptr;                // argument tables of all classes
+    size_t table_bytes = 0;
     std::string last_error;
 };
 
@@ -51,6 +65,96 @@ std::array<double, 4> ext_of(const fm::Context &c) {
     return {c.has_ext ? 1.0 : 0.0, c.hext[0], c.hext[1], c.hext[2]};
 }
 
+bool same_launches(const std::vector<fm::LaunchRec> &a, const std::vector<fm::LaunchRec> &b) {
+    if (a.size() != b.size()) return false;
+    for (size_t i = 0; i < a.size(); ++i) {
+        const fm::LaunchRec &x = a[i], &y = b[i];
+        if (x.multi != y.multi || x.lds != y.lds || x.zdiv != y.zdiv || x.pack.size() != y.pack.size() ||
+            x.grid.x != y.grid.x || x.grid.y != y.grid.y || x.grid.z != y.grid.z || x.block.x != y.block.x ||
+            x.block.y != y.block.y || x.block.z != y.block.z)
+            return false;
+    }
+    return true;
+}
+
+// Records one Heun step of every member, groups members with identical
+// launches into classes (every launch having a multi-instance twin), and
+// uploads the classes' argument tables.  The others step on their own.
+bool plan_classes(fullmag_fdm_batch *b) {
+    std::string &err = b->last_error;
+    b->classes.clear();
+    b->singles.clear();
+    std::vector<std::vector<fm::LaunchRec>> recs(b->members.size());
+    for (size_t i = 0; i < b->members.size(); ++i) {
+        fm::Context &c = *b->members[i];
+        fm::LaunchRecorder r;
+        c.rec = &r;
+        fm::integrator_enqueue_heun(c, b->dts[i], 1);
+        c.rec = nullptr;
+        recs[i] = std::move(r.recs);
+        bool batchable = !recs[i].empty();
+        for (const fm::LaunchRec &l : recs[i]) batchable = batchable && l.multi;
+        if (!batchable) {
+            b->singles.push_back(i);
+            continue;
+        }
+        BatchClass *home = nullptr;
+        for (BatchClass &k : b->classes)
+            if (same_launches(k.proto, recs[i])) home = &k;
+        if (!home) {
+            b->classes.push_back(BatchClass{});
+            home = &b->classes.back();
+            home->proto = recs[i];
+        }
+        home->members.push_back(i);
+    }
+    // one table per launch position of each class: the members' packs back
+    // to back at the pack's own stride, each table 256-byte aligned
+    size_t total = 0;
+    for (BatchClass &k : b->classes) {
+        k.table_off.clear();
+        for (const fm::LaunchRec &l : k.proto) {
+            k.table_off.push_back(total);
+            total += (l.pack.size() * k.members.size() + 255) / 256 * 256;
+        }
+    }
+    if (total == 0) return true;
+    if (b->table_bytes < total) {
+        if (b->d_tables) hipFree(b->d_tables);
+        b->d_tables = nullptr;
+        b->table_bytes = 0;
+        if (!fm::hip_ok(hipMalloc(&b->d_tables, total), "hipMalloc(batch tables)", err)) return false;
+        b->table_bytes = total;
+    }
+    std::vector<unsigned char> host(total, 0);
+    for (BatchClass &k : b->classes)
+        for (size_t p = 0; p < k.proto.size(); ++p)
+            for (size_t m = 0; m < k.members.size(); ++m) {
+                const std::vector<unsigned char> &pk = recs[k.members[m]][p].pack;
+                std::memcpy(host.data() + k.table_off[p] + m * pk.size(), pk.data(), pk.size());
+            }
+    return fm::hip_ok(hipMemcpy(b->d_tables, host.data(), total, hipMemcpyHostToDevice), "hipMemcpy(batch tables)",
+                      err);
+}
+
+// Enqueues `steps` Heun steps of class k on `stream`: one launch per pass for
+// all of its members.
+bool enqueue_class(fullmag_fdm_batch *b, const BatchClass &k, hipStream_t stream, int steps) {
+    const int m = (int)k.members.size();
+    for (int s = 0; s < steps; ++s)
+        for (size_t p = 0; p < k.proto.size(); ++p) {
+            const fm::LaunchRec &l = k.proto[p];
+            const void *tab = static_cast<const unsigned char *>(b->d_tables) + k.table_off[p];
+            int zdiv = l.zdiv;
+            void *args[] = {&tab, &zdiv};
+            const dim3 grid(l.grid.x, l.grid.y, l.grid.z * (unsigned)m);
+            if (!fm::hip_ok(hipLaunchKernel(l.multi, grid, l.block, args, l.lds, stream), "hipLaunchKernel(batch)",
+                            b->last_error))
+                return false;
+        }
+    return true;
+}
+
 bool capture(fullmag_fdm_batch *b) {
     using fm::hip_ok;
     std::string &err = b->last_error;
@@ -60,11 +164,24 @@ bool capture(fullmag_fdm_batch *b) {
     b->graph = nullptr;
     b->ext.clear();
     for (fm::Context *c : b->members) b->ext.push_back(ext_of(*c));
+    if (!plan_classes(b)) return false;
     if (!hip_ok(hipStreamBeginCapture(b->origin, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture", err))
         return false;
     hipEvent_t fork = b->events[0];
     bool ok = hip_ok(hipEventRecord(fork, b->origin), "hipEventRecord(fork)", err);
-    for (size_t i = 0; ok && i < b->members.size(); ++i) {
+    // a class runs on its first member's stream and joins through that
+    // member's event
+    for (size_t ci = 0; ok && ci < b->classes.size(); ++ci) {
+        const BatchClass &k = b->classes[ci];
+        const size_t lead = k.members[0];
+        fm::Context &c = *b->members[lead];
+        ok = hip_ok(hipStreamWaitEvent(c.stream, fork, 0), "hipStreamWaitEvent(class)", err) &&
+             enqueue_class(b, k, c.stream, b->k) &&
+             hip_ok(hipEventRecord(b->events[1 + lead], c.stream), "hipEventRecord(join)", err) &&
+             hip_ok(hipStreamWaitEvent(b->origin, b->events[1 + lead], 0), "hipStreamWaitEvent(origin)", err);
+    }
+    for (size_t si = 0; ok && si < b->singles.size(); ++si) {
+        const size_t i = b->singles[si];
         fm::Context &c = *b->members[i];
         ok = hip_ok(hipStreamWaitEvent(c.stream, fork, 0), "hipStreamWaitEvent(member)", err);
         if (!ok) break;
@@ -187,6 +304,7 @@ void fullmag_fdm_ext_batch_destroy(fullmag_fdm_batch *b) {
     for (auto e : b->events)
         if (e) hipEventDestroy(e);
     if (b->origin) hipStreamDestroy(b->origin);
+    if (b->d_tables) hipFree(b->d_tables);
     delete b;
 }
 

@@ -18,6 +18,7 @@
 #include "fullmag_fdm.h"
 #include "fullmag_fdm_ext.h"
 
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -76,6 +77,21 @@ struct DemagPlan {
     // tensor layout: one block of kblock reals per (kx, oy); the even-oz
     // entries first, the odd-oz ones from kodd (see kern_at)
     int kblock = 0, kodd = 0;
+};
+
+// One kernel launch as recorded for a multi-instance batch: the kernel's
+// multi-instance twin (null when the launch has none), the launch geometry
+// of one instance and its argument pack.  The twin has the signature
+// (const Pack *table, int zdiv) and serves instance blockIdx.z / zdiv.
+struct LaunchRec {
+    const void *multi = nullptr;
+    dim3 grid, block;
+    size_t lds = 0;
+    int zdiv = 1;
+    std::vector<unsigned char> pack;
+};
+struct LaunchRecorder {
+    std::vector<LaunchRec> recs;
 };
 
 struct TimingEntry {
@@ -159,6 +175,10 @@ struct Context {
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> event_pool;
 
+    // non-null: kernel launches are recorded here instead of being issued
+    // (multi-instance batch set-up)
+    LaunchRecorder *rec = nullptr;
+
     std::string last_error;
     fullmag_fdm_device_info info{};
     bool info_valid = false;
@@ -169,6 +189,26 @@ struct Context {
     size_t state_bytes() const { return mode == MODE_SINGLE ? 4 : 8; }
     size_t real_bytes() const { return mode == MODE_DOUBLE ? 8 : 4; }
 };
+
+// Issues a launch through `plain`, or, while c.rec is set, records it with
+// its multi-instance twin `multi` (null: none) and argument pack instead.
+template <class Pack, class F>
+inline void launch_rec(Context &c, const void *multi, dim3 grid, dim3 block, size_t lds, int zdiv, const Pack &pack,
+                       F &&plain) {
+    if (c.rec) {
+        LaunchRec r;
+        r.multi = multi;
+        r.grid = grid;
+        r.block = block;
+        r.lds = lds;
+        r.zdiv = zdiv;
+        r.pack.resize(sizeof(Pack));
+        std::memcpy(r.pack.data(), &pack, sizeof(Pack));
+        c.rec->recs.push_back(std::move(r));
+        return;
+    }
+    plain();
+}
 
 // context.cpp
 bool ctx_alloc(Context &c, void **ptr, size_t bytes, const char *what);

@@ -217,15 +217,15 @@ struct FieldSink {
 // (c = blockIdx.z, z = blockIdx.y).  Rows 2j, 2j+1 are packed as a + i b;
 // with Z = FFT(a + i b):  A_k = (Z_k + conj Z_{N-k}) / 2,
 // B_k = (Z_k - conj Z_{N-k}) / (2i), k = 0..N/2, stored at spec[c][z][k][y].
-template <typename Real, class Src, int N = 0>
-__global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
+template <typename Real, class Src, int N>
+__device__ __forceinline__ void xfwd_body(const XArgs<Real> &a, const Src &src, cplx<Real> *spec, int c) {
     using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int LINES = a.xr >> 1;
     const int n = L::n(a.lp), ls = a.ls, T = L::threads(a.lp);
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
+    const int y0 = blockIdx.x * a.xr, z = blockIdx.y;
     // the T lanes of packed line j (= tid / T, blockDim.x = LINES * T) load it
     const int jl = tid / T, t = tid - jl * T;
     cplx<Real> *line = lds + jl * ls;
@@ -332,18 +332,42 @@ __global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src,
     }
 }
 
+// Component c = blockIdx.z.
+template <typename Real, class Src, int N = 0>
+__global__ void __launch_bounds__(kMaxBlock) xfwd_kernel(XArgs<Real> a, Src src, cplx<Real> *spec) {
+    xfwd_body<Real, Src, N>(a, src, spec, (int)blockIdx.z);
+}
+
+// Multi-instance form: instance blockIdx.z / zdiv of the table, component
+// blockIdx.z % zdiv.  Each multi-instance kernel copies its instance's
+// arguments into a local before the body runs, so the body sees invariant
+// values exactly as the plain kernel does (a reference into the table would be
+// re-read after the body's stores, which changes how the compiler contracts
+// the arithmetic, and with it the last bit of FP32 results).
+template <typename Real, class Src>
+struct XFwdPack {
+    XArgs<Real> a;
+    Src src;
+    cplx<Real> *spec;
+};
+template <typename Real, class Src, int N = 0>
+__global__ void __launch_bounds__(kMaxBlock) xfwd_multi(const XFwdPack<Real, Src> *tab, int zdiv) {
+    const XFwdPack<Real, Src> p = tab[blockIdx.z / zdiv];
+    xfwd_body<Real, Src, N>(p.a, p.src, p.spec, (int)(blockIdx.z % zdiv));
+}
+
 // Inverse complex-to-real along x: Z_k = A_k + i B_k for k <= N/2 and the
 // Hermitian continuation A_{N-k} = conj A_k; z = IFFT(Z); rows a = Re z,
 // b = Im z.  Reads spec[c][z][k][y0..y0+xr-1].
-template <typename Real, class Sink, int N = 0>
-__global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
+template <typename Real, class Sink, int N>
+__device__ __forceinline__ void xinv_body(const XArgs<Real> &a, const cplx<Real> *spec, const Sink &sink, int c) {
     using L = LineOps<N>;
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int LINES = a.xr >> 1;
     const int n = L::n(a.lp), ls = a.ls, T = L::threads(a.lp);
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int y0 = blockIdx.x * a.xr, z = blockIdx.y, c = blockIdx.z;
+    const int y0 = blockIdx.x * a.xr, z = blockIdx.y;
     const int hx = a.g.hx;
     // consecutive lanes read consecutive rows of one kx column
     auto put = [&](int idx, cplx<Real> A, cplx<Real> B) {
@@ -428,6 +452,23 @@ __global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cp
     }
 }
 
+template <typename Real, class Sink, int N = 0>
+__global__ void __launch_bounds__(kMaxBlock) xinv_kernel(XArgs<Real> a, const cplx<Real> *spec, Sink sink) {
+    xinv_body<Real, Sink, N>(a, spec, sink, (int)blockIdx.z);
+}
+
+template <typename Real, class Sink>
+struct XInvPack {
+    XArgs<Real> a;
+    const cplx<Real> *spec;
+    Sink sink;
+};
+template <typename Real, class Sink, int N = 0>
+__global__ void __launch_bounds__(kMaxBlock) xinv_multi(const XInvPack<Real, Sink> *tab, int zdiv) {
+    const XInvPack<Real, Sink> p = tab[blockIdx.z / zdiv];
+    xinv_body<Real, Sink, N>(p.a, p.spec, p.sink, (int)(blockIdx.z % zdiv));
+}
+
 // ---------------------------------------------------------------------------
 // Contiguous line passes (y; and the Newell setup's z when long)
 // ---------------------------------------------------------------------------
@@ -475,7 +516,7 @@ __global__ void __launch_bounds__(kMaxBlock) line_kernel(LineArgs<Real> a) {
 // holds no prefetch registers and more waves share each SIMD instead.
 // Inputs: N / IN_DIV per line (the rest zero); outputs: a.n_out per line.
 template <typename Real, int N, int IN_DIV, bool PIPE>
-__global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
+__device__ __forceinline__ void line_pipe_body(const LineArgs<Real> &a) {
     constexpr int T = N / 16;
     constexpr int G = T >= 64 ? 1 : 64 / T;
     constexpr int NIN = N / IN_DIV;
@@ -526,6 +567,16 @@ __global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
     }
 }
 
+template <typename Real, int N, int IN_DIV, bool PIPE>
+__global__ void __launch_bounds__(256) line_pipe_kernel(LineArgs<Real> a) {
+    line_pipe_body<Real, N, IN_DIV, PIPE>(a);
+}
+template <typename Real, int N, int IN_DIV, bool PIPE>
+__global__ void __launch_bounds__(256) line_pipe_multi(const LineArgs<Real> *tab, int zdiv) {
+    const LineArgs<Real> a = tab[blockIdx.z / zdiv];
+    line_pipe_body<Real, N, IN_DIV, PIPE>(a);
+}
+
 // Thin film: per kx, the three components' ky lines (ny of 2ny inputs) are
 // transformed, multiplied by the tensor at kz = 0, transformed back and the
 // first ny outputs stored.
@@ -541,8 +592,8 @@ struct YConvArgs {
 
 // MAXB: kMaxBlock, or 1024 for lines too long for three of them in
 // kMaxBlock lanes.
-template <typename Real, int MAXB>
-__global__ void __launch_bounds__(MAXB) yconv_kernel(YConvArgs<Real> a) {
+template <typename Real>
+__device__ __forceinline__ void yconv_body(const YConvArgs<Real> &a) {
     extern __shared__ unsigned char smem_raw[];
     cplx<Real> *lds = reinterpret_cast<cplx<Real> *>(smem_raw);
     const int n = a.lp.n, ls = n + kPad, T = a.lp.threads;
@@ -574,6 +625,16 @@ __global__ void __launch_bounds__(MAXB) yconv_kernel(YConvArgs<Real> a) {
         const int c = s / a.kpw, kx = kx0 + (s - c * a.kpw);
         if (kx < a.hx) a.data[a.g.at(c, 0, kx, e)] = lds[s * ls + lslot(a.lp, e)];
     }
+}
+
+template <typename Real, int MAXB>
+__global__ void __launch_bounds__(MAXB) yconv_kernel(YConvArgs<Real> a) {
+    yconv_body<Real>(a);
+}
+template <typename Real, int MAXB>
+__global__ void __launch_bounds__(MAXB) yconv_multi(const YConvArgs<Real> *tab, int zdiv) {
+    const YConvArgs<Real> a = tab[blockIdx.z / zdiv];
+    yconv_body<Real>(a);
 }
 
 // Thin film, split form: the three components' ky spectra (all py entries,
@@ -919,8 +980,8 @@ __device__ __forceinline__ void ztile_convolve(const ZArgs<Real> &a, cplx<Real> 
     }
 }
 
-template <typename Real, int NZ, int TW = kZTile>
-__global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+template <typename Real, int NZ, int TW>
+__device__ __forceinline__ void zconv_tile_body(const ZArgs<Real> &a) {
     static_assert(TW == 128, "one column per lane pair of a 256-lane workgroup");
     using TL = ZTile<Real, NZ, TW>;
     __shared__ cplx<Real> tile[TL::ROWS][TW + TL::V];
@@ -946,6 +1007,16 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
     ztile_store<Real, NZ, TW>(a, tile, col0, t);
 }
 
+template <typename Real, int NZ, int TW = kZTile>
+__global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
+    zconv_tile_body<Real, NZ, TW>(a);
+}
+template <typename Real, int NZ, int TW = kZTile>
+__global__ void __launch_bounds__(256) zconv_tile_multi(const ZArgs<Real> *tab, int zdiv) {
+    const ZArgs<Real> a = tab[blockIdx.z / zdiv];
+    zconv_tile_body<Real, NZ, TW>(a);
+}
+
 // z convolution with four lanes per (kx, ky) column (pz = 2 NZ = 4 M, NZ
 // even): lane q owns kz = 4k + q.  Forward: X[4k+q] = DFT_M(u)[k] with the
 // folded input u_z = x_z w^(q z) + x_(z+M) w^(q (z+M)), z < M; tensor at its
@@ -954,7 +1025,7 @@ __global__ void __launch_bounds__(256) zconv_tile_kernel(ZArgs<Real> a) {
 // renumbered per XCD so the columns ky and py - ky of one kx row, which read
 // the same tensor octant entries, share an L2.
 template <typename Real, int NZ>
-__global__ void __launch_bounds__(256) zconv_quad_kernel(ZArgs<Real> a) {
+__device__ __forceinline__ void zconv_quad_body(const ZArgs<Real> &a) {
     static_assert(NZ % 2 == 0, "four lanes per column need an even nz");
     constexpr int M = NZ / 2, PZ = 2 * NZ;
     const uint32_t blk = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -1007,6 +1078,16 @@ __global__ void __launch_bounds__(256) zconv_quad_kernel(ZArgs<Real> a) {
             for (int z = 0; z < NZ; ++z)
                 if ((z & 3) == q) a.data[c * a.comp_stride + z * a.zstride + col] = y[z];
     }
+}
+
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_quad_kernel(ZArgs<Real> a) {
+    zconv_quad_body<Real, NZ>(a);
+}
+template <typename Real, int NZ>
+__global__ void __launch_bounds__(256) zconv_quad_multi(const ZArgs<Real> *tab, int zdiv) {
+    const ZArgs<Real> a = tab[blockIdx.z / zdiv];
+    zconv_quad_body<Real, NZ>(a);
 }
 
 // Long z lines: a workgroup holds `tile` consecutive columns (contiguous in
@@ -1267,20 +1348,33 @@ void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) 
                 const size_t lds = (size_t)G * (N + kPad) * ce;
                 if (la.iters == 1) {
                     if (div == 1)
-                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, false>), grid, block, lds, c.stream, la);
+                        launch_rec(c, reinterpret_cast<const void *>(&line_pipe_multi<Real, N, 1, false>), grid, block,
+                                   lds, 1, la, [&] {
+                                       hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, false>), grid, block, lds,
+                                                          c.stream, la);
+                                   });
                     else
-                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, false>), grid, block, lds, c.stream, la);
+                        launch_rec(c, reinterpret_cast<const void *>(&line_pipe_multi<Real, N, 2, false>), grid, block,
+                                   lds, 1, la, [&] {
+                                       hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, false>), grid, block, lds,
+                                                          c.stream, la);
+                                   });
                 } else if (div == 1) {
-                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, true>), grid, block, lds, c.stream, la);
+                    launch_rec(c, nullptr, grid, block, lds, 1, la, [&] {
+                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 1, true>), grid, block, lds, c.stream, la);
+                    });
                 } else {
-                    hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, true>), grid, block, lds, c.stream, la);
+                    launch_rec(c, nullptr, grid, block, lds, 1, la, [&] {
+                        hipLaunchKernelGGL((line_pipe_kernel<Real, N, 2, true>), grid, block, lds, c.stream, la);
+                    });
                 }
                 return;
             }
         }
         const dim3 grid((unsigned)cdiv(la.lines, la.lpw)), block((unsigned)(la.lpw * lp.threads));
         const size_t lds = (size_t)la.lpw * (lp.n + kPad) * ce;
-        hipLaunchKernelGGL((line_kernel<Real, N>), grid, block, lds, c.stream, la);
+        launch_rec(c, nullptr, grid, block, lds, 1, la,
+                   [&] { hipLaunchKernelGGL((line_kernel<Real, N>), grid, block, lds, c.stream, la); });
     });
 }
 
@@ -1310,8 +1404,12 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         Timed t(c, "demag_x_r2c");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, N>), xgrid, xblock, xlds, c.stream, xa, ms,
-                               spec);
+            const XFwdPack<Real, MagSource<Real, S>> pk{xa, ms, spec};
+            launch_rec(c, N > 0 ? reinterpret_cast<const void *>(&xfwd_multi<Real, MagSource<Real, S>, N>) : nullptr,
+                       xgrid, xblock, xlds, 3, pk, [&] {
+                           hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, N>), xgrid, xblock, xlds,
+                                              c.stream, xa, ms, spec);
+                       });
         });
     }
     const KernelView<Real> K = kview<Real>(c);
@@ -1364,19 +1462,29 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 if (staged) {
                     const dim3 zt((unsigned)cdiv(za.columns, kZTile));
                     by_nz([&](auto NZC) {
-                        hipLaunchKernelGGL((zconv_tile_kernel<Real, decltype(NZC)::value>), zt, zb, 0, c.stream, za);
+                        constexpr int NZ = decltype(NZC)::value;
+                        launch_rec(c, reinterpret_cast<const void *>(&zconv_tile_multi<Real, NZ>), zt, zb, 0, 1, za,
+                                   [&] { hipLaunchKernelGGL((zconv_tile_kernel<Real, NZ>), zt, zb, 0, c.stream, za); });
                     });
                 } else if (quad) {
                     const dim3 zq((unsigned)cdiv(4 * za.columns, 256));
+                    auto quad_launch = [&](auto NZC) {
+                        constexpr int NZ = decltype(NZC)::value;
+                        launch_rec(c, reinterpret_cast<const void *>(&zconv_quad_multi<Real, NZ>), zq, zb, 0, 1, za,
+                                   [&] { hipLaunchKernelGGL((zconv_quad_kernel<Real, NZ>), zq, zb, 0, c.stream, za); });
+                    };
                     switch (dp.nz) {
-                        case 4: hipLaunchKernelGGL((zconv_quad_kernel<Real, 4>), zq, zb, 0, c.stream, za); break;
-                        case 6: hipLaunchKernelGGL((zconv_quad_kernel<Real, 6>), zq, zb, 0, c.stream, za); break;
-                        default: hipLaunchKernelGGL((zconv_quad_kernel<Real, 8>), zq, zb, 0, c.stream, za); break;
+                        case 4: quad_launch(std::integral_constant<int, 4>{}); break;
+                        case 6: quad_launch(std::integral_constant<int, 6>{}); break;
+                        default: quad_launch(std::integral_constant<int, 8>{}); break;
                     }
                 } else {
                     const dim3 zg((unsigned)cdiv(2 * za.columns, 256));
                     by_nz([&](auto NZC) {
-                        hipLaunchKernelGGL((zconv_pair_kernel<Real, decltype(NZC)::value>), zg, zb, 0, c.stream, za);
+                        launch_rec(c, nullptr, zg, zb, 0, 1, za, [&] {
+                            hipLaunchKernelGGL((zconv_pair_kernel<Real, decltype(NZC)::value>), zg, zb, 0, c.stream,
+                                               za);
+                        });
                     });
                 }
             } else {
@@ -1395,10 +1503,11 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                 za.pitch = dp.rows;
                 za.K = K;
                 with_line(dp.lz, dp.tune.fixed != 0, [&](auto NC) {
-                    hipLaunchKernelGGL((zlds_kernel<Real, decltype(NC)::value>),
-                                       dim3((unsigned)cdiv(za.columns, za.tile)),
-                                       dim3((unsigned)(3 * za.tile * dp.lz.threads)),
-                                       (size_t)3 * za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+                    const dim3 g1((unsigned)cdiv(za.columns, za.tile)), b1((unsigned)(3 * za.tile * dp.lz.threads));
+                    const size_t l1 = (size_t)3 * za.tile * (dp.lz.n + kPad) * ce;
+                    launch_rec(c, nullptr, g1, b1, l1, 1, za, [&] {
+                        hipLaunchKernelGGL((zlds_kernel<Real, decltype(NC)::value>), g1, b1, l1, c.stream, za);
+                    });
                 });
             }
         }
@@ -1426,8 +1535,11 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         }
         {
             Timed t(c, "demag_y_conv");
-            hipLaunchKernelGGL((film_tensor_kernel<Real>), dim3((unsigned)cdiv((int64_t)dp.hx * dp.py, 256)),
-                               dim3(256), 0, c.stream, spec, K, dp.hx, dp.py, (int64_t)dp.rows);
+            const dim3 g1((unsigned)cdiv((int64_t)dp.hx * dp.py, 256));
+            launch_rec(c, nullptr, g1, dim3(256), 0, 1, K, [&] {
+                hipLaunchKernelGGL((film_tensor_kernel<Real>), g1, dim3(256), 0, c.stream, spec, K, dp.hx, dp.py,
+                                   (int64_t)dp.rows);
+            });
         }
         {
             la.n_in = dp.py;
@@ -1450,17 +1562,23 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         const dim3 grid((unsigned)cdiv(dp.hx, ya.kpw)), block((unsigned)(3 * ya.kpw * dp.ly.threads));
         const size_t lds = (size_t)3 * ya.kpw * (dp.ly.n + kPad) * ce;
         if (block.x > (unsigned)kMaxBlock)
-            hipLaunchKernelGGL((yconv_kernel<Real, 1024>), grid, block, lds, c.stream, ya);
+            launch_rec(c, reinterpret_cast<const void *>(&yconv_multi<Real, 1024>), grid, block, lds, 1, ya,
+                       [&] { hipLaunchKernelGGL((yconv_kernel<Real, 1024>), grid, block, lds, c.stream, ya); });
         else
-            hipLaunchKernelGGL((yconv_kernel<Real, kMaxBlock>), grid, block, lds, c.stream, ya);
+            launch_rec(c, reinterpret_cast<const void *>(&yconv_multi<Real, kMaxBlock>), grid, block, lds, 1, ya,
+                       [&] { hipLaunchKernelGGL((yconv_kernel<Real, kMaxBlock>), grid, block, lds, c.stream, ya); });
     }
     {
         FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
         Timed t(c, "demag_x_c2r");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream, xa, spec,
-                               sink);
+            const XInvPack<Real, FieldSink<Real>> pk{xa, spec, sink};
+            launch_rec(c, N > 0 ? reinterpret_cast<const void *>(&xinv_multi<Real, FieldSink<Real>, N>) : nullptr,
+                       xgrid, xblock, xlds, 3, pk, [&] {
+                           hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream,
+                                              xa, spec, sink);
+                       });
         });
     }
 }

@@ -370,7 +370,7 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
 // (predictor stages and the stats pass), so it is not loaded again.
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
           bool HAS_OUT>
-__global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, Real, Src, Out> a) {
+__device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, Src, Out> &a) {
     static_assert(MODE != STAGE_ERROR && NPREV <= 1, "plain and stats stages with at most one earlier slope");
     static_assert(!M0_IS_SRC || sizeof(Src) == sizeof(State), "a base point read from src is a state");
     const FieldArgs &f = a.f;
@@ -478,6 +478,19 @@ __global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, 
     }
     if constexpr (MODE == STAGE_PLAIN) return;
     block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * kNStat);
+}
+
+template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
+          bool HAS_OUT>
+__global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, Real, Src, Out> a) {
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT>(a);
+}
+// Multi-instance form: instance blockIdx.z / zdiv of the table.
+template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
+          bool HAS_OUT>
+__global__ void __launch_bounds__(kBlock) stage_uniform_multi(const StageArgs<State, Real, Src, Out> *tab, int zdiv) {
+    const StageArgs<State, Real, Src, Out> a = tab[blockIdx.z / zdiv];
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT>(a);
 }
 
 // Whether a stage with these arguments may run as stage_uniform_kernel.

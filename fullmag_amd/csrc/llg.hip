@@ -168,23 +168,27 @@ struct Driver {
         a.dt = dt;
         Timed t(c, "llg_stage");
         const dim3 g(grid_stage()), b(kBlock);
+        // the uniform kernel (or its multi-instance twin while a batch records)
+        auto uniform = [&](auto np, auto m0src) {
+            constexpr int NP = decltype(np)::value;
+            constexpr bool MS = decltype(m0src)::value;
+            launch_rec(c, reinterpret_cast<const void *>(&stage_uniform_multi<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true>),
+                       g, b, 0, 1, a, [&] {
+                           hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true>),
+                                              g, b, 0, c.stream, a);
+                       });
+        };
         if (out && stage_uniform_ok(a.f, a.hd, a.nprev)) {
             constexpr bool src_is_state = sizeof(Src) == sizeof(State);
             const bool m0_is_src = src_is_state && static_cast<const void *>(src) == static_cast<const void *>(a.m0);
-            if (a.nprev == 1) {
-                hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 1, false, true>), g, b,
-                                   0, c.stream, a);
-            } else if (m0_is_src) {
-                hipLaunchKernelGGL(
-                    (stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 0, src_is_state, true>), g, b, 0,
-                    c.stream, a);
-            } else {
-                hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, 0, false, true>), g, b,
-                                   0, c.stream, a);
-            }
+            if (a.nprev == 1) uniform(std::integral_constant<int, 1>{}, std::false_type{});
+            else if (m0_is_src) uniform(std::integral_constant<int, 0>{}, std::integral_constant<bool, src_is_state>{});
+            else uniform(std::integral_constant<int, 0>{}, std::false_type{});
             return;
         }
-        hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), g, b, 0, c.stream, a);
+        launch_rec(c, nullptr, g, b, 0, 1, a, [&] {
+            hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), g, b, 0, c.stream, a);
+        });
     }
 
     template <typename Out>

@@ -130,3 +130,49 @@ def test_schedule_replan_knobs_match_oracle(fa, oracle_mod):
     b.refresh_observables()
     np.testing.assert_allclose(b.copy_h_demag(), hd, rtol=0, atol=1e-11 * np.abs(hd).max(), err_msg=chosen)
     b.close()
+
+
+def test_batch_classes_of_equal_grids_match_eager(fa):
+    """Members sharing a grid, precision and schedule run each pass as one
+    multi-instance launch (arguments per member from a device table): three
+    40x24x2 mixed members with different Ms, alpha, dt and fields, two
+    32x32x1 FP64 films, two 16x16x8 mixed blocks, and a masked 40x24x2
+    member that keeps its own launches.  Every member ends bit-identical to
+    eager stepping."""
+    rng = np.random.default_rng(9)
+
+    def make(cells, prec, seed, dt_scale, mask=False):
+        n = cells[0] * cells[1] * cells[2]
+        kw = {}
+        if mask:
+            am = np.ones(n, np.uint8)
+            am[::7] = 0
+            kw["active_mask"] = am
+        plan = fa.FdmPlan(cells=cells, cell_size=(5e-9, 4e-9, 3e-9),
+                          initial_magnetization=fa.generate_random_unit_vectors(seed, n),
+                          saturation_magnetisation=float(rng.uniform(4e5, 1.4e6)),
+                          exchange_stiffness=float(rng.uniform(5e-12, 2e-11)), damping=float(rng.uniform(0.01, 0.5)),
+                          external_field=tuple(rng.uniform(-2e4, 2e4, 3)), precision=prec, demag_kernel="device",
+                          **kw)
+        return plan, 1e-13 * dt_scale
+
+    specs = [((40, 24, 2), "mixed", 1.0, False), ((40, 24, 2), "mixed", 0.7, False), ((40, 24, 2), "mixed", 0.4, False),
+             ((32, 32, 1), "double", 1.0, False), ((32, 32, 1), "double", 0.5, False),
+             ((16, 16, 8), "mixed", 0.8, False), ((16, 16, 8), "mixed", 1.0, False), ((40, 24, 2), "mixed", 1.0, True)]
+    plans = [make(c, p, 100 + i, s, m) for i, (c, p, s, m) in enumerate(specs)]
+    members = [fa.NativeFdmBackend(p) for p, _ in plans]
+    eager = [fa.NativeFdmBackend(p) for p, _ in plans]
+    dts = [dt for _, dt in plans]
+    run = fa.BatchRun(members, dts, steps_per_graph=3)
+    total = 0
+    for n_steps in (7, 6):
+        stats = run.run(n_steps)
+        total += n_steps
+        for b, e, dt, s in zip(members, eager, dts, stats):
+            for _ in range(n_steps):
+                r = e.step(dt)
+            np.testing.assert_array_equal(b.copy_m(), e.copy_m())
+            assert s.step == total and s.e_total == r.e_total
+    run.close()
+    for b in members + eager:
+        b.close()
