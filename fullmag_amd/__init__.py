@@ -8,6 +8,7 @@ ABI (crates/fullmag-runner/src/native_fdm.rs).
 
 from .native_fdm import (  # noqa: F401
     BatchRun,
+    BoundaryGeometry,
     FdmPlan,
     FieldSnapshot,
     NativeFdmBackend,

@@ -33,6 +33,10 @@ INTEGRATOR_ABM3 = 3
 INTEGRATOR_RK4 = 4
 INTEGRATOR_RK23 = 5
 
+BOUNDARY_NONE = 0
+BOUNDARY_VOLUME = 1
+BOUNDARY_FULL = 2
+
 OBSERVABLE_M = 1
 OBSERVABLE_H_EX = 2
 OBSERVABLE_H_DEMAG = 3

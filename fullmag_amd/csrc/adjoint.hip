@@ -363,6 +363,10 @@ int heun_adjoint(Context &c, double dt, const double *m0, const double *lam1, do
         c.last_error = "the Heun adjoint needs an FP64 state (precision double or mixed)";
         return FULLMAG_FDM_ERR_INVALID;
     }
+    if (c.tier) {
+        c.last_error = "the Heun adjoint does not cover the sub-cell boundary tiers";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
     if (c.mode == MODE_DOUBLE) return heun_adjoint_t<double>(c, dt, m0, lam1, lam0, grad_ms, grad_hext);
     return heun_adjoint_t<float>(c, dt, m0, lam1, lam0, grad_ms, grad_hext);
 }

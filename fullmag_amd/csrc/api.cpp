@@ -140,10 +140,56 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
                             std::to_string(p.region_mask_len);
             return c;
         }
-        if (p.boundary_correction != FULLMAG_FDM_BOUNDARY_NONE && p.volume_fraction &&
-            p.volume_fraction_len == n) {
-            c->last_error = "sub-cell boundary correction tiers are not supported by this backend";
-            return c;
+        // sub-cell boundary tiers (reference api.cpp:246-290, context.cu:627-730):
+        // the tier is the plan value as an 8-bit number; volume fractions of
+        // the wrong length switch the tiers off; face links (tier 1) and
+        // distances (tier >= 2) are used only when all six arrays are given.
+        // Single precision ignores the tiers, as the reference's FP32 kernels do.
+        const uint8_t tier = (uint8_t)p.boundary_correction;
+        std::vector<int32_t> corr_cell;
+        std::vector<uint32_t> corr_off, corr_row;
+        if (tier != 0 && p.volume_fraction && p.volume_fraction_len == n && prec != MODE_SINGLE) {
+            c->tier = tier;
+            c->phi_floor = p.boundary_phi_floor > 0.0 ? p.boundary_phi_floor : 0.05;
+            c->delta_min =
+                p.boundary_delta_min > 0.0 ? p.boundary_delta_min : 0.1 * std::min({c->dx, c->dy, c->dz});
+            const bool faces = p.face_link_xp && p.face_link_xm && p.face_link_yp && p.face_link_ym &&
+                               p.face_link_zp && p.face_link_zm;
+            const bool deltas = p.delta_xp && p.delta_xm && p.delta_yp && p.delta_ym && p.delta_zp && p.delta_zm;
+            c->tier_ex = (tier == 1 && faces) ? 1 : (tier >= 2 && deltas) ? 2 : 0;
+            if (p.has_demag_boundary_corr && p.demag_corr_target_idx && p.demag_corr_target_count > 0 &&
+                p.demag_corr_stencil_size > 0 && p.demag_corr_source_idx && p.demag_corr_tensor) {
+                const uint64_t rows = p.demag_corr_target_count, st = p.demag_corr_stencil_size;
+                // an index past the grid would be an out-of-bounds access
+                for (uint64_t t = 0; t < rows; ++t)
+                    if (p.demag_corr_target_idx[t] >= 0 && (uint64_t)p.demag_corr_target_idx[t] >= n) {
+                        c->last_error = "demag_corr_target_idx[" + std::to_string(t) + "] = " +
+                                        std::to_string(p.demag_corr_target_idx[t]) + " is not a cell index";
+                        return c;
+                    }
+                for (uint64_t r = 0; r < rows * st; ++r)
+                    if (p.demag_corr_source_idx[r] >= 0 && (uint64_t)p.demag_corr_source_idx[r] >= n) {
+                        c->last_error = "demag_corr_source_idx[" + std::to_string(r) + "] = " +
+                                        std::to_string(p.demag_corr_source_idx[r]) + " is not a cell index";
+                        return c;
+                    }
+                // rows grouped by target cell, plan order within a cell
+                for (uint64_t t = 0; t < rows; ++t)
+                    if (p.demag_corr_target_idx[t] >= 0) corr_row.push_back((uint32_t)t);
+                std::stable_sort(corr_row.begin(), corr_row.end(), [&](uint32_t a, uint32_t b) {
+                    return p.demag_corr_target_idx[a] < p.demag_corr_target_idx[b];
+                });
+                for (size_t e = 0; e < corr_row.size(); ++e) {
+                    const int32_t cell = p.demag_corr_target_idx[corr_row[e]];
+                    if (e == 0 || cell != corr_cell.back()) {
+                        corr_cell.push_back(cell);
+                        corr_off.push_back((uint32_t)e);
+                    }
+                }
+                corr_off.push_back((uint32_t)corr_row.size());
+                c->corr_groups = (uint32_t)corr_cell.size();
+                c->corr_stencil = (uint32_t)st;
+            }
         }
         const uint64_t len3 = 2ull * (2ull * c->nx) * (2ull * c->ny) * (2ull * c->nz);
         const uint64_t len2 = 2ull * (2ull * c->nx) * (2ull * c->ny);
@@ -219,6 +265,44 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
             if (!ctx_alloc(*c, (void **)&c->d_ms_cell, n * 8, "per-cell Ms")) return c;
             if (!ctx_upload(*c, c->d_ms_cell, opt->ms_cell, n * 8, "hipMemcpy(Ms)", c->last_error))
                 return c;
+        }
+        if (c->tier) {
+            if (!ctx_alloc(*c, (void **)&c->d_phi, n * 8, "volume fraction") ||
+                !ctx_upload(*c, c->d_phi, p.volume_fraction, n * 8, "hipMemcpy(volume fraction)", c->last_error))
+                return c;
+            if (c->tier_ex) {
+                const double *w[6] = {p.face_link_xp, p.face_link_xm, p.face_link_yp,
+                                      p.face_link_ym, p.face_link_zp, p.face_link_zm};
+                if (c->tier_ex == 2) {
+                    const double *dl[6] = {p.delta_xp, p.delta_xm, p.delta_yp, p.delta_ym, p.delta_zp, p.delta_zm};
+                    std::copy(dl, dl + 6, w);
+                }
+                if (!ctx_alloc(*c, (void **)&c->d_tier_w, 6 * n * 8, "boundary face links / distances")) return c;
+                for (int k = 0; k < 6; ++k)
+                    if (!ctx_upload(*c, c->d_tier_w + k * n, w[k], n * 8, "hipMemcpy(boundary arrays)",
+                                    c->last_error))
+                        return c;
+            }
+            if (c->corr_groups > 0) {
+                const uint64_t rows = p.demag_corr_target_count, st = c->corr_stencil;
+                if (!ctx_alloc(*c, (void **)&c->d_corr_cell, corr_cell.size() * 4, "demag correction cells") ||
+                    !ctx_alloc(*c, (void **)&c->d_corr_off, corr_off.size() * 4, "demag correction offsets") ||
+                    !ctx_alloc(*c, (void **)&c->d_corr_row, corr_row.size() * 4, "demag correction rows") ||
+                    !ctx_alloc(*c, (void **)&c->d_corr_src, rows * st * 4, "demag correction sources") ||
+                    !ctx_alloc(*c, (void **)&c->d_corr_n, rows * st * 6 * 8, "demag correction tensors"))
+                    return c;
+                if (!ctx_upload(*c, c->d_corr_cell, corr_cell.data(), corr_cell.size() * 4, "hipMemcpy(corr)",
+                                c->last_error) ||
+                    !ctx_upload(*c, c->d_corr_off, corr_off.data(), corr_off.size() * 4, "hipMemcpy(corr)",
+                                c->last_error) ||
+                    !ctx_upload(*c, c->d_corr_row, corr_row.data(), corr_row.size() * 4, "hipMemcpy(corr)",
+                                c->last_error) ||
+                    !ctx_upload(*c, c->d_corr_src, p.demag_corr_source_idx, rows * st * 4, "hipMemcpy(corr)",
+                                c->last_error) ||
+                    !ctx_upload(*c, c->d_corr_n, p.demag_corr_tensor, rows * st * 6 * 8, "hipMemcpy(corr)",
+                                c->last_error))
+                    return c;
+            }
         }
         if (!alloc_integrator_buffers(*c)) return c;
         const bool newell_dev = opt && opt->newell_on_device;

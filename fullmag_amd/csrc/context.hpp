@@ -126,6 +126,21 @@ struct Context {
     bool has_ms_cell = false;
     double *d_ms_cell = nullptr;
 
+    // sub-cell boundary tiers (double and mixed precision; single precision
+    // ignores them as the reference's FP32 kernels do)
+    int tier = 0;               // plan boundary_correction as an 8-bit value; 0: none
+    int tier_ex = 0;            // exchange stencil: 0 standard, 1 T0 face links, 2 T1 distances
+    double phi_floor = 0.05, delta_min = 0.0;
+    double *d_phi = nullptr;    // volume fraction; non-null: demag packs phi Ms m
+    double *d_tier_w = nullptr; // 6 n face links (T0) or distances (T1): xp, xm, yp, ym, zp, zm
+    // sparse demag correction, rows grouped by target cell: group g adds rows
+    // d_corr_row[d_corr_off[g] .. d_corr_off[g + 1]) (plan order) to cell d_corr_cell[g]
+    uint32_t corr_groups = 0, corr_stencil = 0;
+    int32_t *d_corr_cell = nullptr;
+    uint32_t *d_corr_off = nullptr, *d_corr_row = nullptr;
+    int32_t *d_corr_src = nullptr;  // rows x stencil source cells (< 0: unused)
+    double *d_corr_n = nullptr;     // rows x stencil x 6: Nxx, Nxy, Nxz, Nyy, Nyz, Nzz
+
     double max_err = 1e-5, dt_min = 1e-18, dt_max = 1e-10, headroom = 0.8;
 
     // time stepping
@@ -225,7 +240,9 @@ void timing_resolve(Context &c);
 bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_device);
 // Evaluates H_demag of the field `src` (3n elements, FP64 when src_f64) into
 // `out` (Real): -N * (Ms src), or the bare convolution -N * src when
-// weight_ms is false.  Inactive cells get zero.
+// weight_ms is false.  Inactive cells get zero.  With boundary tiers (and
+// weight_ms) the source is phi Ms src in every cell and the sparse
+// correction rows are added afterwards, also at inactive targets.
 void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true);
 // Re-plans the demag lines and launch geometry under the schedule `t`
 // (radices, lanes, rows per workgroup, tiles) without reallocating; on

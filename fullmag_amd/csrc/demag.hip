@@ -163,15 +163,17 @@ struct MagSource {
     double ms;
     int nx, ny, nz;
     int64_t n;
+    const double *phi;  // boundary tiers: every cell packs phi Ms m, mask or not
     __device__ Real operator()(int c, int z, int y, int x) const {
         const int64_t i = ((int64_t)z * ny + y) * nx + x;
+        if (phi) return Real(double(m[(int64_t)c * n + i]) * (phi[i] * (ms_cell ? ms_cell[i] : ms)));
         if (mask && !mask[i]) return Real(0);
         const double w = ms_cell ? ms_cell[i] : ms;
         return Real(double(m[(int64_t)c * n + i]) * w);
     }
     // No mask and no per-cell Ms: the value is one unconditional load, so a
     // lane's loads can all be issued before any of them is used.
-    __device__ bool uniform() const { return !mask && !ms_cell; }
+    __device__ bool uniform() const { return !mask && !ms_cell && !phi; }
     __device__ Real uniform_at(int c, int z, int y, int x) const {
         return Real(double(m[(int64_t)c * n + ((int64_t)z * ny + y) * nx + x]) * ms);
     }
@@ -1379,6 +1381,54 @@ void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) 
 }
 
 template <typename Real, typename S>
+struct BoundaryCorrArgs {
+    Real *out;
+    const S *m;
+    const double *phi, *ms_cell;
+    double ms;
+    int64_t n;
+    const int32_t *cell;
+    const uint32_t *off, *row;
+    const int32_t *src;
+    const double *tensor;
+    uint32_t groups, stencil;
+};
+
+// Sparse near-field demag correction of the boundary tiers
+// (native/backends/fdm/src/demag_boundary_fp64.cu:23-76): the field at a
+// target cell gains -sum_s dN(t, s) . (phi_s Ms_s m_s), dN symmetric with the
+// components Nxx, Nxy, Nxz, Nyy, Nyz, Nzz; sources < 0 are skipped.  One lane
+// per target cell adds that cell's rows in plan order, so a cell named by
+// several rows gets every row, in FP64, without atomics.
+template <typename Real, typename S>
+__global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Real, S> a) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= a.groups) return;
+    const int64_t ti = a.cell[g], n = a.n;
+    double h[3] = {double(a.out[ti]), double(a.out[n + ti]), double(a.out[2 * n + ti])};
+    for (uint32_t e = a.off[g]; e < a.off[g + 1]; ++e) {
+        const uint64_t r0 = (uint64_t)a.row[e] * a.stencil;
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (uint32_t s = 0; s < a.stencil; ++s) {
+            const int32_t si = a.src[r0 + s];
+            if (si < 0) continue;
+            const double *nn = a.tensor + (r0 + s) * 6;
+            const double w = a.phi[si] * (a.ms_cell ? a.ms_cell[si] : a.ms);
+            const double mx = w * double(a.m[si]), my = w * double(a.m[n + si]), mz = w * double(a.m[2 * n + si]);
+            cx -= nn[0] * mx + nn[1] * my + nn[2] * mz;
+            cy -= nn[1] * mx + nn[3] * my + nn[4] * mz;
+            cz -= nn[2] * mx + nn[4] * my + nn[5] * mz;
+        }
+        h[0] += cx;
+        h[1] += cy;
+        h[2] += cz;
+    }
+    a.out[ti] = Real(h[0]);
+    a.out[n + ti] = Real(h[1]);
+    a.out[2 * n + ti] = Real(h[2]);
+}
+
+template <typename Real, typename S>
 void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     DemagPlan &dp = c.dp;
     cplx<Real> *spec = static_cast<cplx<Real> *>(dp.spec);
@@ -1399,8 +1449,9 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
     const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
     {
-        MagSource<Real, S> ms{src, c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
-                              dp.nx, dp.ny, dp.nz, (int64_t)c.n};
+        MagSource<Real, S> ms{src,    c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
+                              dp.nx,  dp.ny,    dp.nz,
+                              (int64_t)c.n, weight_ms ? c.d_phi : nullptr};
         Timed t(c, "demag_x_r2c");
         with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
@@ -1579,6 +1630,27 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
                            hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream,
                                               xa, spec, sink);
                        });
+        });
+    }
+    if (weight_ms && c.corr_groups > 0) {
+        BoundaryCorrArgs<Real, S> ba{};
+        ba.out = out;
+        ba.m = src;
+        ba.phi = c.d_phi;
+        ba.ms_cell = c.d_ms_cell;
+        ba.ms = c.ms;
+        ba.n = (int64_t)c.n;
+        ba.cell = c.d_corr_cell;
+        ba.off = c.d_corr_off;
+        ba.row = c.d_corr_row;
+        ba.src = c.d_corr_src;
+        ba.tensor = c.d_corr_n;
+        ba.groups = c.corr_groups;
+        ba.stencil = c.corr_stencil;
+        Timed t(c, "demag_boundary");
+        const dim3 g1((unsigned)cdiv((int64_t)c.corr_groups, 256));
+        launch_rec(c, nullptr, g1, dim3(256), 0, 1, ba, [&] {
+            hipLaunchKernelGGL((boundary_corr_kernel<Real, S>), g1, dim3(256), 0, c.stream, ba);
         });
     }
 }

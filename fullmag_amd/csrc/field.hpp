@@ -36,6 +36,15 @@ struct FieldArgs {
     int exchange, has_ext, precession;
     double hx, hy, hz;
     double gamma_bar, alpha;
+    // sub-cell boundary exchange: tier_ex 1 (T0 face links) or 2 (T1
+    // intersection distances) replaces the stencil above
+    int tier_ex;
+    const double *phi;     // volume fraction
+    const double *tier_w;  // 6 n: xp, xm, yp, ym, zp, zm face links or distances
+    double phi_floor, delta_min;
+    double tier_pref;      // 2 A / (mu0 Ms) with the plan's scalar A
+    double a_tier;         // the plan's scalar A (per-cell Ms)
+    double cell[3];        // dx, dy, dz
 };
 
 // Exchange field at cell i in the arithmetic type C of the source, with
@@ -145,6 +154,94 @@ __device__ __forceinline__ void exchange_col(const FieldArgs &f, const S *__rest
     }
 }
 
+// Sub-cell boundary exchange at cell (x, y, z) = i, in the arithmetic type C
+// of the source.  T0 (tier_ex 1, native/backends/fdm/src/exchange_t0_fp64.cu:
+// 19-133): H = 2A / (mu0 Ms) / max(phi, phi_floor) sum f (m_j - m) / d^2 over
+// the faces whose link f > 0; T1 (tier_ex 2, exchange_t1_fp64.cu:29-175):
+// H = 2A / (mu0 Ms) sum 2 (m_j - m) / (d (d + delta')) over the neighbours with
+// nonzero m, delta = max(delta_side, delta_min) and delta' the opposite side's.
+// Neighbours beyond the grid add nothing (z only when nz > 1); cells with
+// phi <= 0 or m = 0 get zero.  The active mask and the region table do not
+// enter.  cc, vzm, vzp: m at i and at i -+ nx ny (read only where that
+// neighbour is inside the grid).
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__restrict__ m, int64_t i, int x, int y,
+                                              int z, const S cc[3], const S vzm[3], const S vzp[3], C h[3]) {
+    h[0] = h[1] = h[2] = C(0);
+    const double phi0 = f.phi[i];
+    if (phi0 <= 0.0 || (cc[0] == S(0) && cc[1] == S(0) && cc[2] == S(0))) return;
+    const int64_t n = f.n;
+    // sides in the order -x, +x, -y, +y, -z, +z; side s reads weight s ^ 1
+    // of the xp, xm, yp, ym, zp, zm arrays
+    const bool in[6] = {x > 0, x < f.nx - 1, y > 0, y < f.ny - 1, f.nz > 1 && z > 0, f.nz > 1 && z < f.nz - 1};
+    const int64_t off[4] = {-1, 1, -(int64_t)f.nx, (int64_t)f.nx};
+    S v[6][3];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) v[s][q] = in[s] ? m[q * n + i + off[s]] : cc[q];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        v[4][q] = vzm[q];
+        v[5][q] = vzp[q];
+    }
+    const C c0[3] = {C(cc[0]), C(cc[1]), C(cc[2])};
+    const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * f.ms_cell[i]) : f.tier_pref;
+    C b[3] = {C(0), C(0), C(0)};
+    if (f.tier_ex == 1) {
+        const double id2[3] = {f.idx2, f.idy2, f.idz2};
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            const double fl = f.tier_w[(int64_t)(s ^ 1) * n + i];
+            if (fl > 0.0 && in[s]) {
+                const C k = C(fl * id2[s >> 1]);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
+            }
+        }
+        const double inv_phi = 1.0 / (phi0 > f.phi_floor ? phi0 : f.phi_floor);
+        const C scale = C(pre * inv_phi);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) h[q] = scale * b[q];
+    } else {
+        double dl[6];
+#pragma unroll
+        for (int s = 0; s < 6; ++s) dl[s] = fmax(f.tier_w[(int64_t)(s ^ 1) * n + i], f.delta_min);
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            if (in[s] && dl[s] > 0.0 && !(v[s][0] == S(0) && v[s][1] == S(0) && v[s][2] == S(0))) {
+                const double d = f.cell[s >> 1];
+                const C k = C(2.0 / (d * (d + dl[s ^ 1])));
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
+            }
+        }
+        const C scale = C(pre);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) h[q] = scale * b[q];
+    }
+}
+
+// exchange_tier at cell i with every neighbour loaded from memory.
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_tier_at(const FieldArgs &f, const S *__restrict__ m, int64_t i, C h[3]) {
+    const int64_t nxy = (int64_t)f.nx * f.ny;
+    const uint32_t ui = (uint32_t)i;
+    const uint32_t z = f.div_nxy.div(ui);
+    const uint32_t r = ui - z * (uint32_t)nxy;
+    const uint32_t y = f.div_nx.div(r);
+    const uint32_t x = r - y * (uint32_t)f.nx;
+    const int64_t zm = z > 0 ? i - nxy : i, zp = (int)z < f.nz - 1 ? i + nxy : i;
+    S cc[3], vzm[3], vzp[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        cc[q] = m[q * f.n + i];
+        vzm[q] = m[q * f.n + zm];
+        vzp[q] = m[q * f.n + zp];
+    }
+    exchange_tier<C, S>(f, m, i, (int)x, (int)y, (int)z, cc, vzm, vzp, h);
+}
+
 // Workgroup b runs on XCD b mod 8 (round-robin dispatch over the 8 XCDs of
 // an MI355X).  Renumbered so that each XCD receives one contiguous range of
 // logical workgroups: neighbouring tiles then share that XCD's L2.
@@ -236,8 +333,10 @@ struct StageArgs {
 };
 
 // Per-cell body of the stage kernel; cc, vzm, vzp: src at i and at its z
-// neighbours (see exchange_col).
-template <typename State, typename Real, typename Src, typename Out, int MODE>
+// neighbours (see exchange_col).  TIER: the exchange is the sub-cell boundary
+// stencil of f.tier_ex (a separate instantiation, so that the plain kernels
+// keep their register budget).
+template <typename State, typename Real, typename Src, typename Out, int MODE, bool TIER = false>
 __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out> &a, int64_t i, int x, int y, int z,
                                            const Src cc[3], const Src vzm[3], const Src vzp[3], double *acc) {
     const FieldArgs &f = a.f;
@@ -249,7 +348,8 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
     if (act) {
         if (f.exchange) {
             Src he[3];
-            exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+            if (TIER) exchange_tier<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+            else exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
 #pragma unroll
             for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
         }
@@ -290,9 +390,19 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
 #pragma unroll
         for (int c = 0; c < 3; ++c) md[c] = double(cc[c]);
         const double w = f.ms_cell ? f.ms_cell[i] : 1.0;
+        // with a boundary tier the exchange energy stays that of the standard
+        // masked stencil (native/backends/fdm/src/reductions_fp64.cu:123-200
+        // sums the pair differences without face links or distances)
+        double he[3] = {double(hex[0]), double(hex[1]), double(hex[2])};
+        if (TIER && f.exchange) {
+            Src hp[3];
+            exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, hp);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) he[c] = double(Real(hp[c]));
+        }
         // a lane's column (at most one z chunk) is summed plainly; lanes and
         // workgroups merge with compensation (block_reduce, finalize)
-        acc[0] += w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2]));
+        acc[0] += w * (md[0] * he[0] + md[1] * he[1] + md[2] * he[2]);
         acc[2] += w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2]));
         if (f.has_ext) acc[4] += w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz);
         const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
@@ -320,7 +430,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
 // plane of src is read from memory once per chunk (plus one halo plane on
 // each side) and the x/y neighbours come from the same or an adjacent tile
 // on the same XCD.
-template <typename State, typename Real, typename Src, typename Out, int MODE>
+template <typename State, typename Real, typename Src, typename Out, int MODE, bool TIER = false>
 __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Src, Out> a) {
     const FieldArgs &f = a.f;
     const int64_t n = f.n;
@@ -346,7 +456,7 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
         for (int z = z0; z < z1; ++z, i += nxy) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) nxt[c] = z + 1 < f.nz ? a.src[c * n + i + nxy] : cur[c];
-            stage_cell<State, Real, Src, Out, MODE>(a, i, x, y, z, cur, prev, nxt, acc);
+            stage_cell<State, Real, Src, Out, MODE, TIER>(a, i, x, y, z, cur, prev, nxt, acc);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 prev[c] = cur[c];
@@ -495,7 +605,7 @@ __global__ void __launch_bounds__(kBlock) stage_uniform_multi(const StageArgs<St
 
 // Whether a stage with these arguments may run as stage_uniform_kernel.
 inline bool stage_uniform_ok(const FieldArgs &f, const void *hd, int nprev) {
-    return !f.mask && !f.region && !f.ms_cell && hd && f.exchange && nprev <= 1;
+    return !f.mask && !f.region && !f.ms_cell && !f.tier_ex && hd && f.exchange && nprev <= 1;
 }
 
 // Column tiling of the stage kernels: tile width TX = 64 (or the smallest
@@ -549,6 +659,16 @@ FieldArgs field_args(const Context &c) {
     f.hz = c.hext[2];
     f.gamma_bar = c.gamma / (1.0 + c.alpha * c.alpha);
     f.alpha = c.alpha;
+    f.tier_ex = c.tier_ex;
+    f.phi = c.d_phi;
+    f.tier_w = c.d_tier_w;
+    f.phi_floor = c.phi_floor;
+    f.delta_min = c.delta_min;
+    f.tier_pref = 2.0 * c.a / (kMu0 * c.ms);
+    f.a_tier = c.a;
+    f.cell[0] = c.dx;
+    f.cell[1] = c.dy;
+    f.cell[2] = c.dz;
     return f;
 }
 

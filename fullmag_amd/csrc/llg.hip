@@ -60,8 +60,9 @@ __global__ void __launch_bounds__(kBlock) combine_kernel(const State *m0, StageA
     }
 }
 
-// Field observables at m: H_ex and H_eff (Real).
-template <typename State, typename Real>
+// Field observables at m: H_ex and H_eff (Real); TIER: sub-cell boundary
+// exchange.
+template <typename State, typename Real, bool TIER>
 __global__ void __launch_bounds__(kBlock) observe_kernel(FieldArgs f, const State *m, const Real *hd, Real *hex_out,
                                                          Real *heff_out) {
     const int64_t n = f.n;
@@ -71,7 +72,8 @@ __global__ void __launch_bounds__(kBlock) observe_kernel(FieldArgs f, const Stat
         if (act) {
             if (f.exchange) {
                 State he[3];
-                exchange_at<State, State>(f, m, i, he);
+                if (TIER) exchange_tier_at<State, State>(f, m, i, he);
+                else exchange_at<State, State>(f, m, i, he);
                 for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
             }
             const Real ext[3] = {f.has_ext ? Real(f.hx) : Real(0), f.has_ext ? Real(f.hy) : Real(0),
@@ -187,7 +189,10 @@ struct Driver {
             return;
         }
         launch_rec(c, nullptr, g, b, 0, 1, a, [&] {
-            hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), g, b, 0, c.stream, a);
+            if (a.f.tier_ex)
+                hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN, true>), g, b, 0, c.stream, a);
+            else
+                hipLaunchKernelGGL((stage_kernel<State, Real, Src, Out, STAGE_PLAIN>), g, b, 0, c.stream, a);
         });
     }
 
@@ -220,8 +225,12 @@ struct Driver {
         const unsigned g = grid_reduce();
         {
             Timed t(c, "llg_error");
-            hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_ERROR>), dim3(g), dim3(kBlock), 0,
-                               c.stream, a);
+            if (a.f.tier_ex)
+                hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_ERROR, true>), dim3(g), dim3(kBlock),
+                                   0, c.stream, a);
+            else
+                hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_ERROR>), dim3(g), dim3(kBlock), 0,
+                                   c.stream, a);
         }
         hipLaunchKernelGGL((finalize_kernel<STAGE_ERROR>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
                            c.d_result);
@@ -260,6 +269,9 @@ struct Driver {
                 else
                     hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false>),
                                        dim3(g), dim3(kBlock), 0, c.stream, a);
+            } else if (a.f.tier_ex) {
+                hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS, true>), dim3(g), dim3(kBlock),
+                                   0, c.stream, a);
             } else {
                 hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS>), dim3(g), dim3(kBlock), 0,
                                    c.stream, a);
@@ -504,9 +516,14 @@ struct Driver {
         if (!c.h_ex_obs && !ctx_alloc(c, &c.h_ex_obs, 3 * c.n * sizeof(Real), "H_ex observable")) return false;
         if (!c.h_eff_obs && !ctx_alloc(c, &c.h_eff_obs, 3 * c.n * sizeof(Real), "H_eff observable")) return false;
         FieldArgs f = field_args(c);
-        hipLaunchKernelGGL((observe_kernel<State, Real>), dim3(grid_plain()), dim3(kBlock), 0, c.stream, f, m(),
-                           c.demag ? hd() : nullptr, static_cast<Real *>(c.h_ex_obs),
-                           static_cast<Real *>(c.h_eff_obs));
+        const Real *hdp = c.demag ? hd() : nullptr;
+        Real *hex = static_cast<Real *>(c.h_ex_obs), *heff = static_cast<Real *>(c.h_eff_obs);
+        if (f.tier_ex)
+            hipLaunchKernelGGL((observe_kernel<State, Real, true>), dim3(grid_plain()), dim3(kBlock), 0, c.stream, f,
+                               m(), hdp, hex, heff);
+        else
+            hipLaunchKernelGGL((observe_kernel<State, Real, false>), dim3(grid_plain()), dim3(kBlock), 0, c.stream, f,
+                               m(), hdp, hex, heff);
         c.obs_valid = true;
         return hip_ok(hipStreamSynchronize(c.stream), "observables", c.last_error);
     }

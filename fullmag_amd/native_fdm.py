@@ -67,6 +67,38 @@ def compute_newell_kernel_spectra(nx: int, ny: int, nz: int, dx: float, dy: floa
 
 
 @dataclass
+class BoundaryGeometry:
+    """Sub-cell boundary geometry (crates/fullmag-ir/src/lib.rs:699-724,
+    BoundaryGeometryIR): per-cell volume fraction, face-link fractions and
+    intersection distances, each of cell_count values (an empty array is
+    passed as NULL), and the sparse demag correction rows: target cell per
+    row, stencil_size source cells per row (-1: none) and a symmetric tensor
+    per (row, source) as Nxx, Nxy, Nxz, Nyy, Nyz, Nzz."""
+
+    volume_fraction: np.ndarray
+    face_link_xp: Optional[np.ndarray] = None
+    face_link_xm: Optional[np.ndarray] = None
+    face_link_yp: Optional[np.ndarray] = None
+    face_link_ym: Optional[np.ndarray] = None
+    face_link_zp: Optional[np.ndarray] = None
+    face_link_zm: Optional[np.ndarray] = None
+    delta_xp: Optional[np.ndarray] = None
+    delta_xm: Optional[np.ndarray] = None
+    delta_yp: Optional[np.ndarray] = None
+    delta_ym: Optional[np.ndarray] = None
+    delta_zp: Optional[np.ndarray] = None
+    delta_zm: Optional[np.ndarray] = None
+    demag_corr_target_idx: Optional[np.ndarray] = None
+    demag_corr_source_idx: Optional[np.ndarray] = None
+    demag_corr_tensor: Optional[np.ndarray] = None
+    demag_corr_stencil_size: int = 0
+
+
+# FdmPlanIR.boundary_correction strings (native_fdm.rs:250-254)
+_BOUNDARY = {"volume": A.BOUNDARY_VOLUME, "full": A.BOUNDARY_FULL}
+
+
+@dataclass
 class FdmPlan:
     """FdmPlanIR subset consumed by the FDM backend."""
 
@@ -95,6 +127,12 @@ class FdmPlan:
     # array is passed through as given.
     demag_kernel: object = "device"
     ms_cell: Optional[np.ndarray] = None  # extension: per-cell Ms [A/m]
+    # sub-cell boundary tiers: "volume" (T0 face links) or "full" (T1
+    # intersection distances), with the planner's geometry
+    boundary_correction: Optional[str] = None
+    boundary_geometry: Optional[BoundaryGeometry] = None
+    boundary_phi_floor: float = 0.0  # 0: the library default 0.05
+    boundary_delta_min: float = 0.0  # 0: the library default 0.1 min(dx, dy, dz)
 
     @property
     def cell_count(self) -> int:
@@ -202,6 +240,47 @@ class NativeFdmBackend:
             keep.append(lut)
             d.exchange_lut = _dp(lut)
             d.exchange_lut_len = lut.size
+        d.boundary_correction = _BOUNDARY.get(plan.boundary_correction, A.BOUNDARY_NONE)
+        d.boundary_phi_floor = plan.boundary_phi_floor
+        d.boundary_delta_min = plan.boundary_delta_min
+        bg = plan.boundary_geometry
+        if bg is not None:
+            def f64(name):
+                v = getattr(bg, name)
+                if v is None or np.asarray(v).size == 0:
+                    return None
+                a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+                keep.append(a)
+                return a
+
+            vf = f64("volume_fraction")
+            if vf is not None:
+                d.volume_fraction = _dp(vf)
+                d.volume_fraction_len = vf.size
+            for name in ("face_link_xp", "face_link_xm", "face_link_yp", "face_link_ym", "face_link_zp",
+                         "face_link_zm", "delta_xp", "delta_xm", "delta_yp", "delta_ym", "delta_zp", "delta_zm"):
+                a = f64(name)
+                if a is not None:
+                    if a.size != n:
+                        raise RunError(f"boundary_geometry.{name} has {a.size} values, expected {n}")
+                    setattr(d, name, _dp(a))
+            tgt = bg.demag_corr_target_idx
+            if tgt is not None and np.asarray(tgt).size > 0:
+                # native_fdm.rs:313-338: the flag follows a non-empty target list
+                rows = np.asarray(tgt).size
+                st = int(bg.demag_corr_stencil_size)
+                t = np.ascontiguousarray(np.asarray(tgt, dtype=np.int32).reshape(-1))
+                src = np.ascontiguousarray(np.asarray(bg.demag_corr_source_idx, dtype=np.int32).reshape(-1))
+                ten = np.ascontiguousarray(np.asarray(bg.demag_corr_tensor, dtype=np.float64).reshape(-1))
+                if src.size != rows * st or ten.size != rows * st * 6:
+                    raise RunError(f"demag correction arrays do not match {rows} rows x stencil {st}")
+                keep += [t, src, ten]
+                d.has_demag_boundary_corr = 1
+                d.demag_corr_target_idx = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+                d.demag_corr_source_idx = src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+                d.demag_corr_tensor = _dp(ten)
+                d.demag_corr_target_count = rows
+                d.demag_corr_stencil_size = st
         d.adaptive_max_error = plan.adaptive_max_error
         d.adaptive_dt_min = plan.adaptive_dt_min
         d.adaptive_dt_max = plan.adaptive_dt_max

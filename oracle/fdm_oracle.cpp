@@ -510,6 +510,20 @@ struct Problem {
     std::vector<uint32_t> region;
     std::vector<double> lut;
     double max_error, dt_min, dt_max, headroom;
+    // sub-cell boundary tiers (plan_desc boundary_correction; the reference's
+    // FP64 kernels native/backends/fdm/src/exchange_t0_fp64.cu,
+    // exchange_t1_fp64.cu, demag_boundary_fp64.cu and the volume-fraction
+    // packing of demag_fp64.cu:54-100): tier 1 = T0 face-link exchange, tier
+    // 2 = T1 intersection-distance exchange; empty phi = no tier
+    int tier = 0;
+    double phi_floor = 0.05, delta_min = 0.0;
+    std::vector<double> phi;
+    std::vector<double> face[6];   // xp, xm, yp, ym, zp, zm
+    std::vector<double> delta[6];  // xp, xm, yp, ym, zp, zm
+    std::vector<int32_t> corr_t, corr_s;
+    std::vector<double> corr_n;    // per (target, stencil entry): Nxx, Nxy, Nxz, Nyy, Nyz, Nzz
+    uint32_t corr_stencil = 0;
+    bool tiers() const { return tier > 0 && !phi.empty(); }
     // state
     std::vector<V3> m;
     double time = 0.0;
@@ -572,10 +586,91 @@ V3 pair_sum(const Problem &p, const std::vector<V3> &q, size_t f) {
     return s;
 }
 
+// T0 exchange, native/backends/fdm/src/exchange_t0_fp64.cu:19-135: empty
+// cells (phi <= 0) and zero m give 0; otherwise
+//   H = 2A / (mu0 Ms) / max(phi, phi_floor) * sum_faces f (m_j - m) / d^2
+// over the faces with link f > 0 and a neighbour inside the grid (z faces
+// only when nz > 1), in the order -x, +x, -y, +y, -z, +z.
+V3 exchange_t0(const Problem &p, const std::vector<V3> &m, size_t f) {
+    const double phi0 = p.phi[f];
+    const V3 c = m[f];
+    if (phi0 <= 0.0 || (c[0] == 0.0 && c[1] == 0.0 && c[2] == 0.0)) return {0, 0, 0};
+    const size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
+    const size_t nxy = p.nx * p.ny;
+    const double inv_phi = 1.0 / (phi0 > p.phi_floor ? phi0 : p.phi_floor);
+    const double ix = 1.0 / (p.dx * p.dx), iy = 1.0 / (p.dy * p.dy), iz = 1.0 / (p.dz * p.dz);
+    V3 b{0, 0, 0};
+    auto face = [&](int k, bool inside, size_t j, double inv2) {
+        const double fl = p.face[k][f];
+        if (fl > 0.0 && inside)
+            for (int c3 = 0; c3 < 3; ++c3) b[c3] += fl * inv2 * (m[j][c3] - c[c3]);
+    };
+    face(1, x > 0, f - 1, ix);
+    face(0, x + 1 < p.nx, f + 1, ix);
+    face(3, y > 0, f - p.nx, iy);
+    face(2, y + 1 < p.ny, f + p.nx, iy);
+    if (p.nz > 1) {
+        face(5, z > 0, f - nxy, iz);
+        face(4, z + 1 < p.nz, f + nxy, iz);
+    }
+    return scale(b, 2.0 * p.a / (MU0 * p.ms_at(f)) * inv_phi);
+}
+
+// T1 exchange, native/backends/fdm/src/exchange_t1_fp64.cu:26-177: empty
+// cells and zero m give 0; with delta = max(delta_side, delta_min) per side,
+// a neighbour inside the grid with nonzero m and delta > 0 on its side adds
+//   2 (m_j - m) / (d (d + delta_opposite))
+// (the opposite side's distance sets the non-uniform spacing); z faces only
+// when nz > 1; H = 2A / (mu0 Ms) * sum.
+V3 exchange_t1(const Problem &p, const std::vector<V3> &m, size_t f) {
+    const double phi0 = p.phi[f];
+    const V3 c = m[f];
+    if (phi0 <= 0.0 || (c[0] == 0.0 && c[1] == 0.0 && c[2] == 0.0)) return {0, 0, 0};
+    const size_t x = f % p.nx, y = (f / p.nx) % p.ny, z = f / (p.nx * p.ny);
+    const size_t nxy = p.nx * p.ny;
+    V3 b{0, 0, 0};
+    auto side = [&](bool inside, double dnear, double dfar, size_t j, double d) {
+        if (!(inside && dnear > 0.0)) return;
+        const V3 mn = m[j];
+        if (mn[0] == 0.0 && mn[1] == 0.0 && mn[2] == 0.0) return;
+        const double coeff = 2.0 / (d * (d + dfar));
+        for (int c3 = 0; c3 < 3; ++c3) b[c3] += coeff * (mn[c3] - c[c3]);
+    };
+    auto dl = [&](int k) { return std::max(p.delta[k][f], p.delta_min); };
+    const double dxp = dl(0), dxm = dl(1), dyp = dl(2), dym = dl(3);
+    side(x > 0, dxm, dxp, f - 1, p.dx);
+    side(x + 1 < p.nx, dxp, dxm, f + 1, p.dx);
+    side(y > 0, dym, dyp, f - p.nx, p.dy);
+    side(y + 1 < p.ny, dyp, dym, f + p.nx, p.dy);
+    if (p.nz > 1) {
+        const double dzp = dl(4), dzm = dl(5);
+        side(z > 0, dzm, dzp, f - nxy, p.dz);
+        side(z + 1 < p.nz, dzp, dzm, f + nxy, p.dz);
+    }
+    return scale(b, 2.0 * p.a / (MU0 * p.ms_at(f)));
+}
+
+// Which exchange the reference launches (native/backends/fdm/src/
+// exchange_fp64.cu:158-214): T0 for tier 1 with face links, T1 for tier >= 2
+// with intersection distances, else the standard stencil.
+int exchange_tier(const Problem &p) {
+    if (!p.tiers()) return 0;
+    if (p.tier == 1 && !p.face[0].empty()) return 1;
+    if (p.tier >= 2 && !p.delta[0].empty()) return 2;
+    return 0;
+}
+
 // lib.rs:2247-2299 (per-cell Ms generalises the prefactor 2A/(mu0 Ms)); with a
-// region table, H_f = 2 / (mu0 Ms_f) pair_sum(m, f).
-std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m) {
+// region table, H_f = 2 / (mu0 Ms_f) pair_sum(m, f).  `standard`: the
+// masked stencil even when a boundary tier replaces it for the field.
+std::vector<V3> exchange_field(const Problem &p, const std::vector<V3> &m, bool standard = false) {
     std::vector<V3> h(p.n);
+    if (const int t = standard ? 0 : exchange_tier(p)) {
+#pragma omp parallel for schedule(static)
+        for (long fl = 0; fl < (long)p.n; ++fl)
+            h[fl] = t == 1 ? exchange_t0(p, m, (size_t)fl) : exchange_t1(p, m, (size_t)fl);
+        return h;
+    }
     const double dx2 = p.dx * p.dx, dy2 = p.dy * p.dy, dz2 = p.dz * p.dz;
 #pragma omp parallel for schedule(static)
     for (long fl = 0; fl < (long)p.n; ++fl) {
@@ -616,7 +711,10 @@ std::vector<V3> demag_apply(Problem &p, const std::vector<V3> &m, bool weight_ms
         for (size_t y = 0; y < p.ny; ++y)
             for (size_t x = 0; x < p.nx; ++x) {
                 size_t s = x + p.nx * (y + p.ny * z), d = padded_index(px, py, x, y, z);
-                V3 mo = p.active(s) ? scale(m[s], weight_ms ? p.ms_at(s) : 1.0) : V3{0, 0, 0};
+                // with tiers every cell packs phi Ms m (demag_fp64.cu:80-85)
+                V3 mo = p.tiers() ? scale(m[s], p.phi[s] * (weight_ms ? p.ms_at(s) : 1.0))
+                        : p.active(s) ? scale(m[s], weight_ms ? p.ms_at(s) : 1.0)
+                                      : V3{0, 0, 0};
                 for (int c = 0; c < 3; ++c) p.bm[c][d] = cd(mo[c], 0.0);
             }
     for (int c = 0; c < 3; ++c) fft3_core(p.bm[c].data(), (int)px, (int)py, (int)pz, false);
@@ -641,6 +739,26 @@ std::vector<V3> demag_apply(Problem &p, const std::vector<V3> &m, bool weight_ms
                                         p.bh[2][s].real() * norm_}
                                    : V3{0, 0, 0};
             }
+    // sparse near-field correction, native/backends/fdm/src/
+    // demag_boundary_fp64.cu:20-76: H_t += -dN(t, s) . (phi_s Ms m_s)
+    if (p.tiers() && p.corr_stencil > 0) {
+        for (size_t t = 0; t < p.corr_t.size(); ++t) {
+            const int32_t ti = p.corr_t[t];
+            if (ti < 0) continue;
+            V3 corr{0, 0, 0};
+            for (uint32_t e = 0; e < p.corr_stencil; ++e) {
+                const int32_t si = p.corr_s[t * p.corr_stencil + e];
+                if (si < 0) continue;
+                const double *nn = &p.corr_n[(t * p.corr_stencil + e) * 6];
+                const double w = p.phi[si] * (weight_ms ? p.ms_at(si) : 1.0);
+                const V3 mm = scale(m[si], w);
+                corr[0] -= nn[0] * mm[0] + nn[1] * mm[1] + nn[2] * mm[2];
+                corr[1] -= nn[1] * mm[0] + nn[3] * mm[1] + nn[4] * mm[2];
+                corr[2] -= nn[2] * mm[0] + nn[4] * mm[1] + nn[5] * mm[2];
+            }
+            h[ti] = add(h[ti], corr);
+        }
+    }
     return h;
 }
 
@@ -711,9 +829,15 @@ Eval full_eval(Problem &p, const std::vector<V3> &m, bool want_stats) {
     if (!want_stats) return e;
     const double v = p.dx * p.dy * p.dz;
     double ex = 0, ed = 0, eext = 0;
+    // with a boundary tier the exchange energy is still that of the standard
+    // masked stencil (native/backends/fdm/src/reductions_fp64.cu:123-200 sums
+    // pair differences without face links or distances)
+    std::vector<V3> h_std;
+    if (p.exchange && exchange_tier(p)) h_std = exchange_field(p, m, true);
+    const std::vector<V3> &h_energy = h_std.empty() ? e.h_ex : h_std;
     for (size_t i = 0; i < p.n; ++i) {
         double ms = p.ms_at(i);
-        if (p.exchange) ex += -0.5 * MU0 * ms * dot(m[i], e.h_ex[i]) * v;
+        if (p.exchange && p.active(i)) ex += -0.5 * MU0 * ms * dot(m[i], h_energy[i]) * v;
         if (p.demag) ed += -0.5 * MU0 * ms * dot(m[i], e.h_d[i]) * v;
         if (p.has_ext) eext += -MU0 * ms * dot(m[i], e.h_ext[i]) * v;
     }
@@ -1200,6 +1324,40 @@ int oracle_problem_set_regions(void *h, const uint32_t *region, const double *lu
 // Error norm (lib.rs:1653-1674) of one attempted RK23 or DP45 step of size
 // dt from the current state, which is left unchanged.  The first slope is
 // evaluated at the state.  Returns -1 for other integrators.
+// Sub-cell boundary tiers: face (6 n: xp, xm, yp, ym, zp, zm) and delta
+// (6 n) may be null; the correction arrays are target (count), source
+// (count * stencil) and tensor (count * stencil * 6).
+int oracle_problem_set_tiers(void *h, int tier, double phi_floor, double delta_min, const double *phi,
+                             const double *face, const double *delta, uint32_t count, uint32_t stencil,
+                             const int32_t *target, const int32_t *source, const double *tensor) {
+    Problem *p = static_cast<Problem *>(h);
+    const size_t n = p->n;
+    p->tier = tier;
+    p->phi_floor = phi_floor > 0.0 ? phi_floor : 0.05;
+    p->delta_min = delta_min > 0.0 ? delta_min : 0.1 * std::min({p->dx, p->dy, p->dz});
+    p->phi.assign(phi, phi + n);
+    for (int k = 0; k < 6; ++k) {
+        p->face[k].clear();
+        p->delta[k].clear();
+        if (face) p->face[k].assign(face + k * n, face + (k + 1) * n);
+        if (delta) p->delta[k].assign(delta + k * n, delta + (k + 1) * n);
+    }
+    p->has_fsal = false;
+    p->k_fsal.clear();
+    p->abm_count = 0;
+    p->corr_stencil = 0;
+    p->corr_t.clear();
+    p->corr_s.clear();
+    p->corr_n.clear();
+    if (count > 0 && stencil > 0 && target && source && tensor) {
+        p->corr_stencil = stencil;
+        p->corr_t.assign(target, target + count);
+        p->corr_s.assign(source, source + (size_t)count * stencil);
+        p->corr_n.assign(tensor, tensor + (size_t)count * stencil * 6);
+    }
+    return 0;
+}
+
 double oracle_problem_trial_error(void *h, double dt) {
     auto *p = static_cast<Problem *>(h);
     std::vector<V3> st, k7;

@@ -69,6 +69,10 @@ def lib():
         L.oracle_problem_observe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_problem_set_regions.restype = i
         L.oracle_problem_set_regions.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_problem_set_tiers.restype = i
+        L.oracle_problem_set_tiers.argtypes = [ctypes.c_void_p, i, d, d, _dp, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]
         L.oracle_problem_trial_error.restype = d
         L.oracle_problem_trial_error.argtypes = [ctypes.c_void_p, d]
         L.oracle_num_threads.restype = i
@@ -160,6 +164,19 @@ class OracleConfig:
     ms_cell: Optional[np.ndarray] = None  # extension: per-cell Ms
     region_mask: Optional[np.ndarray] = None  # region id per cell (< 256)
     exchange_lut: Optional[np.ndarray] = None  # 256 x 256 A_ij; None: A on the diagonal
+    # sub-cell boundary tiers: 1 = T0 face links, >= 2 = T1 distances; phi
+    # (n,), face_links / deltas (6, n) in the order xp, xm, yp, ym, zp, zm,
+    # correction rows: target (rows,), source (rows, stencil), tensor
+    # (rows, stencil, 6) as Nxx, Nxy, Nxz, Nyy, Nyz, Nzz
+    tier: int = 0
+    phi: Optional[np.ndarray] = None
+    face_links: Optional[np.ndarray] = None
+    deltas: Optional[np.ndarray] = None
+    phi_floor: float = 0.0
+    delta_min: float = 0.0
+    corr_target: Optional[np.ndarray] = None
+    corr_source: Optional[np.ndarray] = None
+    corr_tensor: Optional[np.ndarray] = None
     max_error: float = 0.0
     dt_min: float = 0.0
     dt_max: float = 0.0
@@ -204,6 +221,26 @@ class OracleProblem:
                 None if self._lut is None else self._lut.ctypes.data_as(ctypes.c_void_p))
             if rc != 0:
                 raise ValueError("region ids must be below 256")
+        if cfg.tier and cfg.phi is not None:
+            self._tiers = [np.ascontiguousarray(cfg.phi, dtype=np.float64).reshape(self.n)]
+            vp = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+            face = delta = None
+            if cfg.face_links is not None:
+                face = np.ascontiguousarray(cfg.face_links, dtype=np.float64).reshape(6 * self.n)
+            if cfg.deltas is not None:
+                delta = np.ascontiguousarray(cfg.deltas, dtype=np.float64).reshape(6 * self.n)
+            rows = stencil = 0
+            tgt = src = ten = None
+            if cfg.corr_target is not None:
+                tgt = np.ascontiguousarray(cfg.corr_target, dtype=np.int32).reshape(-1)
+                rows = tgt.size
+                src = np.ascontiguousarray(cfg.corr_source, dtype=np.int32).reshape(rows, -1)
+                stencil = src.shape[1]
+                ten = np.ascontiguousarray(cfg.corr_tensor, dtype=np.float64).reshape(rows * stencil * 6)
+            self._tiers += [face, delta, tgt, src, ten]
+            lib().oracle_problem_set_tiers(self._h, int(cfg.tier), cfg.phi_floor, cfg.delta_min,
+                                           _ptr(self._tiers[0]), vp(face), vp(delta), rows, stencil, vp(tgt),
+                                           vp(src), vp(ten))
 
     def __del__(self):
         h = getattr(self, "_h", None)
