@@ -414,8 +414,12 @@ double asymptotic_nxy(double x, double y, double z, double vol) {
 constexpr size_t ASYMPTOTIC_DISTANCE = 40;  // newell.rs:229
 
 // newell.rs:254-519: six real-space components on the (2nx,2ny,2nz) grid.
+// `shifted` is compute_newell_kernels_shifted (newell.rs:528-651): every z
+// coordinate is offset by z_offset, and the mirrored z slots pz - k keep the
+// value and the sign of slot k for all six components (the reference
+// places them so; only the x and y parities are applied).
 void compute_newell_kernels(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz,
-                            std::vector<double> out[6]) {
+                            std::vector<double> out[6], double z_offset = 0.0, bool shifted = false) {
     const size_t px = 2 * nx, py = 2 * ny, pz = 2 * nz;
     const size_t plen = px * py * pz;
     const size_t fsx = std::min(nx, ASYMPTOTIC_DISTANCE) + 2;
@@ -429,7 +433,7 @@ void compute_newell_kernels(size_t nx, size_t ny, size_t nz, double dx, double d
             for (size_t i = 0; i < fsx; ++i) {
                 double x = (double)((long)i - 1) * dx;
                 double y = (double)((long)j - 1) * dy;
-                double z = (double)(kk - 1) * dz;
+                double z = (double)(kk - 1) * dz + z_offset;
                 size_t id = (size_t)kk * fsx * fsy + j * fsx + i;
                 fxx[id] = newell_f(x, y, z);
                 fyy[id] = newell_f(y, x, z);
@@ -450,7 +454,7 @@ void compute_newell_kernels(size_t nx, size_t ny, size_t nz, double dx, double d
                 bool asym = i >= ASYMPTOTIC_DISTANCE || j >= ASYMPTOTIC_DISTANCE || k >= ASYMPTOTIC_DISTANCE ||
                             dist2 >= ASYMPTOTIC_DISTANCE * ASYMPTOTIC_DISTANCE;
                 if (asym) {
-                    double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz;
+                    double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz + z_offset;
                     double vol = dx * dy * dz;
                     nxx = asymptotic_nxx(x, y, z, vol);
                     nyy = asymptotic_nxx(y, x, z, vol);
@@ -467,7 +471,7 @@ void compute_newell_kernels(size_t nx, size_t ny, size_t nz, double dx, double d
                     nyz = ldia(i, j, k, gyz, fsx, fsy, dy, dz, dx);
                 }
                 size_t xs_i[2] = {i, px - i}, ys_i[2] = {j, py - j}, zs_i[2] = {k, pz - k};
-                double xs_s[2] = {1.0, -1.0}, ys_s[2] = {1.0, -1.0}, zs_s[2] = {1.0, -1.0};
+                double xs_s[2] = {1.0, -1.0}, ys_s[2] = {1.0, -1.0}, zs_s[2] = {1.0, shifted ? 1.0 : -1.0};
                 int nxs = i == 0 ? 1 : 2, nys = j == 0 ? 1 : 2, nzs = k == 0 ? 1 : 2;
                 for (int a = 0; a < nxs; ++a)
                     for (int b = 0; b < nys; ++b)
@@ -1057,6 +1061,266 @@ void report_to(const Report &r, double *out) {
     out[7] = r.max_h_eff; out[8] = r.max_h_demag; out[9] = r.max_rhs;
 }
 
+// ---------------------------------------------------------------------------
+// Multilayer stack: layers on their own grids coupled by demag on a common
+// convolution grid (crates/fullmag-engine/src/multilayer.rs:159-262,
+// crates/fullmag-fdm-demag/src/{shifted_kernel.rs, transfer.rs,
+// multiply.rs:25-112}), advanced by the runner's Heun
+// (crates/fullmag-runner/src/multilayer_reference.rs:398-700).
+// ---------------------------------------------------------------------------
+
+// push_m, transfer.rs:18-121: each convolution cell averages the native cells
+// it overlaps, weighted by the overlap volume; grids of equal size are copied.
+std::vector<V3> push_m(const std::vector<V3> &m, const size_t nc[3], const double nd[3], const size_t cc[3],
+                       const double cd[3]) {
+    if (nc[0] == cc[0] && nc[1] == cc[1] && nc[2] == cc[2]) return m;
+    std::vector<V3> out(cc[0] * cc[1] * cc[2], V3{0, 0, 0});
+    for (size_t cz = 0; cz < cc[2]; ++cz)
+        for (size_t cy = 0; cy < cc[1]; ++cy)
+            for (size_t cx = 0; cx < cc[0]; ++cx) {
+                const double lo[3] = {cx * cd[0], cy * cd[1], cz * cd[2]};
+                const double hi[3] = {lo[0] + cd[0], lo[1] + cd[1], lo[2] + cd[2]};
+                long a0[3], a1[3];
+                for (int d = 0; d < 3; ++d) {
+                    a0[d] = (long)std::floor(lo[d] / nd[d]);
+                    a1[d] = (long)std::ceil(hi[d] / nd[d]);
+                }
+                double tot = 0.0;
+                V3 acc{0, 0, 0};
+                for (long z = a0[2]; z < a1[2]; ++z)
+                    for (long y = a0[1]; y < a1[1]; ++y)
+                        for (long x = a0[0]; x < a1[0]; ++x) {
+                            if (x < 0 || y < 0 || z < 0 || x >= (long)nc[0] || y >= (long)nc[1] || z >= (long)nc[2])
+                                continue;
+                            const double n0[3] = {x * nd[0], y * nd[1], z * nd[2]};
+                            double ov = 1.0;
+                            for (int d = 0; d < 3; ++d)
+                                ov *= std::max(std::min(hi[d], n0[d] + nd[d]) - std::max(lo[d], n0[d]), 0.0);
+                            if (ov > 0.0) {
+                                const V3 v = m[(size_t)x + nc[0] * ((size_t)y + nc[1] * (size_t)z)];
+                                for (int k = 0; k < 3; ++k) acc[k] += v[k] * ov;
+                                tot += ov;
+                            }
+                        }
+                if (tot > 0.0) out[cx + cc[0] * (cy + cc[1] * cz)] = {acc[0] / tot, acc[1] / tot, acc[2] / tot};
+            }
+    return out;
+}
+
+// pull_h, transfer.rs:124-198: trilinear interpolation of the convolution
+// grid at each native cell centre, indices clamped at the edges.
+std::vector<V3> pull_h(const std::vector<V3> &h, const size_t cc[3], const double cd[3], const size_t nc[3],
+                       const double nd[3]) {
+    if (nc[0] == cc[0] && nc[1] == cc[1] && nc[2] == cc[2]) return h;
+    std::vector<V3> out(nc[0] * nc[1] * nc[2]);
+    for (size_t z = 0; z < nc[2]; ++z)
+        for (size_t y = 0; y < nc[1]; ++y)
+            for (size_t x = 0; x < nc[0]; ++x) {
+                const double f[3] = {(x + 0.5) * nd[0] / cd[0] - 0.5, (y + 0.5) * nd[1] / cd[1] - 0.5,
+                                     (z + 0.5) * nd[2] / cd[2] - 0.5};
+                long b[3];
+                double w[3];
+                for (int d = 0; d < 3; ++d) {
+                    b[d] = (long)std::floor(f[d]);
+                    w[d] = f[d] - std::floor(f[d]);
+                }
+                V3 r{0, 0, 0};
+                for (int dz = 0; dz < 2; ++dz)
+                    for (int dy = 0; dy < 2; ++dy)
+                        for (int dx = 0; dx < 2; ++dx) {
+                            const long ix = std::min(std::max(b[0] + dx, 0L), (long)cc[0] - 1);
+                            const long iy = std::min(std::max(b[1] + dy, 0L), (long)cc[1] - 1);
+                            const long iz = std::min(std::max(b[2] + dz, 0L), (long)cc[2] - 1);
+                            const double ww = (dx == 0 ? 1.0 - w[0] : w[0]) * (dy == 0 ? 1.0 - w[1] : w[1]) *
+                                              (dz == 0 ? 1.0 - w[2] : w[2]);
+                            const V3 v = h[(size_t)ix + cc[0] * ((size_t)iy + cc[1] * (size_t)iz)];
+                            for (int k = 0; k < 3; ++k) r[k] += v[k] * ww;
+                        }
+                out[x + nc[0] * (y + nc[1] * z)] = r;
+            }
+    return out;
+}
+
+struct Stack {
+    std::vector<Problem *> layer;  // exchange, Zeeman, mask and material of each layer
+    std::vector<double> oz;        // native origin z per layer
+    size_t cc[3];                  // common convolution grid
+    double csz[3];                 // its cell size
+    bool demag = true;
+    size_t px, py, pz;
+    std::vector<std::array<std::vector<cd>, 6>> kern;  // pair src * L + dst
+    double time = 0.0;
+};
+
+// Kernel pairs in the runner's order (multilayer_reference.rs:398-439): self
+// kernel for src == dst, shifted kernel with dst.z - src.z otherwise.
+void stack_build(Stack &s) {
+    s.px = 2 * s.cc[0];
+    s.py = 2 * s.cc[1];
+    s.pz = 2 * s.cc[2];
+    const size_t L = s.layer.size(), plen = s.px * s.py * s.pz;
+    s.kern.assign(L * L, {});
+    if (!s.demag) return;
+    for (size_t a = 0; a < L; ++a)
+        for (size_t b = 0; b < L; ++b) {
+            std::vector<double> nk[6];
+            if (a == b) compute_newell_kernels(s.cc[0], s.cc[1], s.cc[2], s.csz[0], s.csz[1], s.csz[2], nk);
+            else
+                compute_newell_kernels(s.cc[0], s.cc[1], s.cc[2], s.csz[0], s.csz[1], s.csz[2], nk, s.oz[b] - s.oz[a],
+                                       true);
+            for (int c = 0; c < 6; ++c) {
+                auto &k = s.kern[a * L + b][c];
+                k.resize(plen);
+                for (size_t i = 0; i < plen; ++i) k[i] = cd(nk[c][i], 0.0);
+                fft3_core(k.data(), (int)s.px, (int)s.py, (int)s.pz, false);
+            }
+        }
+}
+
+// MultilayerDemagRuntime::compute_demag_fields (multilayer.rs:159-262): the
+// fields of all layers, not yet zeroed outside their masks.
+std::vector<std::vector<V3>> stack_demag(Stack &s, const std::vector<std::vector<V3>> &ms) {
+    const size_t L = s.layer.size(), px = s.px, py = s.py, pz = s.pz, plen = px * py * pz;
+    std::vector<std::vector<V3>> out(L);
+    if (!s.demag) {
+        for (size_t l = 0; l < L; ++l) out[l].assign(s.layer[l]->n, V3{0, 0, 0});
+        return out;
+    }
+    std::vector<std::array<std::vector<cd>, 3>> mf(L), hf(L);
+    for (size_t l = 0; l < L; ++l) {
+        const Problem &p = *s.layer[l];
+        const size_t nc[3] = {p.nx, p.ny, p.nz};
+        const double nd[3] = {p.dx, p.dy, p.dz};
+        const std::vector<V3> cm = push_m(ms[l], nc, nd, s.cc, s.csz);
+        for (int c = 0; c < 3; ++c) {
+            mf[l][c].assign(plen, cd(0, 0));
+            hf[l][c].assign(plen, cd(0, 0));
+        }
+        for (size_t z = 0; z < s.cc[2]; ++z)
+            for (size_t y = 0; y < s.cc[1]; ++y)
+                for (size_t x = 0; x < s.cc[0]; ++x) {
+                    const V3 v = cm[x + s.cc[0] * (y + s.cc[1] * z)];
+                    const size_t d = padded_index(px, py, x, y, z);
+                    for (int c = 0; c < 3; ++c) mf[l][c][d] = cd(v[c] * p.ms, 0.0);
+                }
+        for (int c = 0; c < 3; ++c) fft3_core(mf[l][c].data(), (int)px, (int)py, (int)pz, false);
+    }
+    // accumulate_tensor_convolution (multiply.rs:25-112) over the pairs
+    for (size_t a = 0; a < L; ++a)
+        for (size_t b = 0; b < L; ++b) {
+            const auto &K = s.kern[a * L + b];
+            auto &H = hf[b];
+            const auto &M = mf[a];
+#pragma omp parallel for schedule(static)
+            for (long il = 0; il < (long)plen; ++il) {
+                const size_t i = (size_t)il;
+                const cd mx = M[0][i], my = M[1][i], mz = M[2][i];
+                H[0][i] += K[0][i] * mx + K[3][i] * my + K[4][i] * mz;
+                H[1][i] += K[3][i] * mx + K[1][i] * my + K[5][i] * mz;
+                H[2][i] += K[4][i] * mx + K[5][i] * my + K[2][i] * mz;
+            }
+        }
+    const double norm_ = 1.0 / (double)plen;
+    for (size_t l = 0; l < L; ++l) {
+        for (int c = 0; c < 3; ++c) {
+            for (auto &v : hf[l][c]) v = -v;
+            fft3_core(hf[l][c].data(), (int)px, (int)py, (int)pz, true);
+        }
+        std::vector<V3> ch(s.cc[0] * s.cc[1] * s.cc[2]);
+        for (size_t z = 0; z < s.cc[2]; ++z)
+            for (size_t y = 0; y < s.cc[1]; ++y)
+                for (size_t x = 0; x < s.cc[0]; ++x) {
+                    const size_t i = padded_index(px, py, x, y, z);
+                    ch[x + s.cc[0] * (y + s.cc[1] * z)] = {hf[l][0][i].real() * norm_, hf[l][1][i].real() * norm_,
+                                                         hf[l][2][i].real() * norm_};
+                }
+        const Problem &p = *s.layer[l];
+        const size_t nc[3] = {p.nx, p.ny, p.nz};
+        const double nd[3] = {p.dx, p.dy, p.dz};
+        out[l] = pull_h(ch, s.cc, s.csz, nc, nd);
+    }
+    return out;
+}
+
+struct StackEval {
+    std::vector<std::vector<V3>> h_ex, h_d, h_ext, h_eff, rhs;
+    Report r{};
+};
+
+// llg_rhs_multilayer / observe_multilayer (multilayer_reference.rs:441-651):
+// per layer H_eff = (H_ex + H_d) + H_ext with H_d and H_ext zeroed outside the
+// layer's mask, the layer's LLG right-hand side, and the stack's energies
+// (sums) and maxima.
+StackEval stack_eval(Stack &s, const std::vector<std::vector<V3>> &ms, bool want_stats) {
+    const size_t L = s.layer.size();
+    StackEval e;
+    e.h_d = stack_demag(s, ms);
+    e.h_ex.resize(L);
+    e.h_ext.resize(L);
+    e.h_eff.resize(L);
+    e.rhs.resize(L);
+    double ex = 0, ed = 0, eext = 0, mh = 0, md = 0, mr = 0;
+    for (size_t l = 0; l < L; ++l) {
+        Problem &p = *s.layer[l];
+        const auto &m = ms[l];
+        e.h_ex[l] = p.exchange ? exchange_field(p, m) : std::vector<V3>(p.n, V3{0, 0, 0});
+        e.h_ext[l].resize(p.n);
+        e.h_eff[l].resize(p.n);
+        e.rhs[l].resize(p.n);
+        const V3 ext = p.has_ext ? p.hext : V3{0, 0, 0};
+        for (size_t i = 0; i < p.n; ++i) {
+            if (!p.active(i)) e.h_d[l][i] = {0, 0, 0};
+            e.h_ext[l][i] = p.active(i) ? ext : V3{0, 0, 0};
+            e.h_eff[l][i] = p.active(i) ? add(add(e.h_ex[l][i], e.h_d[l][i]), e.h_ext[l][i]) : V3{0, 0, 0};
+            e.rhs[l][i] = llg_rhs_from_field(p, m[i], e.h_eff[l][i]);
+        }
+        if (!want_stats) continue;
+        const double v = p.dx * p.dy * p.dz;
+        for (size_t i = 0; i < p.n; ++i) {
+            if (p.exchange && p.active(i)) ex += -0.5 * MU0 * p.ms * dot(m[i], e.h_ex[l][i]) * v;
+            if (s.demag) ed += -0.5 * MU0 * p.ms * dot(m[i], e.h_d[l][i]) * v;
+            if (p.has_ext) eext += -MU0 * p.ms * dot(m[i], e.h_ext[l][i]) * v;
+        }
+        mh = std::max(mh, max_norm(e.h_eff[l]));
+        md = std::max(md, max_norm(e.h_d[l]));
+        mr = std::max(mr, max_norm(e.rhs[l]));
+    }
+    e.r.e_ex = ex;
+    e.r.e_demag = ed;
+    e.r.e_ext = eext;
+    e.r.e_total = ex + ed + eext;
+    e.r.max_h_eff = mh;
+    e.r.max_h_demag = md;
+    e.r.max_rhs = mr;
+    e.r.time_seconds = s.time;
+    return e;
+}
+
+// step_multilayer (multilayer_reference.rs:543-589): Heun over all layers.
+Report stack_heun(Stack &s, double dt) {
+    const size_t L = s.layer.size();
+    std::vector<std::vector<V3>> m0(L), pred(L);
+    for (size_t l = 0; l < L; ++l) m0[l] = s.layer[l]->m;
+    const StackEval e1 = stack_eval(s, m0, false);
+    for (size_t l = 0; l < L; ++l) {
+        pred[l].resize(m0[l].size());
+        for (size_t i = 0; i < m0[l].size(); ++i) pred[l][i] = normalized(add(m0[l][i], scale(e1.rhs[l][i], dt)));
+    }
+    const StackEval e2 = stack_eval(s, pred, false);
+    for (size_t l = 0; l < L; ++l) {
+        Problem &p = *s.layer[l];
+        for (size_t i = 0; i < p.n; ++i)
+            p.m[i] = normalized(add(m0[l][i], scale(add(e1.rhs[l][i], e2.rhs[l][i]), 0.5 * dt)));
+        p.time += dt;
+    }
+    s.time += dt;
+    std::vector<std::vector<V3>> m1(L);
+    for (size_t l = 0; l < L; ++l) m1[l] = s.layer[l]->m;
+    Report r = stack_eval(s, m1, true).r;
+    r.dt_used = dt;
+    return r;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1429,6 +1693,79 @@ int oracle_problem_heun_adjoint(void *h, double dt, const double *m0, const doub
     if (grad_hext)
         for (int k = 0; k < 3; ++k) grad_hext[k] += gh[k];
     return 0;
+}
+
+// Shifted Newell kernel (newell.rs:528-651) on the padded (2nx, 2ny, 2nz)
+// grid, out[6][2nz][2ny][2nx].
+void oracle_newell_kernels_shifted(uint64_t nx, uint64_t ny, uint64_t nz, double dx, double dy, double dz,
+                                   double z_offset, double *out) {
+    std::vector<double> k[6];
+    compute_newell_kernels(nx, ny, nz, dx, dy, dz, k, z_offset, true);
+    const size_t plen = 8 * nx * ny * nz;
+    for (int c = 0; c < 6; ++c) std::memcpy(out + c * plen, k[c].data(), plen * sizeof(double));
+}
+
+// Grid transfer operators (AoS vectors): native -> convolution and back.
+void oracle_push_m(const double *m, const uint64_t *nc, const double *nd, const uint64_t *cc, const double *cdd,
+                   double *out) {
+    const size_t n[3] = {nc[0], nc[1], nc[2]}, c[3] = {cc[0], cc[1], cc[2]};
+    std::vector<V3> v(n[0] * n[1] * n[2]);
+    std::memcpy(v.data(), m, v.size() * sizeof(V3));
+    const std::vector<V3> r = push_m(v, n, nd, c, cdd);
+    std::memcpy(out, r.data(), r.size() * sizeof(V3));
+}
+void oracle_pull_h(const double *h, const uint64_t *cc, const double *cdd, const uint64_t *nc, const double *nd,
+                   double *out) {
+    const size_t n[3] = {nc[0], nc[1], nc[2]}, c[3] = {cc[0], cc[1], cc[2]};
+    std::vector<V3> v(c[0] * c[1] * c[2]);
+    std::memcpy(v.data(), h, v.size() * sizeof(V3));
+    const std::vector<V3> r = pull_h(v, c, cdd, n, nd);
+    std::memcpy(out, r.data(), r.size() * sizeof(V3));
+}
+
+// A stack of layer problems (created with demag off; the stack owns the
+// coupling).  The problems stay owned by the caller and must outlive it.
+void *oracle_stack_create(uint32_t count, void *const *layers, const double *origins, const uint64_t *common,
+                          const double *conv_cell, int demag) {
+    auto *s = new Stack();
+    for (uint32_t l = 0; l < count; ++l) {
+        s->layer.push_back(static_cast<Problem *>(layers[l]));
+        s->oz.push_back(origins[3 * l + 2]);
+    }
+    for (int d = 0; d < 3; ++d) {
+        s->cc[d] = common[d];
+        s->csz[d] = conv_cell[d];
+    }
+    s->demag = demag != 0;
+    stack_build(*s);
+    return s;
+}
+void oracle_stack_destroy(void *h) { delete static_cast<Stack *>(h); }
+int oracle_stack_step(void *h, double dt, double *report) {
+    auto *s = static_cast<Stack *>(h);
+    if (!(dt > 0.0)) return -1;
+    const Report r = stack_heun(*s, dt);
+    if (report) report_to(r, report);
+    return 0;
+}
+// fields: per layer in order, 5 AoS arrays (H_ex, H_demag, H_ext, H_eff, rhs)
+// of 3 n_l doubles; scalars: the step report layout.
+void oracle_stack_observe(void *h, double *fields, double *scalars) {
+    auto *s = static_cast<Stack *>(h);
+    std::vector<std::vector<V3>> ms;
+    for (Problem *p : s->layer) ms.push_back(p->m);
+    const StackEval e = stack_eval(*s, ms, true);
+    size_t off = 0;
+    for (size_t l = 0; l < s->layer.size(); ++l) {
+        const size_t n = s->layer[l]->n;
+        const std::vector<V3> *arr[5] = {&e.h_ex[l], &e.h_d[l], &e.h_ext[l], &e.h_eff[l], &e.rhs[l]};
+        if (fields)
+            for (int a = 0; a < 5; ++a) {
+                std::memcpy(fields + off, arr[a]->data(), 3 * n * sizeof(double));
+                off += 3 * n;
+            }
+    }
+    if (scalars) report_to(e.r, scalars);
 }
 
 }  // extern "C"

@@ -73,6 +73,16 @@ def lib():
         L.oracle_problem_set_tiers.argtypes = [ctypes.c_void_p, i, d, d, _dp, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]
+        L.oracle_newell_kernels_shifted.argtypes = [u64, u64, u64, d, d, d, d, _dp]
+        _u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.oracle_push_m.argtypes = [_dp, _u64p, _dp, _u64p, _dp, _dp]
+        L.oracle_pull_h.argtypes = [_dp, _u64p, _dp, _u64p, _dp, _dp]
+        L.oracle_stack_create.restype = ctypes.c_void_p
+        L.oracle_stack_create.argtypes = [ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p), _dp, _u64p, _dp, i]
+        L.oracle_stack_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_stack_step.restype = i
+        L.oracle_stack_step.argtypes = [ctypes.c_void_p, d, _dp]
+        L.oracle_stack_observe.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_problem_trial_error.restype = d
         L.oracle_problem_trial_error.argtypes = [ctypes.c_void_p, d]
         L.oracle_num_threads.restype = i
@@ -301,4 +311,74 @@ class OracleProblem:
                                      s.ctypes.data_as(ctypes.c_void_p))
         out = dict(zip(REPORT_FIELDS, s.tolist()))
         out.update(h_ex=f[0], h_demag=f[1], h_ext=f[2], h_eff=f[3], rhs=f[4])
+        return out
+
+
+def newell_kernels_shifted(nx, ny, nz, dx, dy, dz, z_offset) -> np.ndarray:
+    """Shifted cross-layer kernel (newell.rs:528-651), shape (6, 2nz, 2ny, 2nx)."""
+    out = np.zeros((6, 2 * nz, 2 * ny, 2 * nx))
+    lib().oracle_newell_kernels_shifted(nx, ny, nz, dx, dy, dz, z_offset, _ptr(out))
+    return out
+
+
+def _u64(v):
+    return (ctypes.c_uint64 * 3)(*[int(x) for x in v])
+
+
+def push_m(m, native_cells, native_cell, conv_cells, conv_cell) -> np.ndarray:
+    """transfer.rs push_m: native (N, 3) -> convolution grid (Nc, 3)."""
+    m = np.ascontiguousarray(m, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((int(np.prod(conv_cells)), 3))
+    lib().oracle_push_m(_ptr(m), _u64(native_cells), _ptr(np.asarray(native_cell, dtype=np.float64)),
+                        _u64(conv_cells), _ptr(np.asarray(conv_cell, dtype=np.float64)), _ptr(out))
+    return out
+
+
+def pull_h(h, conv_cells, conv_cell, native_cells, native_cell) -> np.ndarray:
+    """transfer.rs pull_h: convolution grid (Nc, 3) -> native (N, 3)."""
+    h = np.ascontiguousarray(h, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((int(np.prod(native_cells)), 3))
+    lib().oracle_pull_h(_ptr(h), _u64(conv_cells), _ptr(np.asarray(conv_cell, dtype=np.float64)),
+                        _u64(native_cells), _ptr(np.asarray(native_cell, dtype=np.float64)), _ptr(out))
+    return out
+
+
+class OracleStack:
+    """Multilayer stack (crates/fullmag-engine/src/multilayer.rs, runner
+    multilayer_reference.rs): `layers` are OracleProblem objects built with
+    demag=False (their exchange, Zeeman, mask and material), coupled by the
+    stack's demag on the common convolution grid; steps are Heun."""
+
+    def __init__(self, layers, origins, common_cells, conv_cell, demag=True):
+        self.layers = list(layers)
+        self._org = np.ascontiguousarray(origins, dtype=np.float64).reshape(len(self.layers), 3)
+        arr = (ctypes.c_void_p * len(self.layers))(*[p._h for p in self.layers])
+        self._h = lib().oracle_stack_create(len(self.layers), arr, _ptr(self._org), _u64(common_cells),
+                                            _ptr(np.asarray(conv_cell, dtype=np.float64)), 1 if demag else 0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().oracle_stack_destroy(h)
+            self._h = None
+
+    def step(self, dt: float) -> dict:
+        r = np.zeros(10)
+        if lib().oracle_stack_step(self._h, dt, _ptr(r)) != 0:
+            raise ValueError("dt must be positive")
+        return dict(zip(REPORT_FIELDS, r.tolist()))
+
+    def observe(self) -> dict:
+        ns = [p.n for p in self.layers]
+        f = np.zeros(15 * sum(ns))
+        sc = np.zeros(10)
+        lib().oracle_stack_observe(self._h, f.ctypes.data_as(ctypes.c_void_p), sc.ctypes.data_as(ctypes.c_void_p))
+        out = dict(zip(REPORT_FIELDS, sc.tolist()))
+        off = 0
+        for key in ("h_ex", "h_demag", "h_ext", "h_eff", "rhs"):
+            out[key] = []
+        for n in ns:
+            for key in ("h_ex", "h_demag", "h_ext", "h_eff", "rhs"):
+                out[key].append(f[off:off + 3 * n].reshape(n, 3).copy())
+                off += 3 * n
         return out
