@@ -9,12 +9,15 @@ ABI (crates/fullmag-runner/src/native_fdm.rs).
 from .native_fdm import (  # noqa: F401
     BatchRun,
     BoundaryGeometry,
+    FdmLayerPlan,
+    FdmMultilayerPlan,
     FdmPlan,
     FieldSnapshot,
     NativeFdmBackend,
     RunError,
     StepStats,
     compute_newell_kernel_spectra,
+    MultilayerStack,
     generate_random_unit_vectors,
     is_available,
 )

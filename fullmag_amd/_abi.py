@@ -138,6 +138,14 @@ class ExtOptions(ctypes.Structure):
                 ("ms_cell", _pd), ("ms_cell_len", _u64)]
 
 
+class StackDesc(ctypes.Structure):
+    """fullmag_fdm_ext_stack_desc."""
+    _fields_ = [("struct_size", _u32), ("layer_count", _u32),
+                ("layers", ctypes.POINTER(ctypes.POINTER(PlanDesc))), ("layer_origins", _pd),
+                ("common_cells", _u32 * 3), ("reserved", _u32), ("convolution_cell_size", _d * 3),
+                ("enable_demag", ctypes.c_int32), ("precision_override", ctypes.c_int32)]
+
+
 # Every exported entry point and its signature: (restype, argtypes).
 SIGNATURES = {
     # fullmag_fdm.h
@@ -183,6 +191,12 @@ SIGNATURES = {
     "fullmag_fdm_ext_set_state_device": (_i32, [ctypes.c_void_p, ctypes.c_void_p, _u64]),
     "fullmag_fdm_ext_copy_field_preview_noalloc": (_i32, [ctypes.c_void_p, _i32, _u32, _u32, _u32, _u32, _u32,
                                                           _i32, ctypes.c_void_p, _u64]),
+    "fullmag_fdm_ext_stack_create": (ctypes.c_void_p, [ctypes.POINTER(StackDesc)]),
+    "fullmag_fdm_ext_stack_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "fullmag_fdm_ext_stack_step": (_i32, [ctypes.c_void_p, _d, _u64, ctypes.POINTER(StepStats)]),
+    "fullmag_fdm_ext_stack_refresh": (_i32, [ctypes.c_void_p]),
+    "fullmag_fdm_ext_stack_layer": (ctypes.c_void_p, [ctypes.c_void_p, _u32]),
+    "fullmag_fdm_ext_stack_destroy": (None, [ctypes.c_void_p]),
 }
 
 # fullmag_fdm_ext.h: FULLMAG_FDM_EXT_NOALLOC_SCRATCH_BYTES

@@ -608,6 +608,113 @@ int copy_preview_noalloc(fullmag_fdm_backend *h, fullmag_fdm_observable obs, uin
     return preview_staged<T>(c, buf, elem, pnx, pny, pnz, z0, zs, stage, out, len);
 }
 
+// Layers of a stack are stepped, and owned, by their stack.
+bool refuse_stack_layer(Context &c) {
+    if (!c.stack) return false;
+    c.last_error = "this backend is a layer of a multilayer stack: step the stack";
+    return true;
+}
+
+void stack_free(Stack *s) {
+    hipStream_t stream = s->conv.stream;
+    if (stream) hipStreamSynchronize(stream);
+    for (Context *c : s->layers) {
+        if (c->stack) c->stream = nullptr;  // the stack's stream
+        ctx_free_all(*c);
+        delete c;
+    }
+    s->conv.stream = nullptr;
+    ctx_free_all(s->conv);
+    if (stream) hipStreamDestroy(stream);
+    delete s;
+}
+
+Stack *stack_create_impl(const fullmag_fdm_ext_stack_desc &d, Stack *s) {
+    if (d.layer_count == 0 || !d.layers || !d.layer_origins) {
+        s->last_error = "a stack needs at least one layer, its plans and its origins";
+        return s;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (d.common_cells[k] == 0 || !(d.convolution_cell_size[k] > 0.0)) {
+            s->last_error = "common_cells and convolution_cell_size must be positive";
+            return s;
+        }
+    const uint64_t ncv = (uint64_t)d.common_cells[0] * d.common_cells[1] * d.common_cells[2];
+    if (ncv >= (uint64_t(1) << 32)) {
+        s->last_error = "the common grid has too many cells";
+        return s;
+    }
+    for (uint32_t l = 0; l < d.layer_count; ++l) {
+        if (!d.layers[l]) {
+            s->last_error = "layer " + std::to_string(l) + ": null plan";
+            return s;
+        }
+        fullmag_fdm_plan_desc p = *d.layers[l];
+        if (p.integrator != FULLMAG_FDM_INTEGRATOR_HEUN) {
+            // the reference's multilayer planner and runner step Heun only
+            s->last_error = "layer " + std::to_string(l) + ": multilayer stacks step with the Heun integrator";
+            return s;
+        }
+        if (p.boundary_correction != FULLMAG_FDM_BOUNDARY_NONE && p.volume_fraction) {
+            s->last_error = "layer " + std::to_string(l) + ": boundary tiers are not supported in a stack";
+            return s;
+        }
+        p.enable_demag = 0;  // the stack owns the demag coupling
+        p.demag_kernel_spectrum_len = 0;
+        fullmag_fdm_ext_options opt{};
+        opt.struct_size = sizeof(opt);
+        opt.precision_override = d.precision_override;
+        Context *c = create_impl(&p, &opt);
+        if (!c) {
+            s->last_error = "layer " + std::to_string(l) + ": allocation failed";
+            return s;
+        }
+        if (!c->ready) {
+            s->last_error = "layer " + std::to_string(l) + ": " + c->last_error;
+            ctx_free_all(*c);
+            delete c;
+            return s;
+        }
+        s->layers.push_back(c);
+        s->oz.push_back(d.layer_origins[3 * l + 2]);
+        if (c->mode != s->layers[0]->mode) {
+            s->last_error = "all layers of a stack must use one precision";
+            return s;
+        }
+    }
+    Context &cv = s->conv;
+    cv.nx = d.common_cells[0];
+    cv.ny = d.common_cells[1];
+    cv.nz = d.common_cells[2];
+    cv.n = ncv;
+    cv.dx = d.convolution_cell_size[0];
+    cv.dy = d.convolution_cell_size[1];
+    cv.dz = d.convolution_cell_size[2];
+    cv.mode = s->layers[0]->mode;
+    cv.device = s->layers[0]->device;
+    s->demag = d.enable_demag != 0;
+    if (!hip_ok(hipStreamCreateWithFlags(&cv.stream, hipStreamNonBlocking), "hipStreamCreate", s->last_error))
+        return s;
+    for (Context *c : s->layers) {
+        hipStreamSynchronize(c->stream);
+        hipStreamDestroy(c->stream);
+        c->stream = cv.stream;
+        c->stack = s;
+        if (s->demag) {
+            c->demag = true;
+            c->hd_at_m = false;
+            if (!ctx_alloc(*c, &c->hd, 3 * c->n * c->real_bytes(), "H_demag")) {
+                s->last_error = c->last_error;
+                return s;
+            }
+        }
+    }
+    if (!stack_demag_setup(*s, s->last_error)) return s;
+    if (!stack_refresh(*s)) return s;
+    s->ready = true;
+    return s;
+}
+
 }  // namespace
 
 namespace fm {
@@ -650,6 +757,7 @@ int fullmag_fdm_backend_step(fullmag_fdm_backend *handle, double dt_seconds, ful
         c.last_error = "dt must be positive and finite";
         return FULLMAG_FDM_ERR_INVALID;
     }
+    if (refuse_stack_layer(c)) return FULLMAG_FDM_ERR_INVALID;
     return integrator_step(c, dt_seconds, out_stats);
 }
 
@@ -661,6 +769,7 @@ int fullmag_fdm_ext_step_many(fullmag_fdm_backend *handle, double dt_seconds, ui
         c.last_error = "step_many needs an initialised backend, dt > 0 and n_steps > 0";
         return FULLMAG_FDM_ERR_INVALID;
     }
+    if (refuse_stack_layer(c)) return FULLMAG_FDM_ERR_INVALID;
     return integrator_step_many(c, dt_seconds, n_steps, out_stats);
 }
 
@@ -810,6 +919,7 @@ const char *fullmag_fdm_backend_last_error(fullmag_fdm_backend *handle) {
 void fullmag_fdm_backend_destroy(fullmag_fdm_backend *handle) {
     if (!handle) return;
     Context *c = as_ctx(handle);
+    if (c->stack) return;  // destroyed with its stack
     ctx_free_all(*c);
     delete c;
 }
@@ -855,6 +965,7 @@ int fullmag_fdm_ext_heun_adjoint(fullmag_fdm_backend *handle, double dt_seconds,
         c.last_error = "heun_adjoint needs an initialised backend, dt > 0 and device arrays m0, lam1, lam0";
         return FULLMAG_FDM_ERR_INVALID;
     }
+    if (refuse_stack_layer(c)) return FULLMAG_FDM_ERR_INVALID;
     return heun_adjoint(c, dt_seconds, m0_dev, lam1_dev, lam0_dev, grad_ms_dev, grad_hext);
 }
 
@@ -935,6 +1046,55 @@ void fullmag_fdm_ext_random_unit_vectors(uint64_t seed, uint64_t count, double *
 
 uint64_t fullmag_fdm_ext_device_bytes(fullmag_fdm_backend *handle) {
     return handle ? as_ctx(handle)->device_bytes : 0;
+}
+
+fullmag_fdm_stack *fullmag_fdm_ext_stack_create(const fullmag_fdm_ext_stack_desc *desc) {
+    Stack *s = new (std::nothrow) Stack();
+    if (!s) return nullptr;
+    try {
+        if (!desc || desc->struct_size != sizeof(fullmag_fdm_ext_stack_desc)) s->last_error = "fullmag_fdm_ext_stack_desc missing or of another size";
+        else stack_create_impl(*desc, s);
+    } catch (const std::bad_alloc &) {
+        s->last_error = "host allocation failed";
+    } catch (const std::exception &e) {
+        s->last_error = e.what();
+    }
+    return reinterpret_cast<fullmag_fdm_stack *>(s);
+}
+
+const char *fullmag_fdm_ext_stack_last_error(fullmag_fdm_stack *stack) {
+    if (!stack) return "null handle";
+    Stack *s = reinterpret_cast<Stack *>(stack);
+    return s->last_error.empty() ? nullptr : s->last_error.c_str();
+}
+
+int fullmag_fdm_ext_stack_step(fullmag_fdm_stack *stack, double dt_seconds, uint64_t n_steps,
+                               fullmag_fdm_step_stats *out_stats) {
+    if (!stack) return FULLMAG_FDM_ERR_INVALID;
+    Stack &s = *reinterpret_cast<Stack *>(stack);
+    if (!s.ready || !(dt_seconds > 0.0) || !std::isfinite(dt_seconds) || n_steps == 0) {
+        s.last_error = "stack_step needs an initialised stack, dt > 0 and n_steps > 0";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
+    return stack_step(s, dt_seconds, n_steps, out_stats);
+}
+
+int fullmag_fdm_ext_stack_refresh(fullmag_fdm_stack *stack) {
+    if (!stack) return FULLMAG_FDM_ERR_INVALID;
+    Stack &s = *reinterpret_cast<Stack *>(stack);
+    if (!s.ready) return FULLMAG_FDM_ERR_INVALID;
+    return stack_refresh(s) ? FULLMAG_FDM_OK : FULLMAG_FDM_ERR_CUDA;
+}
+
+fullmag_fdm_backend *fullmag_fdm_ext_stack_layer(fullmag_fdm_stack *stack, uint32_t index) {
+    if (!stack) return nullptr;
+    Stack &s = *reinterpret_cast<Stack *>(stack);
+    if (!s.ready || index >= s.layers.size()) return nullptr;
+    return as_handle(s.layers[index]);
+}
+
+void fullmag_fdm_ext_stack_destroy(fullmag_fdm_stack *stack) {
+    if (stack) stack_free(reinterpret_cast<Stack *>(stack));
 }
 
 }  // extern "C"

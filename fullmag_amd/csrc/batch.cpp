@@ -218,6 +218,10 @@ fullmag_fdm_batch *fullmag_fdm_ext_batch_create(fullmag_fdm_backend *const *hand
             fail(b, "batch member " + std::to_string(i) + " is not an initialised backend");
             return b;
         }
+        if (c->stack) {
+            fail(b, "batch member " + std::to_string(i) + " is a layer of a multilayer stack");
+            return b;
+        }
         if (!(dts[i] > 0.0)) {
             fail(b, "batch member " + std::to_string(i) + " has dt <= 0");
             return b;
@@ -318,6 +322,10 @@ int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen
     fm::Context *cp = handle ? fm::backend_context(handle) : nullptr;
     if (!cp || !cp->ready) return FULLMAG_FDM_ERR_INVALID;
     fm::Context &c = *cp;
+    if (c.stack) {
+        c.last_error = "autotune: this backend is a layer of a multilayer stack";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
     reps = std::max(1, reps);
     struct Knob {
         const char *name;
@@ -381,6 +389,10 @@ int fullmag_fdm_ext_autotune(fullmag_fdm_backend *handle, int reps, char *chosen
 int fullmag_fdm_ext_set_schedule(fullmag_fdm_backend *handle, const char *schedule) {
     fm::Context *cp = handle ? fm::backend_context(handle) : nullptr;
     if (!cp || !cp->ready || !schedule) return FULLMAG_FDM_ERR_INVALID;
+    if (cp->stack) {
+        cp->last_error = "set_schedule: this backend is a layer of a multilayer stack";
+        return FULLMAG_FDM_ERR_INVALID;
+    }
     fm::DemagTuning t = cp->dp.tune;
     fm::parse_tuning_string(t, schedule);
     t.y_split = cp->dp.tune.y_split;  // the spectrum layout stays

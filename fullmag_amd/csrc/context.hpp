@@ -194,6 +194,10 @@ struct Context {
     // (multi-instance batch set-up)
     LaunchRecorder *rec = nullptr;
 
+    // non-null: this backend is a layer of a multilayer stack, whose coupled
+    // demag fills `hd` (the layer's own demag plan is unused)
+    struct Stack *stack = nullptr;
+
     std::string last_error;
     fullmag_fdm_device_info info{};
     bool info_valid = false;
@@ -203,6 +207,33 @@ struct Context {
 
     size_t state_bytes() const { return mode == MODE_SINGLE ? 4 : 8; }
     size_t real_bytes() const { return mode == MODE_DOUBLE ? 8 : 4; }
+};
+
+// A multilayer stack: layer backends (one Context each, all on the stack's
+// stream) coupled by demag on a common convolution grid
+// (crates/fullmag-engine/src/multilayer.rs).  Spectra are laid out
+// [layer][c][plane][kx][ky] with ky < py; the pair kernels
+// [kernel][6][plane][kx][ky] (xx, yy, zz, xy, xz, yz).
+struct Stack {
+    std::vector<Context *> layers;
+    std::vector<double> oz;   // native origin z per layer [m]
+    Context conv;             // the common grid: transform plan, twiddles, buffers
+    bool demag = true;
+    int planes = 1;           // pz (3-D) or 1 (thin layers, 2-D transform)
+    uint64_t layer_spec = 0;  // complex elements per layer spectrum
+    void *spec_m = nullptr, *spec_h = nullptr;
+    void *kern = nullptr;
+    uint64_t kern_stride = 0;  // complex elements per pair kernel
+    int nkern = 0;
+    std::vector<int> pair;    // kernel of (dst, src) at dst * L + src
+    int *d_pair = nullptr;
+    // layers on another grid than the common one (push/pull transfer):
+    // conv-grid m (FP64) and H (Real); null where the grids agree
+    std::vector<void *> conv_m, conv_h;
+    uint64_t step_count = 0;
+    double time = 0.0;
+    bool ready = false;
+    std::string last_error;
 };
 
 // Issues a launch through `plain`, or, while c.rec is set, records it with
@@ -244,6 +275,14 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
 // weight_ms) the source is phi Ms src in every cell and the sparse
 // correction rows are added afterwards, also at inactive targets.
 void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true);
+// Multilayer stack demag (demag.hip).  stack_demag_setup plans the common
+// grid's transforms and builds the pair kernels (self kernel for a layer
+// with itself, the shifted kernel of newell.rs:528-651 otherwise);
+// stack_demag_eval writes every layer's H_demag into its `hd` from the
+// sources src[l] (the layers' State type when src_state, else Real).
+bool stack_demag_setup(Stack &s, std::string &err);
+void stack_demag_eval(Stack &s, const void *const *src, bool src_state);
+
 // Re-plans the demag lines and launch geometry under the schedule `t`
 // (radices, lanes, rows per workgroup, tiles) without reallocating; on
 // failure the backend keeps its plan and `err` says why.
@@ -252,13 +291,19 @@ bool newell_spectra_full(uint32_t nx, uint32_t ny, uint32_t nz, double dx, doubl
                          double *out, std::string &err);
 
 // newell.cpp: Newell tensor on the non-negative displacement octant,
-// out[c][k][j][i] for i < nx, j < ny, k < nz, c = xx, yy, zz, xy, xz, yz.
-void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz, std::vector<double> &out);
+// out[c][k][j][i] for i < nx, j < ny, k < nz, c = xx, yy, zz, xy, xz, yz, at
+// displacements (i dx, j dy, k dz + z_offset).
+void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz, std::vector<double> &out,
+                   double z_offset = 0.0);
 
 // llg.hip
 int integrator_step(Context &c, double dt, fullmag_fdm_step_stats *st);
 bool refresh_observables(Context &c);
 bool materialize_observables(Context &c);  // fills h_ex_obs / h_eff_obs at m
+// Heun steps of a whole stack (n_steps, stats of the final state summed over
+// the layers; energies add, maxima combine) and its demag at the layers' m.
+int stack_step(Stack &s, double dt, uint64_t n_steps, fullmag_fdm_step_stats *st);
+bool stack_refresh(Stack &s);
 bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
                    uint32_t z0, uint32_t zs, void *dev_out, bool out_f64);
 bool alloc_integrator_buffers(Context &c);

@@ -186,6 +186,7 @@ struct MagSource {
 struct NewellSource {
     const double *oct;  // [6][nz][ny][nx]
     int nx, ny, nz, px, py, pz;
+    int z_even;  // the cross-layer kernel: mirrored z slots keep every sign (newell.rs:630-647)
     __device__ bool uniform() const { return false; }
     __device__ double uniform_at(int, int, int, int) const { return 0.0; }
     __device__ double operator()(int c, int zz, int yy, int x) const {
@@ -194,7 +195,7 @@ struct NewellSource {
         const int yi = yy < ny ? yy : py - yy;
         const int zi = (pz == 1 || zz < nz) ? zz : pz - zz;
         const double sx = x < nx ? 1.0 : -1.0, sy = yy < ny ? 1.0 : -1.0;
-        const double sz = (pz == 1 || zz < nz) ? 1.0 : -1.0;
+        const double sz = (pz == 1 || zz < nz || z_even) ? 1.0 : -1.0;
         double s = 1.0;
         if (c == 3) s = sx * sy;
         else if (c == 4) s = sx * sz;
@@ -1093,8 +1094,9 @@ __global__ void __launch_bounds__(256) zconv_quad_multi(const ZArgs<Real> *tab, 
 }
 
 // Long z lines: a workgroup holds `tile` consecutive columns (contiguous in
-// ky) of all three components in LDS.  conv = 0: plain forward transform of
-// one component (blockIdx.y = component), used by the Newell setup.
+// ky) of all three components in LDS.  conv = 0: plain forward (or, with
+// `inverse`, inverse) transform of one component (blockIdx.y = component),
+// used by the Newell setup and the multilayer stack.
 template <typename Real>
 struct ZLdsArgs {
     cplx<Real> *data;
@@ -1102,6 +1104,7 @@ struct ZLdsArgs {
     LinePlan lp;      // length pz
     int n_in, n_out;
     int tile, conv;
+    int inverse;  // conv = 0: the inverse transform (n_in = pz, n_out <= pz)
     int64_t columns, zstride, comp_stride;
     int py, pitch;
     KernelView<Real> K;
@@ -1142,7 +1145,7 @@ __global__ void __launch_bounds__(kMaxBlock) zlds_kernel(ZLdsArgs<Real> a) {
         }
     }
     __syncthreads();
-    L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, false);
+    L::template fft<Real>(lds + (tid / T) * ls, a.lp, tid % T, a.tw, !a.conv && a.inverse);
     __syncthreads();  // the tensor product and the stores cross lines
     if (a.conv) {
         // one pass per octant entry oz <= nz, applied at kz = oz and at its
@@ -1716,7 +1719,7 @@ void parse_tuning(DemagTuning &t) { parse_tuning_string(t, std::getenv("FULLMAG_
 // Padded forward transform of the Newell kernel, FP64, into a temporary
 // spectrum spec[6][pz][hx][py].
 bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &oct, cplx<double> **spec_out,
-                    std::string &err) {
+                    std::string &err, bool z_even = false) {
     void *d_oct = nullptr, *d_spec = nullptr, *twx = nullptr, *twy = nullptr, *twz = nullptr;
     const int pz = dp.three_d ? dp.pz : 1;
     const SpecGeom g{dp.hx, pz, dp.py};
@@ -1738,7 +1741,8 @@ bool newell_forward(Context &c, const DemagPlan &dp, const std::vector<double> &
     }
     if (ok) {
         cplx<double> *spec = static_cast<cplx<double> *>(d_spec);
-        NewellSource src{static_cast<const double *>(d_oct), dp.nx, dp.ny, dp.three_d ? dp.nz : 1, dp.px, dp.py, pz};
+        NewellSource src{static_cast<const double *>(d_oct), dp.nx, dp.ny, dp.three_d ? dp.nz : 1, dp.px, dp.py, pz,
+                         z_even ? 1 : 0};
         XArgs<double> xa{};
         xa.tw = static_cast<const cplx<double> *>(twx);
         xa.lp = dp.lx;
@@ -2027,6 +2031,418 @@ void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weigh
         demag_eval_t<float, double>(c, static_cast<const double *>(src), static_cast<float *>(out), weight_ms);
     } else {
         demag_eval_t<float, float>(c, static_cast<const float *>(src), static_cast<float *>(out), weight_ms);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multilayer stack demag (crates/fullmag-engine/src/multilayer.rs:159-262):
+// every layer's Ms m on the common convolution grid is transformed (x R2C,
+// y lines, z lines for a 3-D grid; a thin common grid uses the 2-D
+// transform, whose padded z plane would only repeat the first), the pair
+// kernels are applied point by point in k-space,
+//   H_d(k) = -sum_s K(d, s)(k) . M_s(k),
+// and every layer's field is transformed back.  Layers on another grid than
+// the common one go through the push/pull transfers of
+// crates/fullmag-fdm-demag/src/transfer.rs.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename Real>
+struct StackTensorArgs {
+    const cplx<Real> *m;
+    cplx<Real> *h;
+    const cplx<Real> *kern;
+    const int *pair;
+    int layers;
+    int64_t points;        // planes * hx * py, the same linear index in spectra and kernels
+    int64_t layer_stride;  // complex elements per layer spectrum
+    int64_t comp_stride;   // complex elements per component of a spectrum (= points)
+    int64_t kern_stride;   // complex elements per pair kernel (6 * points)
+};
+
+template <typename T>
+__device__ __forceinline__ void cmad(cplx<T> &acc, const cplx<T> &a, const cplx<T> &b) {
+    acc.x += a.x * b.x - a.y * b.y;
+    acc.y += a.x * b.y + a.y * b.x;
+}
+
+// One lane per k-point; for every destination layer the sources are summed
+// in layer order (the order of the reference's kernel pairs for that
+// destination).  Kernel components: xx, yy, zz, xy, xz, yz.
+template <typename Real>
+__global__ void __launch_bounds__(256) stack_tensor_kernel(StackTensorArgs<Real> a) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.points) return;
+    const int64_t cs = a.comp_stride;
+    for (int d = 0; d < a.layers; ++d) {
+        cplx<Real> hx{Real(0), Real(0)}, hy{Real(0), Real(0)}, hz{Real(0), Real(0)};
+        for (int s = 0; s < a.layers; ++s) {
+            const cplx<Real> *K = a.kern + (int64_t)a.pair[d * a.layers + s] * a.kern_stride + i;
+            const cplx<Real> *M = a.m + (int64_t)s * a.layer_stride + i;
+            const cplx<Real> mx = M[0], my = M[cs], mz = M[2 * cs];
+            const cplx<Real> kxx = K[0], kyy = K[cs], kzz = K[2 * cs], kxy = K[3 * cs], kxz = K[4 * cs],
+                             kyz = K[5 * cs];
+            cmad(hx, kxx, mx); cmad(hx, kxy, my); cmad(hx, kxz, mz);
+            cmad(hy, kxy, mx); cmad(hy, kyy, my); cmad(hy, kyz, mz);
+            cmad(hz, kxz, mx); cmad(hz, kyz, my); cmad(hz, kzz, mz);
+        }
+        cplx<Real> *H = a.h + (int64_t)d * a.layer_stride + i;
+        H[0] = {-hx.x, -hx.y};
+        H[cs] = {-hy.x, -hy.y};
+        H[2 * cs] = {-hz.x, -hz.y};
+    }
+}
+
+struct GridXfer {
+    int nx, ny, nz;     // native grid
+    double ndx, ndy, ndz;
+    int cx, cy, cz;     // common grid
+    double cdx, cdy, cdz;
+};
+
+// push_m (transfer.rs:18-121): every common-grid cell is the overlap-volume
+// weighted mean of the native cells it overlaps (FP64); SoA in, SoA out.
+template <typename S>
+__global__ void __launch_bounds__(256) stack_push_kernel(GridXfer g, const S *m, double *out) {
+    const int64_t nc = (int64_t)g.cx * g.cy * g.cz, nn = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= nc) return;
+    const int x = (int)(p % g.cx), y = (int)((p / g.cx) % g.cy), z = (int)(p / ((int64_t)g.cx * g.cy));
+    const double lo[3] = {x * g.cdx, y * g.cdy, z * g.cdz};
+    const double hi[3] = {lo[0] + g.cdx, lo[1] + g.cdy, lo[2] + g.cdz};
+    const double nd[3] = {g.ndx, g.ndy, g.ndz};
+    const int nmax[3] = {g.nx, g.ny, g.nz};
+    int a0[3], a1[3];
+    for (int d = 0; d < 3; ++d) {
+        a0[d] = max((int)floor(lo[d] / nd[d]), 0);
+        a1[d] = min((int)ceil(hi[d] / nd[d]), nmax[d]);
+    }
+    double tot = 0.0, acc[3] = {0.0, 0.0, 0.0};
+    for (int k = a0[2]; k < a1[2]; ++k)
+        for (int j = a0[1]; j < a1[1]; ++j)
+            for (int i = a0[0]; i < a1[0]; ++i) {
+                const double o[3] = {i * g.ndx, j * g.ndy, k * g.ndz};
+                double ov = 1.0;
+                for (int d = 0; d < 3; ++d) ov *= fmax(fmin(hi[d], o[d] + nd[d]) - fmax(lo[d], o[d]), 0.0);
+                if (ov > 0.0) {
+                    const int64_t q = ((int64_t)k * g.ny + j) * g.nx + i;
+                    for (int c = 0; c < 3; ++c) acc[c] += double(m[c * nn + q]) * ov;
+                    tot += ov;
+                }
+            }
+    for (int c = 0; c < 3; ++c) out[c * nc + p] = tot > 0.0 ? acc[c] / tot : 0.0;
+}
+
+// pull_h (transfer.rs:124-198): trilinear interpolation of the common grid at
+// every native cell centre, neighbour indices clamped; inactive cells zero.
+template <typename Real>
+__global__ void __launch_bounds__(256) stack_pull_kernel(GridXfer g, const Real *h, const uint8_t *mask, Real *out) {
+    const int64_t nc = (int64_t)g.cx * g.cy * g.cz, nn = (int64_t)g.nx * g.ny * g.nz;
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p >= nn) return;
+    const int x = (int)(p % g.nx), y = (int)((p / g.nx) % g.ny), z = (int)(p / ((int64_t)g.nx * g.ny));
+    const double f[3] = {(x + 0.5) * g.ndx / g.cdx - 0.5, (y + 0.5) * g.ndy / g.cdy - 0.5,
+                         (z + 0.5) * g.ndz / g.cdz - 0.5};
+    const int cmax[3] = {g.cx, g.cy, g.cz};
+    int b[3];
+    double w[3];
+    for (int d = 0; d < 3; ++d) {
+        b[d] = (int)floor(f[d]);
+        w[d] = f[d] - floor(f[d]);
+    }
+    double r[3] = {0.0, 0.0, 0.0};
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const int ix = min(max(b[0] + dx, 0), cmax[0] - 1);
+                const int iy = min(max(b[1] + dy, 0), cmax[1] - 1);
+                const int iz = min(max(b[2] + dz, 0), cmax[2] - 1);
+                const double ww = (dx == 0 ? 1.0 - w[0] : w[0]) * (dy == 0 ? 1.0 - w[1] : w[1]) *
+                                  (dz == 0 ? 1.0 - w[2] : w[2]);
+                const int64_t q = ((int64_t)iz * g.cy + iy) * g.cx + ix;
+                for (int c = 0; c < 3; ++c) r[c] += double(h[c * nc + q]) * ww;
+            }
+    const bool act = !mask || mask[p];
+    for (int c = 0; c < 3; ++c) out[c * nn + p] = act ? Real(r[c]) : Real(0);
+}
+
+template <typename Real>
+__global__ void __launch_bounds__(256) spec_narrow_kernel(const cplx<double> *in, cplx<Real> *out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = {Real(in[i].x), Real(in[i].y)};
+}
+
+GridXfer xfer_of(const Stack &s, const Context &l) {
+    const Context &c = s.conv;
+    return GridXfer{(int)l.nx, (int)l.ny, (int)l.nz, l.dx, l.dy, l.dz,
+                    (int)c.nx, (int)c.ny, (int)c.nz, c.dx, c.dy, c.dz};
+}
+
+template <typename Real>
+bool stack_setup_t(Stack &s, std::string &err) {
+    Context &c = s.conv;
+    DemagPlan &dp = c.dp;
+    const size_t ce = sizeof(cplx<Real>);
+    const int L = (int)s.layers.size();
+    dp = DemagPlan{};
+    dp.enabled = true;
+    dp.nx = c.nx;
+    dp.ny = c.ny;
+    dp.nz = c.nz;
+    dp.three_d = c.nz > 1;
+    dp.px = 2 * c.nx;
+    dp.py = 2 * c.ny;
+    dp.pz = dp.three_d ? 2 * c.nz : 1;
+    dp.hx = c.nx + 1;
+    dp.rows = dp.py;
+    dp.inv_npad = 1.0 / ((double)dp.px * dp.py * dp.pz);
+    dp.lds_limit = device_lds_limit(c.device);
+    if (!plan_lines(dp, err)) return false;
+    if (!pick_x(dp.lx, ce, dp.xr, dp.x_ls, 16, dp.lds_limit)) {
+        err = "stack: x line FFT does not fit in one workgroup";
+        return false;
+    }
+    dp.y_lines_per_wg = pick_lpw(dp.ly, ce);
+    if ((size_t)dp.y_lines_per_wg * (dp.ly.n + kPad) * ce > dp.lds_limit ||
+        dp.y_lines_per_wg * dp.ly.threads > kMaxBlock) {
+        err = "stack: y line FFT does not fit in LDS or 512 lanes";
+        return false;
+    }
+    if (dp.three_d) {
+        dp.z_tile = pick_ztile(dp.lz, 1, ce);
+        if ((size_t)dp.z_tile * (dp.lz.n + kPad) * ce > dp.lds_limit || dp.z_tile * dp.lz.threads > kMaxBlock) {
+            err = "stack: z line FFT does not fit in LDS or 512 lanes";
+            return false;
+        }
+    }
+    if (!upload_twiddles<Real>(c, dp.px, &dp.tw_x) || !upload_twiddles<Real>(c, dp.py, &dp.tw_y)) {
+        err = c.last_error;
+        return false;
+    }
+    if (dp.three_d && !upload_twiddles<Real>(c, dp.pz, &dp.tw_z)) {
+        err = c.last_error;
+        return false;
+    }
+    s.planes = dp.pz;
+    const int64_t points = (int64_t)s.planes * dp.hx * dp.py;
+    s.layer_spec = 3 * (uint64_t)points;
+    if (!ctx_alloc(c, &s.spec_m, (size_t)L * s.layer_spec * ce, "stack spectra") ||
+        !ctx_alloc(c, &s.spec_h, (size_t)L * s.layer_spec * ce, "stack field spectra")) {
+        err = c.last_error;
+        return false;
+    }
+    // pair kernels: the self kernel once, one shifted kernel per distinct
+    // z distance dst - src (compared exactly)
+    std::vector<double> shifts;
+    s.pair.assign((size_t)L * L, 0);
+    for (int d = 0; d < L; ++d)
+        for (int q = 0; q < L; ++q) {
+            if (d == q) continue;
+            const double sh = s.oz[d] - s.oz[q];
+            auto it = std::find(shifts.begin(), shifts.end(), sh);
+            if (it == shifts.end()) {
+                shifts.push_back(sh);
+                it = shifts.end() - 1;
+            }
+            s.pair[(size_t)d * L + q] = 1 + (int)(it - shifts.begin());
+        }
+    s.nkern = 1 + (int)shifts.size();
+    s.kern_stride = 6 * (uint64_t)points;
+    if (!ctx_alloc(c, &s.kern, (size_t)s.nkern * s.kern_stride * ce, "stack pair kernels") ||
+        !ctx_alloc(c, (void **)&s.d_pair, s.pair.size() * sizeof(int), "stack pair table") ||
+        !ctx_upload(c, s.d_pair, s.pair.data(), s.pair.size() * sizeof(int), "hipMemcpy(pair table)", err))
+        return false;
+    for (int u = 0; u < s.nkern; ++u) {
+        std::vector<double> oct;
+        newell_octant(dp.nx, dp.ny, dp.three_d ? dp.nz : 1, c.dx, c.dy, c.dz, oct, u == 0 ? 0.0 : shifts[u - 1]);
+        cplx<double> *spec = nullptr;
+        if (!newell_forward(c, dp, oct, &spec, err, u > 0)) return false;
+        cplx<Real> *dst = static_cast<cplx<Real> *>(s.kern) + (uint64_t)u * s.kern_stride;
+        hipLaunchKernelGGL((spec_narrow_kernel<Real>), dim3((unsigned)cdiv(s.kern_stride, 256)), dim3(256), 0,
+                           c.stream, spec, dst, (int64_t)s.kern_stride);
+        const bool ok = hip_ok(hipStreamSynchronize(c.stream), "stack kernel", err);
+        (void)hipFree(spec);
+        if (!ok) return false;
+    }
+    s.conv_m.assign(L, nullptr);
+    s.conv_h.assign(L, nullptr);
+    const uint64_t ncv = (uint64_t)c.nx * c.ny * c.nz;
+    for (int l = 0; l < L; ++l) {
+        const Context &lc = *s.layers[l];
+        if (lc.nx == c.nx && lc.ny == c.ny && lc.nz == c.nz) continue;  // transfer.rs identity fast path
+        if (!ctx_alloc(c, &s.conv_m[l], 3 * ncv * sizeof(double), "stack transfer m") ||
+            !ctx_alloc(c, &s.conv_h[l], 3 * ncv * sizeof(Real), "stack transfer H")) {
+            err = c.last_error;
+            return false;
+        }
+    }
+    return hip_ok(hipGetLastError(), "stack setup", err);
+}
+
+template <typename Real, typename Src>
+void stack_xfwd(Context &c, const XArgs<Real> &xa, const Src &src, cplx<Real> *spec) {
+    const DemagPlan &dp = c.dp;
+    const dim3 grid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
+    const dim3 block((unsigned)((dp.xr / 2) * dp.lx.threads));
+    const size_t lds = (size_t)(dp.xr / 2) * dp.x_ls * sizeof(cplx<Real>);
+    with_line(dp.lx, true, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        hipLaunchKernelGGL((xfwd_kernel<Real, Src, N>), grid, block, lds, c.stream, xa, src, spec);
+    });
+}
+
+// y lines (3-D: per component, over the nz stored planes) and z lines of one
+// spectrum, forward or inverse.
+template <typename Real>
+void stack_yz(Stack &s, cplx<Real> *spec, bool inverse) {
+    Context &c = s.conv;
+    const DemagPlan &dp = c.dp;
+    const size_t ce = sizeof(cplx<Real>);
+    const int64_t comp = (int64_t)s.planes * dp.hx * dp.py;
+    auto zpass = [&] {
+        ZLdsArgs<Real> za{};
+        za.data = spec;
+        za.tw = static_cast<const cplx<Real> *>(dp.tw_z);
+        za.lp = dp.lz;
+        za.n_in = inverse ? dp.pz : dp.nz;
+        za.n_out = inverse ? dp.nz : dp.pz;
+        za.tile = dp.z_tile;
+        za.conv = 0;
+        za.inverse = inverse ? 1 : 0;
+        za.columns = (int64_t)dp.hx * dp.py;
+        za.zstride = (int64_t)dp.hx * dp.py;
+        za.comp_stride = comp;
+        za.py = dp.py;
+        za.pitch = dp.py;
+        with_line(dp.lz, true, [&](auto NC) {
+            hipLaunchKernelGGL((zlds_kernel<Real, decltype(NC)::value>), dim3((unsigned)cdiv(za.columns, za.tile), 3),
+                               dim3((unsigned)(za.tile * dp.lz.threads)),
+                               (size_t)za.tile * (dp.lz.n + kPad) * ce, c.stream, za);
+        });
+    };
+    if (dp.three_d && inverse) zpass();
+    LineArgs<Real> la{};
+    la.tw = static_cast<const cplx<Real> *>(dp.tw_y);
+    la.lp = dp.ly;
+    la.line_pitch = dp.py;
+    la.lpw = dp.y_lines_per_wg;
+    la.n_in = inverse ? dp.py : dp.ny;
+    la.n_out = inverse ? dp.ny : dp.py;
+    la.inverse = inverse ? 1 : 0;
+    if (dp.three_d) {
+        la.lines = (int64_t)dp.nz * dp.hx;
+        for (int k = 0; k < 3; ++k) {
+            la.data = spec + k * comp;
+            launch_lines<Real>(c, dp.ly, la, ce);
+        }
+    } else {
+        la.data = spec;
+        la.lines = 3ll * dp.hx;
+        launch_lines<Real>(c, dp.ly, la, ce);
+    }
+    if (dp.three_d && !inverse) zpass();
+}
+
+template <typename Real, typename S>
+void stack_eval_t(Stack &s, const S *const *src) {
+    Context &c = s.conv;
+    DemagPlan &dp = c.dp;
+    const int L = (int)s.layers.size();
+    XArgs<Real> xa{};
+    xa.tw = static_cast<const cplx<Real> *>(dp.tw_x);
+    xa.lp = dp.lx;
+    xa.rows = dp.ny;
+    xa.n_in = dp.nx;
+    xa.n_out = dp.nx;
+    xa.scale = Real(dp.inv_npad);
+    xa.g = SpecGeom{dp.hx, s.planes, dp.py};
+    xa.xr = dp.xr;
+    xa.xr_log2 = ilog2(dp.xr);
+    xa.ls = dp.x_ls;
+    const int64_t ncv = (int64_t)c.nx * c.ny * c.nz;
+    cplx<Real> *sm = static_cast<cplx<Real> *>(s.spec_m), *sh = static_cast<cplx<Real> *>(s.spec_h);
+    {
+        Timed t(c, "stack_forward");
+        for (int l = 0; l < L; ++l) {
+            Context &lc = *s.layers[l];
+            cplx<Real> *spec = sm + (uint64_t)l * s.layer_spec;
+            if (s.conv_m[l]) {
+                double *cm = static_cast<double *>(s.conv_m[l]);
+                hipLaunchKernelGGL((stack_push_kernel<S>), dim3((unsigned)cdiv(ncv, 256)), dim3(256), 0, c.stream,
+                                   xfer_of(s, lc), src[l], cm);
+                MagSource<Real, double> ms{cm, nullptr, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
+                stack_xfwd<Real>(c, xa, ms, spec);
+            } else {
+                MagSource<Real, S> ms{src[l], lc.d_mask, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
+                stack_xfwd<Real>(c, xa, ms, spec);
+            }
+            stack_yz<Real>(s, spec, false);
+        }
+    }
+    {
+        Timed t(c, "stack_tensor");
+        StackTensorArgs<Real> ta{};
+        ta.m = sm;
+        ta.h = sh;
+        ta.kern = static_cast<const cplx<Real> *>(s.kern);
+        ta.pair = s.d_pair;
+        ta.layers = L;
+        ta.points = (int64_t)s.planes * dp.hx * dp.py;
+        ta.layer_stride = (int64_t)s.layer_spec;
+        ta.comp_stride = ta.points;
+        ta.kern_stride = (int64_t)s.kern_stride;
+        hipLaunchKernelGGL((stack_tensor_kernel<Real>), dim3((unsigned)cdiv(ta.points, 256)), dim3(256), 0, c.stream,
+                           ta);
+    }
+    {
+        Timed t(c, "stack_inverse");
+        const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
+        const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
+        const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * sizeof(cplx<Real>);
+        for (int l = 0; l < L; ++l) {
+            Context &lc = *s.layers[l];
+            cplx<Real> *spec = sh + (uint64_t)l * s.layer_spec;
+            stack_yz<Real>(s, spec, true);
+            const bool xfer = s.conv_h[l] != nullptr;
+            Real *out = xfer ? static_cast<Real *>(s.conv_h[l]) : static_cast<Real *>(lc.hd);
+            FieldSink<Real> sink{out, xfer ? nullptr : lc.d_mask, dp.nx, dp.ny, ncv};
+            with_line(dp.lx, true, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((xinv_kernel<Real, FieldSink<Real>, N>), xgrid, xblock, xlds, c.stream, xa, spec,
+                                   sink);
+            });
+            if (xfer)
+                hipLaunchKernelGGL((stack_pull_kernel<Real>), dim3((unsigned)cdiv(lc.n, 256)), dim3(256), 0,
+                                   c.stream, xfer_of(s, lc), static_cast<const Real *>(s.conv_h[l]), lc.d_mask,
+                                   static_cast<Real *>(lc.hd));
+        }
+    }
+}
+
+}  // namespace
+
+bool stack_demag_setup(Stack &s, std::string &err) {
+    if (!s.demag) return true;
+    if (s.conv.mode == MODE_DOUBLE) return stack_setup_t<double>(s, err);
+    return stack_setup_t<float>(s, err);
+}
+
+void stack_demag_eval(Stack &s, const void *const *src, bool src_state) {
+    if (!s.demag) return;
+    const int mode = s.conv.mode;
+    const size_t L = s.layers.size();
+    if (mode == MODE_DOUBLE) {
+        std::vector<const double *> v(L);
+        for (size_t l = 0; l < L; ++l) v[l] = static_cast<const double *>(src[l]);
+        stack_eval_t<double, double>(s, v.data());
+    } else if (mode == MODE_MIXED && src_state) {
+        std::vector<const double *> v(L);
+        for (size_t l = 0; l < L; ++l) v[l] = static_cast<const double *>(src[l]);
+        stack_eval_t<float, double>(s, v.data());
+    } else {
+        std::vector<const float *> v(L);
+        for (size_t l = 0; l < L; ++l) v[l] = static_cast<const float *>(src[l]);
+        stack_eval_t<float, float>(s, v.data());
     }
 }
 

@@ -121,6 +121,18 @@ __global__ void preview_kernel(const In *fx, const In *fy, const In *fz, uint32_
     out[p * 3 + 2] = OutT(sz / cnt);
 }
 
+// H_demag of every layer of a stack at the layers' present m.
+void stack_eval_at_m(Stack &s) {
+    std::vector<const void *> src;
+    for (Context *l : s.layers) src.push_back(l->m);
+    stack_demag_eval(s, src.data(), true);
+    for (Context *l : s.layers) {
+        l->hd_at_m = true;
+        l->obs_valid = false;
+        l->obs_frozen = false;
+    }
+}
+
 // Typed driver for one precision mode.
 template <typename State, typename Real>
 struct Driver {
@@ -240,11 +252,14 @@ struct Driver {
         return std::sqrt(c.h_result[0]);
     }
 
+    // A stack layer's field couples to the other layers: it is evaluated for
+    // the whole stack at the layers' m (the stack steps its layers itself).
     void demag_at_state(const State *src) {
-        if (c.demag) demag_eval(c, src, kStateF64, hd());
+        if (c.stack) stack_eval_at_m(*c.stack);
+        else if (c.demag) demag_eval(c, src, kStateF64, hd());
     }
     void demag_at_stage(const Real *src) {
-        if (c.demag) demag_eval(c, src, kRealF64, hd());
+        if (c.demag && !c.stack) demag_eval(c, src, kRealF64, hd());
     }
     void ensure_hd() {
         if (!c.hd_at_m) demag_at_state(m());
@@ -645,6 +660,99 @@ double integrator_probe_ms(Context &c, int reps) {
     c.hd_at_m = true;
     c.pre_valid = false;  // hd was recomputed under other schedules
     return (double)ms;
+}
+
+// Heun steps of a stack (crates/fullmag-runner/src/multilayer_reference.rs:
+// 543-589): every stage evaluates all layers, then the coupled demag of all
+// layers at the new points, then the next stage.  The stats pass at the new
+// state also stores each layer's next first slope and predictor, as for one
+// backend.
+template <typename State, typename Real>
+int stack_step_t(Stack &s, double dt, uint64_t nsteps, fullmag_fdm_step_stats *st) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t L = s.layers.size();
+    std::vector<Driver<State, Real>> d;
+    for (Context *c : s.layers) d.emplace_back(*c);
+    auto eval = [&](bool at_m) {
+        std::vector<const void *> src(L);
+        for (size_t l = 0; l < L; ++l) src[l] = at_m ? (const void *)d[l].m() : (const void *)d[l].mt(0);
+        stack_demag_eval(s, src.data(), at_m);
+    };
+    bool have_hd = true;
+    for (Context *c : s.layers) have_hd = have_hd && c->hd_at_m;
+    if (!have_hd) {
+        // a layer's m changed since the last step: every layer's field, and
+        // so every stored predictor, depends on it
+        eval(true);
+        for (Context *c : s.layers) c->pre_valid = false;
+    }
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        for (size_t l = 0; l < L; ++l) {
+            Context &c = *s.layers[l];
+            if (!(c.pre_valid && c.pre_dt == dt))
+                d[l].template stage<State, Real>(d[l].m(), d[l].k(0), d[l].mt(0), {}, 1.0, dt);
+            c.pre_valid = false;
+        }
+        eval(false);
+        for (size_t l = 0; l < L; ++l) d[l].template stage<Real, State>(d[l].mt(0), nullptr, d[l].m(), {{0, 0.5}}, 0.5, dt);
+        eval(true);
+    }
+    for (size_t l = 0; l < L; ++l) {
+        Context &c = *s.layers[l];
+        d[l].stats(d[l].k(0), d[l].mt(0), dt);
+        c.pre_valid = true;
+        c.pre_dt = dt;
+    }
+    Context &c0 = *s.layers[0];
+    if (!hip_ok(hipStreamSynchronize(c0.stream), "stack steps", s.last_error)) return FULLMAG_FDM_ERR_CUDA;
+    if (!hip_ok(hipGetLastError(), "stack steps launch", s.last_error)) return FULLMAG_FDM_ERR_CUDA;
+    s.step_count += nsteps;
+    s.time += dt * (double)nsteps;
+    fullmag_fdm_step_stats sum{};
+    for (size_t l = 0; l < L; ++l) {
+        Context &c = *s.layers[l];
+        timing_resolve(c);
+        c.hd_at_m = true;
+        c.obs_valid = false;
+        c.obs_frozen = false;
+        c.step_count += nsteps;
+        c.time += dt * (double)nsteps;
+        fullmag_fdm_step_stats one{};
+        d[l].fill(&one, dt, 0.0);
+        sum.exchange_energy_joules += one.exchange_energy_joules;
+        sum.demag_energy_joules += one.demag_energy_joules;
+        sum.external_energy_joules += one.external_energy_joules;
+        sum.total_energy_joules += one.total_energy_joules;
+        sum.max_effective_field_amplitude = std::max(sum.max_effective_field_amplitude, one.max_effective_field_amplitude);
+        sum.max_demag_field_amplitude = std::max(sum.max_demag_field_amplitude, one.max_demag_field_amplitude);
+        sum.max_rhs_amplitude = std::max(sum.max_rhs_amplitude, one.max_rhs_amplitude);
+    }
+    timing_resolve(s.conv);
+    if (st) {
+        *st = sum;
+        st->step = s.step_count;
+        st->time_seconds = s.time;
+        st->dt_seconds = dt;
+        st->suggested_next_dt = 0.0;
+        st->wall_time_ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                               std::chrono::steady_clock::now() - t0)
+                               .count();
+    }
+    return FULLMAG_FDM_OK;
+}
+
+int stack_step(Stack &s, double dt, uint64_t n_steps, fullmag_fdm_step_stats *st) {
+    const int mode = s.layers[0]->mode;
+    if (mode == MODE_DOUBLE) return stack_step_t<double, double>(s, dt, n_steps, st);
+    if (mode == MODE_MIXED) return stack_step_t<double, float>(s, dt, n_steps, st);
+    return stack_step_t<float, float>(s, dt, n_steps, st);
+}
+
+bool stack_refresh(Stack &s) {
+    stack_eval_at_m(s);
+    for (Context *c : s.layers) c->pre_valid = false;
+    return hip_ok(hipStreamSynchronize(s.conv.stream), "stack refresh", s.last_error) &&
+           hip_ok(hipGetLastError(), "stack refresh launch", s.last_error);
 }
 
 int integrator_step(Context &c, double dt, fullmag_fdm_step_stats *st) {

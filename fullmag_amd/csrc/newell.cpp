@@ -7,7 +7,9 @@
 // the octant into all eight sign octants of the padded grid on the host; here
 // only the octant is produced and the device mirrors it with the parity signs
 // while it transforms (demag.hip, NewellSource), so the host never holds the
-// padded arrays.
+// padded arrays.  A nonzero z_offset gives the cross-layer kernel of layers
+// that far apart in z (newell.rs:528-651: every z coordinate, of the f/g
+// tables and of the far field, is offset).
 #include "context.hpp"
 
 #include <algorithm>
@@ -128,7 +130,8 @@ void parallel_for(size_t n, F f) {
 
 }  // namespace
 
-void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz, std::vector<double> &out) {
+void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double dz, std::vector<double> &out,
+                   double z_offset) {
     const size_t sx = std::min(nx, kFar) + 2, sy = std::min(ny, kFar) + 2, sz = std::min(nz, kFar) + 2;
     // six base-function tables on offsets -1 .. min(n, 40) along each axis
     Table t[6];
@@ -137,7 +140,7 @@ void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double
         tb.v.assign(sx * sy * sz, 0.0);
     }
     parallel_for(sz, [&](size_t k) {
-        const double z = ((double)k - 1.0) * dz;
+        const double z = ((double)k - 1.0) * dz + z_offset;
         for (size_t j = 0; j < sy; ++j) {
             const double y = ((double)j - 1.0) * dy;
             for (size_t i = 0; i < sx; ++i) {
@@ -162,7 +165,7 @@ void newell_octant(size_t nx, size_t ny, size_t nz, double dx, double dy, double
                 const bool far = i >= kFar || j >= kFar || k >= kFar || i * i + j * j + k * k >= kFar * kFar;
                 double v[6];
                 if (far) {
-                    const double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz;
+                    const double x = (double)i * dx, y = (double)j * dy, z = (double)k * dz + z_offset;
                     v[0] = dip_diag(x, y, z, vol);
                     v[1] = dip_diag(y, x, z, vol);
                     v[2] = dip_diag(z, y, x, vol);

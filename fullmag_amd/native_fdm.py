@@ -168,6 +168,134 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _describe(plan: FdmPlan):
+    """fullmag_fdm_plan_desc and extension options of a plan, as the runner
+    fills the descriptor (native_fdm.rs:90-348), with the host arrays that
+    must outlive the create call."""
+    n = plan.cell_count
+    nx, ny, nz = (int(v) for v in plan.cells)
+    dx, dy, dz = (float(v) for v in plan.cell_size)
+    keep = []  # host arrays that must outlive the create() call
+
+    d = A.PlanDesc()
+    d.grid = A.GridDesc(nx, ny, nz, dx, dy, dz)
+    d.material = A.MaterialDesc(plan.saturation_magnetisation, plan.exchange_stiffness, plan.damping,
+                                plan.gyromagnetic_ratio)
+    prec = _PRECISIONS[plan.precision]
+    d.precision = A.PRECISION_SINGLE if prec == A.PRECISION_MIXED else prec
+    d.integrator = _INTEGRATORS[plan.integrator]
+    d.disable_precession = 1 if plan.disable_precession else 0
+    d.enable_exchange = 1 if plan.enable_exchange else 0
+    d.enable_demag = 1 if plan.enable_demag else 0
+    d.has_external_field = 1 if plan.external_field is not None else 0
+    ext = plan.external_field if plan.external_field is not None else (0.0, 0.0, 0.0)
+    d.external_field_am = (ctypes.c_double * 3)(*[float(v) for v in ext])
+
+    m0 = np.ascontiguousarray(plan.initial_magnetization, dtype=np.float64).reshape(-1)
+    if m0.size != 3 * n:
+        raise RunError(f"initial magnetization has {m0.size} values, expected {3 * n}")
+    keep.append(m0)
+    d.initial_magnetization_xyz = _dp(m0)
+    d.initial_magnetization_len = m0.size
+
+    newell_on_device = 0
+    if plan.enable_demag:
+        spectra = None
+        if isinstance(plan.demag_kernel, np.ndarray):
+            spectra = np.ascontiguousarray(plan.demag_kernel, dtype=np.float64)
+        elif plan.demag_kernel == "spectra":
+            spectra = compute_newell_kernel_spectra(nx, ny, nz, dx, dy, dz)
+        elif plan.demag_kernel == "device":
+            newell_on_device = 1
+        elif plan.demag_kernel not in (None, "projection"):
+            raise RunError(f"unknown demag_kernel option {plan.demag_kernel!r}")
+        if spectra is not None:
+            keep.append(spectra)
+            ptrs = [spectra[c].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in range(6)]
+            (d.demag_kernel_xx_spectrum, d.demag_kernel_yy_spectrum, d.demag_kernel_zz_spectrum,
+             d.demag_kernel_xy_spectrum, d.demag_kernel_xz_spectrum, d.demag_kernel_yz_spectrum) = ptrs
+            d.demag_kernel_spectrum_len = spectra.shape[1]
+
+    if plan.active_mask is not None:
+        mask = np.ascontiguousarray(np.asarray(plan.active_mask).reshape(-1).astype(np.uint8))
+        keep.append(mask)
+        d.active_mask = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        d.active_mask_len = mask.size
+    if plan.region_mask is not None:
+        reg = np.ascontiguousarray(np.asarray(plan.region_mask).reshape(-1).astype(np.uint32))
+        keep.append(reg)
+        d.region_mask = reg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+        d.region_mask_len = reg.size
+        # native_fdm.rs:173-196: diagonal = material A, caller overrides symmetric
+        R = A.MAX_EXCHANGE_REGIONS
+        lut = np.zeros(R * R, dtype=np.float64)
+        lut[np.arange(R) * R + np.arange(R)] = plan.exchange_stiffness
+        for ri, rj, aij in plan.inter_region_exchange:
+            if ri < R and rj < R:
+                lut[ri * R + rj] = aij
+                lut[rj * R + ri] = aij
+        keep.append(lut)
+        d.exchange_lut = _dp(lut)
+        d.exchange_lut_len = lut.size
+    d.boundary_correction = _BOUNDARY.get(plan.boundary_correction, A.BOUNDARY_NONE)
+    d.boundary_phi_floor = plan.boundary_phi_floor
+    d.boundary_delta_min = plan.boundary_delta_min
+    bg = plan.boundary_geometry
+    if bg is not None:
+        def f64(name):
+            v = getattr(bg, name)
+            if v is None or np.asarray(v).size == 0:
+                return None
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+            keep.append(a)
+            return a
+
+        vf = f64("volume_fraction")
+        if vf is not None:
+            d.volume_fraction = _dp(vf)
+            d.volume_fraction_len = vf.size
+        for name in ("face_link_xp", "face_link_xm", "face_link_yp", "face_link_ym", "face_link_zp",
+                     "face_link_zm", "delta_xp", "delta_xm", "delta_yp", "delta_ym", "delta_zp", "delta_zm"):
+            a = f64(name)
+            if a is not None:
+                if a.size != n:
+                    raise RunError(f"boundary_geometry.{name} has {a.size} values, expected {n}")
+                setattr(d, name, _dp(a))
+        tgt = bg.demag_corr_target_idx
+        if tgt is not None and np.asarray(tgt).size > 0:
+            # native_fdm.rs:313-338: the flag follows a non-empty target list
+            rows = np.asarray(tgt).size
+            st = int(bg.demag_corr_stencil_size)
+            t = np.ascontiguousarray(np.asarray(tgt, dtype=np.int32).reshape(-1))
+            src = np.ascontiguousarray(np.asarray(bg.demag_corr_source_idx, dtype=np.int32).reshape(-1))
+            ten = np.ascontiguousarray(np.asarray(bg.demag_corr_tensor, dtype=np.float64).reshape(-1))
+            if src.size != rows * st or ten.size != rows * st * 6:
+                raise RunError(f"demag correction arrays do not match {rows} rows x stencil {st}")
+            keep += [t, src, ten]
+            d.has_demag_boundary_corr = 1
+            d.demag_corr_target_idx = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            d.demag_corr_source_idx = src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            d.demag_corr_tensor = _dp(ten)
+            d.demag_corr_target_count = rows
+            d.demag_corr_stencil_size = st
+    d.adaptive_max_error = plan.adaptive_max_error
+    d.adaptive_dt_min = plan.adaptive_dt_min
+    d.adaptive_dt_max = plan.adaptive_dt_max
+    d.adaptive_headroom = plan.adaptive_headroom
+
+    opt = A.ExtOptions()
+    opt.struct_size = ctypes.sizeof(A.ExtOptions)
+    opt.precision_override = prec
+    opt.newell_on_device = newell_on_device
+    if plan.ms_cell is not None:
+        ms = np.ascontiguousarray(plan.ms_cell, dtype=np.float64).reshape(-1)
+        keep.append(ms)
+        opt.ms_cell = _dp(ms)
+        opt.ms_cell_len = ms.size
+
+    return d, opt, keep
+
+
 class NativeFdmBackend:
     """RAII owner of one backend handle (native_fdm.rs:44-47, 708-715)."""
 
@@ -175,127 +303,7 @@ class NativeFdmBackend:
         L = A.lib()
         self.plan = plan
         self.cell_count = plan.cell_count
-        n = self.cell_count
-        nx, ny, nz = (int(v) for v in plan.cells)
-        dx, dy, dz = (float(v) for v in plan.cell_size)
-        keep = []  # host arrays that must outlive the create() call
-
-        d = A.PlanDesc()
-        d.grid = A.GridDesc(nx, ny, nz, dx, dy, dz)
-        d.material = A.MaterialDesc(plan.saturation_magnetisation, plan.exchange_stiffness, plan.damping,
-                                    plan.gyromagnetic_ratio)
-        prec = _PRECISIONS[plan.precision]
-        d.precision = A.PRECISION_SINGLE if prec == A.PRECISION_MIXED else prec
-        d.integrator = _INTEGRATORS[plan.integrator]
-        d.disable_precession = 1 if plan.disable_precession else 0
-        d.enable_exchange = 1 if plan.enable_exchange else 0
-        d.enable_demag = 1 if plan.enable_demag else 0
-        d.has_external_field = 1 if plan.external_field is not None else 0
-        ext = plan.external_field if plan.external_field is not None else (0.0, 0.0, 0.0)
-        d.external_field_am = (ctypes.c_double * 3)(*[float(v) for v in ext])
-
-        m0 = np.ascontiguousarray(plan.initial_magnetization, dtype=np.float64).reshape(-1)
-        if m0.size != 3 * n:
-            raise RunError(f"initial magnetization has {m0.size} values, expected {3 * n}")
-        keep.append(m0)
-        d.initial_magnetization_xyz = _dp(m0)
-        d.initial_magnetization_len = m0.size
-
-        newell_on_device = 0
-        if plan.enable_demag:
-            spectra = None
-            if isinstance(plan.demag_kernel, np.ndarray):
-                spectra = np.ascontiguousarray(plan.demag_kernel, dtype=np.float64)
-            elif plan.demag_kernel == "spectra":
-                spectra = compute_newell_kernel_spectra(nx, ny, nz, dx, dy, dz)
-            elif plan.demag_kernel == "device":
-                newell_on_device = 1
-            elif plan.demag_kernel not in (None, "projection"):
-                raise RunError(f"unknown demag_kernel option {plan.demag_kernel!r}")
-            if spectra is not None:
-                keep.append(spectra)
-                ptrs = [spectra[c].ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for c in range(6)]
-                (d.demag_kernel_xx_spectrum, d.demag_kernel_yy_spectrum, d.demag_kernel_zz_spectrum,
-                 d.demag_kernel_xy_spectrum, d.demag_kernel_xz_spectrum, d.demag_kernel_yz_spectrum) = ptrs
-                d.demag_kernel_spectrum_len = spectra.shape[1]
-
-        if plan.active_mask is not None:
-            mask = np.ascontiguousarray(np.asarray(plan.active_mask).reshape(-1).astype(np.uint8))
-            keep.append(mask)
-            d.active_mask = mask.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
-            d.active_mask_len = mask.size
-        if plan.region_mask is not None:
-            reg = np.ascontiguousarray(np.asarray(plan.region_mask).reshape(-1).astype(np.uint32))
-            keep.append(reg)
-            d.region_mask = reg.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
-            d.region_mask_len = reg.size
-            # native_fdm.rs:173-196: diagonal = material A, caller overrides symmetric
-            R = A.MAX_EXCHANGE_REGIONS
-            lut = np.zeros(R * R, dtype=np.float64)
-            lut[np.arange(R) * R + np.arange(R)] = plan.exchange_stiffness
-            for ri, rj, aij in plan.inter_region_exchange:
-                if ri < R and rj < R:
-                    lut[ri * R + rj] = aij
-                    lut[rj * R + ri] = aij
-            keep.append(lut)
-            d.exchange_lut = _dp(lut)
-            d.exchange_lut_len = lut.size
-        d.boundary_correction = _BOUNDARY.get(plan.boundary_correction, A.BOUNDARY_NONE)
-        d.boundary_phi_floor = plan.boundary_phi_floor
-        d.boundary_delta_min = plan.boundary_delta_min
-        bg = plan.boundary_geometry
-        if bg is not None:
-            def f64(name):
-                v = getattr(bg, name)
-                if v is None or np.asarray(v).size == 0:
-                    return None
-                a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
-                keep.append(a)
-                return a
-
-            vf = f64("volume_fraction")
-            if vf is not None:
-                d.volume_fraction = _dp(vf)
-                d.volume_fraction_len = vf.size
-            for name in ("face_link_xp", "face_link_xm", "face_link_yp", "face_link_ym", "face_link_zp",
-                         "face_link_zm", "delta_xp", "delta_xm", "delta_yp", "delta_ym", "delta_zp", "delta_zm"):
-                a = f64(name)
-                if a is not None:
-                    if a.size != n:
-                        raise RunError(f"boundary_geometry.{name} has {a.size} values, expected {n}")
-                    setattr(d, name, _dp(a))
-            tgt = bg.demag_corr_target_idx
-            if tgt is not None and np.asarray(tgt).size > 0:
-                # native_fdm.rs:313-338: the flag follows a non-empty target list
-                rows = np.asarray(tgt).size
-                st = int(bg.demag_corr_stencil_size)
-                t = np.ascontiguousarray(np.asarray(tgt, dtype=np.int32).reshape(-1))
-                src = np.ascontiguousarray(np.asarray(bg.demag_corr_source_idx, dtype=np.int32).reshape(-1))
-                ten = np.ascontiguousarray(np.asarray(bg.demag_corr_tensor, dtype=np.float64).reshape(-1))
-                if src.size != rows * st or ten.size != rows * st * 6:
-                    raise RunError(f"demag correction arrays do not match {rows} rows x stencil {st}")
-                keep += [t, src, ten]
-                d.has_demag_boundary_corr = 1
-                d.demag_corr_target_idx = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-                d.demag_corr_source_idx = src.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
-                d.demag_corr_tensor = _dp(ten)
-                d.demag_corr_target_count = rows
-                d.demag_corr_stencil_size = st
-        d.adaptive_max_error = plan.adaptive_max_error
-        d.adaptive_dt_min = plan.adaptive_dt_min
-        d.adaptive_dt_max = plan.adaptive_dt_max
-        d.adaptive_headroom = plan.adaptive_headroom
-
-        opt = A.ExtOptions()
-        opt.struct_size = ctypes.sizeof(A.ExtOptions)
-        opt.precision_override = prec
-        opt.newell_on_device = newell_on_device
-        if plan.ms_cell is not None:
-            ms = np.ascontiguousarray(plan.ms_cell, dtype=np.float64).reshape(-1)
-            keep.append(ms)
-            opt.ms_cell = _dp(ms)
-            opt.ms_cell_len = ms.size
-
+        d, opt, keep = _describe(plan)
         self._handle = L.fullmag_fdm_ext_backend_create(ctypes.byref(d), ctypes.byref(opt))
         if not self._handle:
             raise RunError("fullmag_fdm_backend_create returned NULL")
@@ -532,6 +540,121 @@ class BatchRun:
     def close(self) -> None:
         if getattr(self, "_h", None):
             A.lib().fullmag_fdm_ext_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+@dataclass
+class FdmLayerPlan:
+    """FdmLayerPlanIR (crates/fullmag-ir/src/lib.rs:414-427): one magnet of a
+    multilayer plan on its own grid."""
+
+    magnet_name: str
+    native_grid: Sequence[int]
+    native_cell_size: Sequence[float]
+    native_origin: Sequence[float]
+    initial_magnetization: np.ndarray
+    saturation_magnetisation: float
+    exchange_stiffness: float
+    damping: float
+    native_active_mask: Optional[np.ndarray] = None
+
+
+@dataclass
+class FdmMultilayerPlan:
+    """FdmMultilayerPlanIR (crates/fullmag-ir/src/lib.rs:395-411): layers
+    coupled by demag on the common convolution grid `common_cells` of cell
+    size `convolution_cell_size`, stepped together with Heun."""
+
+    common_cells: Sequence[int]
+    convolution_cell_size: Sequence[float]
+    layers: list
+    enable_exchange: bool = True
+    enable_demag: bool = True
+    external_field: Optional[Sequence[float]] = None  # A/m, every layer
+    gyromagnetic_ratio: float = 2.211e5
+    precision: str = "double"
+    disable_precession: bool = False
+
+
+class _StackLayer(NativeFdmBackend):
+    """A layer of a MultilayerStack: the backend handle belongs to the stack."""
+
+    def __init__(self, handle, plan: FdmPlan, stack):
+        self._handle = handle
+        self.plan = plan
+        self.cell_count = plan.cell_count
+        self.precision = plan.precision
+        self._stack = stack  # keeps the stack, and so the handle, alive
+
+    def close(self) -> None:
+        self._handle = None
+
+
+class MultilayerStack:
+    """A multilayer plan on the device (fullmag_fdm_ext_stack_*): one backend
+    per layer for exchange, Zeeman, torque and stats, the demag coupling of
+    all layers (self and shifted kernels, push/pull transfers) and Heun steps
+    of the whole stack, as crates/fullmag-runner/src/multilayer_reference.rs
+    advances a FdmMultilayerPlanIR."""
+
+    def __init__(self, plan: FdmMultilayerPlan):
+        L = A.lib()
+        self.plan = plan
+        self.layer_plans = [
+            FdmPlan(cells=tuple(int(v) for v in lp.native_grid), cell_size=tuple(lp.native_cell_size),
+                    initial_magnetization=lp.initial_magnetization,
+                    saturation_magnetisation=lp.saturation_magnetisation,
+                    exchange_stiffness=lp.exchange_stiffness, damping=lp.damping,
+                    gyromagnetic_ratio=plan.gyromagnetic_ratio, precision=plan.precision, integrator="heun",
+                    enable_exchange=plan.enable_exchange, enable_demag=False,
+                    external_field=plan.external_field, active_mask=lp.native_active_mask,
+                    disable_precession=plan.disable_precession)
+            for lp in plan.layers]
+        described = [_describe(p) for p in self.layer_plans]
+        n = len(described)
+        ptrs = (ctypes.POINTER(A.PlanDesc) * n)(*[ctypes.pointer(d) for d, _, _ in described])
+        origins = np.ascontiguousarray([list(lp.native_origin) for lp in plan.layers], dtype=np.float64)
+        sd = A.StackDesc()
+        sd.struct_size = ctypes.sizeof(A.StackDesc)
+        sd.layer_count = n
+        sd.layers = ptrs
+        sd.layer_origins = _dp(origins)
+        sd.common_cells = (ctypes.c_uint32 * 3)(*[int(v) for v in plan.common_cells])
+        sd.convolution_cell_size = (ctypes.c_double * 3)(*[float(v) for v in plan.convolution_cell_size])
+        sd.enable_demag = 1 if plan.enable_demag else 0
+        sd.precision_override = _PRECISIONS[plan.precision]
+        self._h = L.fullmag_fdm_ext_stack_create(ctypes.byref(sd))
+        if not self._h:
+            raise RunError("fullmag_fdm_ext_stack_create returned NULL")
+        err = L.fullmag_fdm_ext_stack_last_error(self._h)
+        if err is not None:
+            msg = err.decode(errors="replace")
+            self.close()
+            raise RunError(msg)
+        self.layers = [_StackLayer(L.fullmag_fdm_ext_stack_layer(self._h, i), p, self)
+                       for i, p in enumerate(self.layer_plans)]
+
+    def step(self, dt: float, n_steps: int = 1) -> StepStats:
+        s = A.StepStats()
+        rc = A.lib().fullmag_fdm_ext_stack_step(self._h, float(dt), int(n_steps), ctypes.byref(s))
+        if rc != A.FULLMAG_FDM_OK:
+            err = A.lib().fullmag_fdm_ext_stack_last_error(self._h)
+            raise RunError(err.decode(errors="replace") if err else f"stack_step failed ({rc})")
+        return StepStats.from_raw(s)
+
+    def refresh(self) -> None:
+        if A.lib().fullmag_fdm_ext_stack_refresh(self._h) != A.FULLMAG_FDM_OK:
+            err = A.lib().fullmag_fdm_ext_stack_last_error(self._h)
+            raise RunError(err.decode(errors="replace") if err else "stack_refresh failed")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            for layer in getattr(self, "layers", []):
+                layer._handle = None
+            A.lib().fullmag_fdm_ext_stack_destroy(self._h)
             self._h = None
 
     def __del__(self):

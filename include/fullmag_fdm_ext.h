@@ -149,6 +149,53 @@ int fullmag_fdm_ext_copy_field_preview_noalloc(fullmag_fdm_backend *handle, full
                                                uint32_t z_origin, uint32_t z_stride, int out_f64, void *out_xyz,
                                                uint64_t out_len);
 
+/* Multilayer stacks: layers on their own grids (FdmMultilayerPlanIR,
+ * crates/fullmag-ir/src/lib.rs:395-427) coupled by demag on a common
+ * convolution grid, with the self kernel for a layer with itself and the
+ * shifted kernel of crates/fullmag-fdm-demag/src/shifted_kernel.rs between
+ * layers whose origins differ in z, native grids other than the common one
+ * going through the push/pull transfers of transfer.rs, all layers stepped
+ * together with Heun (crates/fullmag-runner/src/multilayer_reference.rs).
+ * The reference runs this coupling on the host around per-layer backends;
+ * here the whole step runs on the device.  No counterpart in the reference
+ * ABI; for runners of multilayer plans. */
+typedef struct fullmag_fdm_stack fullmag_fdm_stack;
+
+typedef struct {
+    uint32_t struct_size;                        /* sizeof(fullmag_fdm_ext_stack_desc) */
+    uint32_t layer_count;
+    const fullmag_fdm_plan_desc *const *layers;  /* per layer: grid, cell size, material, mask, m0,
+                                                    exchange, external field, precision; integrator
+                                                    must be Heun; its enable_demag and kernel spectra
+                                                    are not used; boundary tiers are refused */
+    const double *layer_origins;                 /* 3 per layer: native origin [m] (z sets the shifts) */
+    uint32_t common_cells[3];                    /* convolution grid */
+    uint32_t reserved;
+    double convolution_cell_size[3];             /* [m] */
+    int32_t enable_demag;
+    int32_t precision_override;                  /* 0: the layers' precision; else 1, 2 or 3 */
+} fullmag_fdm_ext_stack_desc;
+
+/* A failed create returns a handle whose stack_last_error is set (NULL only
+ * when the handle itself cannot be allocated). */
+fullmag_fdm_stack *fullmag_fdm_ext_stack_create(const fullmag_fdm_ext_stack_desc *desc);
+const char *fullmag_fdm_ext_stack_last_error(fullmag_fdm_stack *stack);
+
+/* n_steps fixed-dt Heun steps of every layer; one stats record for the
+ * stack: energies summed over the layers, maxima over the layers. */
+int fullmag_fdm_ext_stack_step(fullmag_fdm_stack *stack, double dt_seconds, uint64_t n_steps,
+                               fullmag_fdm_step_stats *out_stats);
+
+/* Recomputes every layer's demag field at its present m (after uploads). */
+int fullmag_fdm_ext_stack_refresh(fullmag_fdm_stack *stack);
+
+/* Backend handle of layer `index`, owned by the stack: the copy_field,
+ * preview, snapshot, upload, refresh and device-info entry points work on it;
+ * step, step_many, batches, the adjoint and schedule changes are refused, and
+ * backend_destroy on it does nothing. */
+fullmag_fdm_backend *fullmag_fdm_ext_stack_layer(fullmag_fdm_stack *stack, uint32_t index);
+void fullmag_fdm_ext_stack_destroy(fullmag_fdm_stack *stack);
+
 #ifdef __cplusplus
 }
 #endif

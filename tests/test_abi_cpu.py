@@ -72,3 +72,33 @@ def test_random_unit_vectors_match_oracle_bit_for_bit(oracle_mod):
     for seed in (1, 42, 2 ** 63 + 5):
         np.testing.assert_array_equal(generate_random_unit_vectors(seed, 500),
                                       oracle_mod.random_unit_vectors(seed, 500))
+
+
+def test_extension_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of fullmag_fdm_ext_options and
+    fullmag_fdm_ext_stack_desc against the C compiler's layout of the header."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "fullmag_fdm_ext.h"\n'
+        "int main(void) {\n"
+        '  printf("%zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(fullmag_fdm_ext_options),\n'
+        "         sizeof(fullmag_fdm_ext_stack_desc), offsetof(fullmag_fdm_ext_stack_desc, layers),\n"
+        "         offsetof(fullmag_fdm_ext_stack_desc, layer_origins), offsetof(fullmag_fdm_ext_stack_desc, common_cells),\n"
+        "         offsetof(fullmag_fdm_ext_stack_desc, convolution_cell_size),\n"
+        "         offsetof(fullmag_fdm_ext_stack_desc, enable_demag),\n"
+        "         offsetof(fullmag_fdm_ext_stack_desc, precision_override));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    S = A.StackDesc
+    want = [ctypes.sizeof(A.ExtOptions), ctypes.sizeof(S), S.layers.offset, S.layer_origins.offset,
+            S.common_cells.offset, S.convolution_cell_size.offset, S.enable_demag.offset,
+            S.precision_override.offset]
+    assert got == want
