@@ -438,10 +438,113 @@ def adjoint_film(fa, torch, args):
                     "max_rel_dev_g_ms_vs_fp64": mixed_vs_fp64}})
 
 
+def stack_trilayer(fa, torch, args):
+    """Three 512x512x1 films of 4x4x1 nm at z = 0, 2 and 4 nm (1 nm spacers),
+    Ms 8e5 / 6e5 / 8e5 A/m, A 1.3e-11 J/m, alpha 0.02, seeded random m0,
+    B = 10 mT along x, coupled by demag on the common 512x512x1 grid (self
+    kernel and shifted kernels for +-2 and +-4 nm), mixed-precision Heun
+    dt 5e-14 s; a step is one fullmag_fdm_ext_stack_step with its stats.
+    Beside it: the same three films stepped as independent backends (no
+    coupling), and the oracle's stack on the host at 3 x 128x128 (one step)."""
+    nx, ny = 512, 512
+    cell = (4e-9, 4e-9, 1e-9)
+    h = (0.01 / MU0, 0.0, 0.0)
+    specs = [(0.0, 8e5, 11), (2e-9, 6e5, 12), (4e-9, 8e5, 13)]
+
+    def plan(n_x, n_y, prec):
+        layers = [fa.FdmLayerPlan(magnet_name=f"L{i}", native_grid=(n_x, n_y, 1), native_cell_size=cell,
+                                  native_origin=(0.0, 0.0, z), initial_magnetization=fa.generate_random_unit_vectors(
+                                      seed, n_x * n_y), saturation_magnetisation=ms, exchange_stiffness=1.3e-11,
+                                  damping=0.02) for i, (z, ms, seed) in enumerate(specs)]
+        return fa.FdmMultilayerPlan(common_cells=(n_x, n_y, 1), convolution_cell_size=cell, layers=layers,
+                                    external_field=h, precision=prec)
+
+    dt = 5e-14
+    cells = 3 * nx * ny
+    stack = fa.MultilayerStack(plan(nx, ny, "mixed"))
+    for _ in range(args.warmup):
+        stack.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s = stack.step(dt)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    stack.close()
+    # the same films as three independent backends, each with its own demag
+    films = [fa.NativeFdmBackend(fa.FdmPlan(cells=(nx, ny, 1), cell_size=cell, initial_magnetization=lp.initial_magnetization,
+                                            saturation_magnetisation=lp.saturation_magnetisation,
+                                            exchange_stiffness=1.3e-11, damping=0.02, external_field=h,
+                                            precision="mixed", demag_kernel="device"))
+             for lp in plan(nx, ny, "mixed").layers]
+    el_f = sum(time_steps(b, args.steps, args.warmup, dt) for b in films)
+    for b in films:
+        b.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        cn = 128
+        probs = [O.OracleProblem(O.OracleConfig(nx=cn, ny=cn, nz=1, dx=cell[0], dy=cell[1], dz=cell[2], ms=ms,
+                                                a=1.3e-11, alpha=0.02, demag=False, h_ext=h),
+                                 O.random_unit_vectors(seed, cn * cn)) for _, ms, seed in specs]
+        st = O.OracleStack(probs, [(0.0, 0.0, z) for z, _, _ in specs], (cn, cn, 1), cell)
+        t1 = time.perf_counter()
+        st.step(dt)
+        el_c = time.perf_counter() - t1
+        cpu = {"value": 3 * cn * cn / el_c, "unit": "cell-steps/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"1 Heun step of a 3 x {cn}x{cn}x1 stack (FP64 oracle, OpenMP)"}
+    line("Heun cell-steps/s of a coupled three-layer stack (mixed precision)", cells * args.steps / el,
+         "cell-steps/s", args.steps, args.warmup, 1e3 * el / args.steps, "f32 fields / f64 state",
+         {"workload": "stack of three 512x512x1 films, 4x4x1 nm cells, z = 0 / 2 / 4 nm, Ms 8e5 / 6e5 / 8e5, "
+                      "A 1.3e-11, alpha 0.02, B 10 mT x, seeded m0, Heun dt 5e-14 s, demag on the common grid",
+          "cells": cells, "independent_films_ms_per_step": 1e3 * el_f / args.steps,
+          "coupling_cost_ratio": el / el_f, "e_total": s.e_total},
+         cpu_baseline=cpu)
+
+
+def tiers_film(fa, torch, args):
+    """The film workload's grid (512x512x8 cells of 4x4x2 nm) holding a
+    cylinder of radius 860 nm sampled by the planner's rules
+    (fullmag_amd.boundary): T0 face-link exchange ("volume"), volume-fraction
+    demag source and the sparse correction rows, active mask phi > 0,
+    Ms 8.6e5, A 1.3e-11, alpha 0.02, seeded m0, mixed-precision Heun
+    dt 5e-14 s.  Beside it: the same masked cylinder without tiers."""
+    from fullmag_amd import boundary
+
+    nx, ny, nz = 512, 512, 8
+    cell = (4e-9, 4e-9, 2e-9)
+    t0 = time.perf_counter()
+    g = boundary.compute_boundary_geometry(boundary.cylinder_sdf(860e-9, nx * cell[0] / 2, ny * cell[1] / 2), nx, ny,
+                                           nz, *cell, False)
+    geo_s = time.perf_counter() - t0
+    mask = (g.volume_fraction > 0).astype(np.uint8)
+    m0 = fa.generate_random_unit_vectors(21, nx * ny * nz)
+    base = dict(cells=(nx, ny, nz), cell_size=cell, initial_magnetization=m0, saturation_magnetisation=8.6e5,
+                exchange_stiffness=1.3e-11, damping=0.02, precision="mixed", demag_kernel="device", active_mask=mask)
+    dt = 5e-14
+    res = {}
+    for key, kw in (("tiers", dict(boundary_correction="volume", boundary_geometry=g)), ("plain", {})):
+        b = fa.NativeFdmBackend(fa.FdmPlan(**base, **kw))
+        res[key] = time_steps(b, args.steps, args.warmup, dt)
+        b.close()
+    n = nx * ny * nz
+    line("Heun cell-steps/s, 512x512x8 cylinder with the T0 boundary tier (mixed precision)",
+         n * args.steps / res["tiers"], "cell-steps/s", args.steps, args.warmup, 1e3 * res["tiers"] / args.steps,
+         "f32 fields / f64 state",
+         {"workload": "512x512x8 cells of 4x4x2 nm holding a cylinder of radius 860 nm: T0 face links, phi-weighted "
+                      "demag source, sparse correction rows from the planner's sampling, mask phi > 0, Ms 8.6e5, "
+                      "A 1.3e-11, alpha 0.02, seeded m0, Heun dt 5e-14 s",
+          "correction_rows": int(g.demag_corr_target_idx.size), "stencil": int(g.demag_corr_stencil_size),
+          "geometry_s": geo_s, "plain_masked_ms_per_step": 1e3 * res["plain"] / args.steps,
+          "tier_cost_ratio": res["tiers"] / res["plain"]})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["film512", "sweep96", "dp45-block", "adjoint-film"], default="film512")
+    ap.add_argument("--workload", choices=["film512", "sweep96", "dp45-block", "adjoint-film", "stack-trilayer",
+                                           "tiers-film"], default="film512")
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fp64-steps", type=int, default=100)
@@ -473,7 +576,8 @@ def main():
             torch.cuda.synchronize()
             del src, dst
     if args.workload != "film512":
-        {"sweep96": sweep96, "dp45-block": dp45_block, "adjoint-film": adjoint_film}[args.workload](fa, torch, args)
+        {"sweep96": sweep96, "dp45-block": dp45_block, "adjoint-film": adjoint_film, "stack-trilayer": stack_trilayer,
+         "tiers-film": tiers_film}[args.workload](fa, torch, args)
         return
     dt = 5e-14
     base, extra = workload()

@@ -1399,20 +1399,23 @@ struct BoundaryCorrArgs {
 
 // Sparse near-field demag correction of the boundary tiers
 // (native/backends/fdm/src/demag_boundary_fp64.cu:23-76): the field at a
-// target cell gains -sum_s dN(t, s) . (phi_s Ms_s m_s), dN symmetric with the
-// components Nxx, Nxy, Nxz, Nyy, Nyz, Nzz; sources < 0 are skipped.  One lane
-// per target cell adds that cell's rows in plan order, so a cell named by
-// several rows gets every row, in FP64, without atomics.
+// target cell gains -sum_s dN(t, s) . (phi_s Ms_s m_s) per row, dN symmetric
+// with the components Nxx, Nxy, Nxz, Nyy, Nyz, Nzz; sources < 0 are skipped.
+// One wavefront per target cell: its lanes share each row's stencil entries
+// (coalesced reads of the row's sources and tensors), a butterfly reduction
+// gives the row's sum in FP64, and the cell's rows are added in plan order,
+// so a cell named by several rows gets all of them without atomics.
 template <typename Real, typename S>
 __global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Real, S> a) {
-    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
-    if (g >= a.groups) return;
+    const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (g >= a.groups) return;  // a whole wavefront
     const int64_t ti = a.cell[g], n = a.n;
     double h[3] = {double(a.out[ti]), double(a.out[n + ti]), double(a.out[2 * n + ti])};
     for (uint32_t e = a.off[g]; e < a.off[g + 1]; ++e) {
         const uint64_t r0 = (uint64_t)a.row[e] * a.stencil;
         double cx = 0.0, cy = 0.0, cz = 0.0;
-        for (uint32_t s = 0; s < a.stencil; ++s) {
+        for (uint32_t s = lane; s < a.stencil; s += 64u) {
             const int32_t si = a.src[r0 + s];
             if (si < 0) continue;
             const double *nn = a.tensor + (r0 + s) * 6;
@@ -1422,13 +1425,21 @@ __global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Rea
             cy -= nn[1] * mx + nn[3] * my + nn[4] * mz;
             cz -= nn[2] * mx + nn[4] * my + nn[5] * mz;
         }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            cx += __shfl_xor(cx, off);
+            cy += __shfl_xor(cy, off);
+            cz += __shfl_xor(cz, off);
+        }
         h[0] += cx;
         h[1] += cy;
         h[2] += cz;
     }
-    a.out[ti] = Real(h[0]);
-    a.out[n + ti] = Real(h[1]);
-    a.out[2 * n + ti] = Real(h[2]);
+    if (lane == 0) {
+        a.out[ti] = Real(h[0]);
+        a.out[n + ti] = Real(h[1]);
+        a.out[2 * n + ti] = Real(h[2]);
+    }
 }
 
 template <typename Real, typename S>
@@ -1651,7 +1662,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         ba.groups = c.corr_groups;
         ba.stencil = c.corr_stencil;
         Timed t(c, "demag_boundary");
-        const dim3 g1((unsigned)cdiv((int64_t)c.corr_groups, 256));
+        const dim3 g1((unsigned)cdiv((int64_t)c.corr_groups, 4));  // one wavefront per target cell
         launch_rec(c, nullptr, g1, dim3(256), 0, 1, ba, [&] {
             hipLaunchKernelGGL((boundary_corr_kernel<Real, S>), g1, dim3(256), 0, c.stream, ba);
         });

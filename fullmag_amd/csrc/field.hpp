@@ -167,59 +167,61 @@ __device__ __forceinline__ void exchange_col(const FieldArgs &f, const S *__rest
 template <typename C, typename S>
 __device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__restrict__ m, int64_t i, int x, int y,
                                               int z, const S cc[3], const S vzm[3], const S vzp[3], C h[3]) {
-    h[0] = h[1] = h[2] = C(0);
-    const double phi0 = f.phi[i];
-    if (phi0 <= 0.0 || (cc[0] == S(0) && cc[1] == S(0) && cc[2] == S(0))) return;
     const int64_t n = f.n;
     // sides in the order -x, +x, -y, +y, -z, +z; side s reads weight s ^ 1
-    // of the xp, xm, yp, ym, zp, zm arrays
+    // of the xp, xm, yp, ym, zp, zm arrays.  Every load is issued
+    // unconditionally (clamped to the cell itself beyond the grid) and a
+    // side that does not count enters with coefficient zero, which adds
+    // exactly nothing.
     const bool in[6] = {x > 0, x < f.nx - 1, y > 0, y < f.ny - 1, f.nz > 1 && z > 0, f.nz > 1 && z < f.nz - 1};
     const int64_t off[4] = {-1, 1, -(int64_t)f.nx, (int64_t)f.nx};
     S v[6][3];
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
+    for (int s = 0; s < 4; ++s) {
+        const int64_t j = in[s] ? i + off[s] : i;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) v[s][q] = in[s] ? m[q * n + i + off[s]] : cc[q];
+        for (int q = 0; q < 3; ++q) v[s][q] = m[q * n + j];
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         v[4][q] = vzm[q];
         v[5][q] = vzp[q];
     }
+    double wt[6];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) wt[s] = f.tier_w[(int64_t)(s ^ 1) * n + i];
+    const double phi0 = f.phi[i];
     const C c0[3] = {C(cc[0]), C(cc[1]), C(cc[2])};
     const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * f.ms_cell[i]) : f.tier_pref;
     C b[3] = {C(0), C(0), C(0)};
+    double scale;
     if (f.tier_ex == 1) {
         const double id2[3] = {f.idx2, f.idy2, f.idz2};
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            const double fl = f.tier_w[(int64_t)(s ^ 1) * n + i];
-            if (fl > 0.0 && in[s]) {
-                const C k = C(fl * id2[s >> 1]);
+            const C k = (wt[s] > 0.0 && in[s]) ? C(wt[s] * id2[s >> 1]) : C(0);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
-            }
+            for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
         }
-        const double inv_phi = 1.0 / (phi0 > f.phi_floor ? phi0 : f.phi_floor);
-        const C scale = C(pre * inv_phi);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) h[q] = scale * b[q];
+        scale = pre * (1.0 / (phi0 > f.phi_floor ? phi0 : f.phi_floor));
     } else {
         double dl[6];
 #pragma unroll
-        for (int s = 0; s < 6; ++s) dl[s] = fmax(f.tier_w[(int64_t)(s ^ 1) * n + i], f.delta_min);
+        for (int s = 0; s < 6; ++s) dl[s] = fmax(wt[s], f.delta_min);
 #pragma unroll
         for (int s = 0; s < 6; ++s) {
-            if (in[s] && dl[s] > 0.0 && !(v[s][0] == S(0) && v[s][1] == S(0) && v[s][2] == S(0))) {
-                const double d = f.cell[s >> 1];
-                const C k = C(2.0 / (d * (d + dl[s ^ 1])));
+            const bool nz = !(v[s][0] == S(0) && v[s][1] == S(0) && v[s][2] == S(0));
+            const double d = f.cell[s >> 1];
+            const C k = (in[s] && dl[s] > 0.0 && nz) ? C(2.0 / (d * (d + dl[s ^ 1]))) : C(0);
 #pragma unroll
-                for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
-            }
+            for (int q = 0; q < 3; ++q) b[q] += k * (C(v[s][q]) - c0[q]);
         }
-        const C scale = C(pre);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) h[q] = scale * b[q];
+        scale = pre;
     }
+    // empty cells (phi <= 0) and zero m get no field
+    const bool live = phi0 > 0.0 && !(cc[0] == S(0) && cc[1] == S(0) && cc[2] == S(0));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) h[q] = live ? C(scale) * b[q] : C(0);
 }
 
 // exchange_tier at cell i with every neighbour loaded from memory.
@@ -348,8 +350,11 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
     if (act) {
         if (f.exchange) {
             Src he[3];
-            if (TIER) exchange_tier<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
-            else exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+            if (TIER) {
+                exchange_tier<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+            } else {
+                exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
         }
