@@ -213,7 +213,8 @@ struct Context {
 // stream) coupled by demag on a common convolution grid
 // (crates/fullmag-engine/src/multilayer.rs).  Spectra are laid out
 // [layer][c][plane][kx][ky] with ky < py; the pair kernels
-// [kernel][6][plane][kx][ky] (xx, yy, zz, xy, xz, yz).
+// [kernel][6][plane][kx][ky] (xx, yy, zz, xy, xz, yz), one real each (the
+// transform of a component is purely real or purely imaginary).
 struct Stack {
     std::vector<Context *> layers;
     std::vector<double> oz;   // native origin z per layer [m]
@@ -222,11 +223,12 @@ struct Stack {
     int planes = 1;           // pz (3-D) or 1 (thin layers, 2-D transform)
     uint64_t layer_spec = 0;  // complex elements per layer spectrum
     void *spec_m = nullptr, *spec_h = nullptr;
-    void *kern = nullptr;
-    uint64_t kern_stride = 0;  // complex elements per pair kernel
+    void *kern = nullptr;      // per kernel 6 components of one real per k-point
+    uint64_t kern_stride = 0;  // reals per pair kernel
     int nkern = 0;
     std::vector<int> pair;    // kernel of (dst, src) at dst * L + src
     int *d_pair = nullptr;
+    uint32_t *d_kimag = nullptr;  // per kernel: bit c set when component c is imaginary
     // layers on another grid than the common one (push/pull transfer):
     // conv-grid m (FP64) and H (Real); null where the grids agree
     std::vector<void *> conv_m, conv_h;

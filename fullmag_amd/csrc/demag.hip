@@ -2062,40 +2062,78 @@ template <typename Real>
 struct StackTensorArgs {
     const cplx<Real> *m;
     cplx<Real> *h;
-    const cplx<Real> *kern;
-    const int *pair;
+    const Real *kern;       // per kernel 6 components of `points` reals
+    const int *pair;        // kernel of (dst, src)
+    const uint32_t *kimag;  // per kernel: bit c set when component c is imaginary
     int layers;
     int64_t points;        // planes * hx * py, the same linear index in spectra and kernels
     int64_t layer_stride;  // complex elements per layer spectrum
     int64_t comp_stride;   // complex elements per component of a spectrum (= points)
-    int64_t kern_stride;   // complex elements per pair kernel (6 * points)
+    int64_t kern_stride;   // reals per pair kernel (6 * points)
 };
 
+// acc += k m for a purely real (im = false) or purely imaginary (k i) kernel
+// entry.
 template <typename T>
-__device__ __forceinline__ void cmad(cplx<T> &acc, const cplx<T> &a, const cplx<T> &b) {
-    acc.x += a.x * b.x - a.y * b.y;
-    acc.y += a.x * b.y + a.y * b.x;
+__device__ __forceinline__ void kmad(cplx<T> &acc, T k, bool im, const cplx<T> &m) {
+    if (im) {
+        acc.x -= k * m.y;
+        acc.y += k * m.x;
+    } else {
+        acc.x += k * m.x;
+        acc.y += k * m.y;
+    }
 }
 
 // One lane per k-point; for every destination layer the sources are summed
 // in layer order (the order of the reference's kernel pairs for that
-// destination).  Kernel components: xx, yy, zz, xy, xz, yz.
-template <typename Real>
+// destination).  Kernel components: xx, yy, zz, xy, xz, yz.  Every padded
+// Newell component has a definite parity along each axis, so its transform
+// is purely real (an even number of odd axes) or purely imaginary (odd): one
+// real per component is stored.  LT > 0: that many layers, their sources held
+// in registers; 0: any count, sources reloaded per destination.
+template <typename Real, int LT>
 __global__ void __launch_bounds__(256) stack_tensor_kernel(StackTensorArgs<Real> a) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= a.points) return;
     const int64_t cs = a.comp_stride;
-    for (int d = 0; d < a.layers; ++d) {
-        cplx<Real> hx{Real(0), Real(0)}, hy{Real(0), Real(0)}, hz{Real(0), Real(0)};
-        for (int s = 0; s < a.layers; ++s) {
-            const cplx<Real> *K = a.kern + (int64_t)a.pair[d * a.layers + s] * a.kern_stride + i;
+    const int L = LT > 0 ? LT : a.layers;
+    constexpr int NR = LT > 0 ? LT : 1;
+    cplx<Real> mr[NR][3];
+    if (LT > 0) {
+#pragma unroll
+        for (int s = 0; s < NR; ++s) {
             const cplx<Real> *M = a.m + (int64_t)s * a.layer_stride + i;
-            const cplx<Real> mx = M[0], my = M[cs], mz = M[2 * cs];
-            const cplx<Real> kxx = K[0], kyy = K[cs], kzz = K[2 * cs], kxy = K[3 * cs], kxz = K[4 * cs],
-                             kyz = K[5 * cs];
-            cmad(hx, kxx, mx); cmad(hx, kxy, my); cmad(hx, kxz, mz);
-            cmad(hy, kxy, mx); cmad(hy, kyy, my); cmad(hy, kyz, mz);
-            cmad(hz, kxz, mx); cmad(hz, kyz, my); cmad(hz, kzz, mz);
+            mr[s][0] = M[0];
+            mr[s][1] = M[cs];
+            mr[s][2] = M[2 * cs];
+        }
+    }
+    for (int d = 0; d < L; ++d) {
+        cplx<Real> hx{Real(0), Real(0)}, hy{Real(0), Real(0)}, hz{Real(0), Real(0)};
+#pragma unroll
+        for (int s = 0; s < (LT > 0 ? LT : 1); ++s) {
+            for (int s2 = s; s2 < (LT > 0 ? s + 1 : L); ++s2) {
+                const int u = a.pair[d * L + s2];
+                const uint32_t im = a.kimag[u];
+                const Real *K = a.kern + (int64_t)u * a.kern_stride + i;
+                cplx<Real> mx, my, mz;
+                if (LT > 0) {
+                    mx = mr[s][0];
+                    my = mr[s][1];
+                    mz = mr[s][2];
+                } else {
+                    const cplx<Real> *M = a.m + (int64_t)s2 * a.layer_stride + i;
+                    mx = M[0];
+                    my = M[cs];
+                    mz = M[2 * cs];
+                }
+                const Real kxx = K[0], kyy = K[cs], kzz = K[2 * cs], kxy = K[3 * cs], kxz = K[4 * cs], kyz = K[5 * cs];
+                const bool ixx = im & 1u, iyy = im & 2u, izz = im & 4u, ixy = im & 8u, ixz = im & 16u, iyz = im & 32u;
+                kmad(hx, kxx, ixx, mx); kmad(hx, kxy, ixy, my); kmad(hx, kxz, ixz, mz);
+                kmad(hy, kxy, ixy, mx); kmad(hy, kyy, iyy, my); kmad(hy, kyz, iyz, mz);
+                kmad(hz, kxz, ixz, mx); kmad(hz, kyz, iyz, my); kmad(hz, kzz, izz, mz);
+            }
         }
         cplx<Real> *H = a.h + (int64_t)d * a.layer_stride + i;
         H[0] = {-hx.x, -hx.y};
@@ -2177,10 +2215,15 @@ __global__ void __launch_bounds__(256) stack_pull_kernel(GridXfer g, const Real 
     for (int c = 0; c < 3; ++c) out[c * nn + p] = act ? Real(r[c]) : Real(0);
 }
 
+// One real per kernel component: the imaginary part where bit c of `imag`
+// is set, else the real part (the other part is zero up to rounding).
 template <typename Real>
-__global__ void __launch_bounds__(256) spec_narrow_kernel(const cplx<double> *in, cplx<Real> *out, int64_t n) {
+__global__ void __launch_bounds__(256) spec_part_kernel(const cplx<double> *in, Real *out, int64_t points,
+                                                        uint32_t imag) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) out[i] = {Real(in[i].x), Real(in[i].y)};
+    if (i >= 6 * points) return;
+    const int c = (int)(i / points);
+    out[i] = Real(((imag >> c) & 1u) ? in[i].y : in[i].x);
 }
 
 GridXfer xfer_of(const Stack &s, const Context &l) {
@@ -2259,18 +2302,26 @@ bool stack_setup_t(Stack &s, std::string &err) {
         }
     s.nkern = 1 + (int)shifts.size();
     s.kern_stride = 6 * (uint64_t)points;
-    if (!ctx_alloc(c, &s.kern, (size_t)s.nkern * s.kern_stride * ce, "stack pair kernels") ||
+    // components with an odd number of odd axes transform to imaginary
+    // values: xz (odd in x) and yz (odd in y) of the shifted kernels, whose
+    // z mirror keeps the sign, and of the thin self kernel (zero there); in
+    // the 3-D self kernel they are also odd in z, so every component is real
+    std::vector<uint32_t> kimag(s.nkern, (1u << 4) | (1u << 5));
+    if (dp.three_d) kimag[0] = 0;
+    if (!ctx_alloc(c, &s.kern, (size_t)s.nkern * s.kern_stride * sizeof(Real), "stack pair kernels") ||
         !ctx_alloc(c, (void **)&s.d_pair, s.pair.size() * sizeof(int), "stack pair table") ||
-        !ctx_upload(c, s.d_pair, s.pair.data(), s.pair.size() * sizeof(int), "hipMemcpy(pair table)", err))
+        !ctx_alloc(c, (void **)&s.d_kimag, kimag.size() * sizeof(uint32_t), "stack kernel parities") ||
+        !ctx_upload(c, s.d_pair, s.pair.data(), s.pair.size() * sizeof(int), "hipMemcpy(pair table)", err) ||
+        !ctx_upload(c, s.d_kimag, kimag.data(), kimag.size() * sizeof(uint32_t), "hipMemcpy(parities)", err))
         return false;
     for (int u = 0; u < s.nkern; ++u) {
         std::vector<double> oct;
         newell_octant(dp.nx, dp.ny, dp.three_d ? dp.nz : 1, c.dx, c.dy, c.dz, oct, u == 0 ? 0.0 : shifts[u - 1]);
         cplx<double> *spec = nullptr;
         if (!newell_forward(c, dp, oct, &spec, err, u > 0)) return false;
-        cplx<Real> *dst = static_cast<cplx<Real> *>(s.kern) + (uint64_t)u * s.kern_stride;
-        hipLaunchKernelGGL((spec_narrow_kernel<Real>), dim3((unsigned)cdiv(s.kern_stride, 256)), dim3(256), 0,
-                           c.stream, spec, dst, (int64_t)s.kern_stride);
+        Real *dst = static_cast<Real *>(s.kern) + (uint64_t)u * s.kern_stride;
+        hipLaunchKernelGGL((spec_part_kernel<Real>), dim3((unsigned)cdiv(s.kern_stride, 256)), dim3(256), 0,
+                           c.stream, spec, dst, (int64_t)points, kimag[u]);
         const bool ok = hip_ok(hipStreamSynchronize(c.stream), "stack kernel", err);
         (void)hipFree(spec);
         if (!ok) return false;
@@ -2395,15 +2446,22 @@ void stack_eval_t(Stack &s, const S *const *src) {
         StackTensorArgs<Real> ta{};
         ta.m = sm;
         ta.h = sh;
-        ta.kern = static_cast<const cplx<Real> *>(s.kern);
+        ta.kern = static_cast<const Real *>(s.kern);
         ta.pair = s.d_pair;
+        ta.kimag = s.d_kimag;
         ta.layers = L;
         ta.points = (int64_t)s.planes * dp.hx * dp.py;
         ta.layer_stride = (int64_t)s.layer_spec;
         ta.comp_stride = ta.points;
         ta.kern_stride = (int64_t)s.kern_stride;
-        hipLaunchKernelGGL((stack_tensor_kernel<Real>), dim3((unsigned)cdiv(ta.points, 256)), dim3(256), 0, c.stream,
-                           ta);
+        const dim3 tg((unsigned)cdiv(ta.points, 256)), tb(256);
+        switch (L) {
+            case 1: hipLaunchKernelGGL((stack_tensor_kernel<Real, 1>), tg, tb, 0, c.stream, ta); break;
+            case 2: hipLaunchKernelGGL((stack_tensor_kernel<Real, 2>), tg, tb, 0, c.stream, ta); break;
+            case 3: hipLaunchKernelGGL((stack_tensor_kernel<Real, 3>), tg, tb, 0, c.stream, ta); break;
+            case 4: hipLaunchKernelGGL((stack_tensor_kernel<Real, 4>), tg, tb, 0, c.stream, ta); break;
+            default: hipLaunchKernelGGL((stack_tensor_kernel<Real, 0>), tg, tb, 0, c.stream, ta); break;
+        }
     }
     {
         Timed t(c, "stack_inverse");

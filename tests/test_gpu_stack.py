@@ -164,3 +164,14 @@ def test_stack_rejections(fa):
     with pytest.raises(fa.RunError, match="at least one layer"):
         fa.MultilayerStack(fa.FdmMultilayerPlan(common_cells=(8, 8, 1), convolution_cell_size=(DX, DY, DZ),
                                                 layers=[]))
+
+
+def test_five_layers_match_oracle(fa, oracle_mod):
+    """Five thin layers (more than the register-held source count, so every
+    destination reloads its sources), one of them 1.5 dz off the cell grid."""
+    z = [0.0, 2 * DZ, 3.5 * DZ, 6 * DZ, 9 * DZ]
+    specs = [((12, 10, 1), (DX, DY, DZ), (0.0, 0.0, zi), 7e5 + 1e5 * i, 20 + i) for i, zi in enumerate(z)]
+    stack, cpu, probs = build(fa, oracle_mod, specs, (12, 10, 1), (DX, DY, DZ))
+    check_fields(stack, cpu.observe(), 1e-11)
+    run_and_compare(stack, cpu, probs, 10, 1e-13, 1e-12, 1e-11)
+    stack.close()
