@@ -175,3 +175,27 @@ def test_five_layers_match_oracle(fa, oracle_mod):
     check_fields(stack, cpu.observe(), 1e-11)
     run_and_compare(stack, cpu, probs, 10, 1e-13, 1e-12, 1e-11)
     stack.close()
+
+
+def test_stack_without_demag_is_independent_layers(fa, oracle_mod):
+    """enable_demag off: every layer steps exactly as its own backend with
+    demag off (the same kernels, bit for bit)."""
+    lps = [layer(fa, oracle_mod, (20, 12, 2), (DX, DY, DZ), (0.0, 0.0, 3 * DZ * i), 8e5 + 2e5 * i, 30 + i)[0]
+           for i in range(2)]
+    stack = fa.MultilayerStack(fa.FdmMultilayerPlan(common_cells=(20, 12, 2), convolution_cell_size=(DX, DY, DZ),
+                                                    layers=lps, external_field=H_EXT, enable_demag=False))
+    singles = [fa.NativeFdmBackend(fa.FdmPlan(cells=lp.native_grid, cell_size=lp.native_cell_size,
+                                              initial_magnetization=lp.initial_magnetization,
+                                              saturation_magnetisation=lp.saturation_magnetisation,
+                                              exchange_stiffness=lp.exchange_stiffness, damping=lp.damping,
+                                              external_field=H_EXT, enable_demag=False)) for lp in lps]
+    for _ in range(6):
+        g = stack.step(1e-13)
+        rs = [b.step(1e-13) for b in singles]
+    for lay, b in zip(stack.layers, singles):
+        np.testing.assert_array_equal(lay.copy_m(), b.copy_m())
+        assert np.all(lay.copy_h_demag() == 0.0)
+    assert g.e_total == rs[0].e_total + rs[1].e_total
+    stack.close()
+    for b in singles:
+        b.close()

@@ -157,3 +157,41 @@ def test_tier_member_in_batch_matches_eager(fa, oracle_mod):
     run.close()
     for x in batch + eager:
         x.close()
+
+
+def test_t1_per_cell_ms_dp45_mixed_matches_oracle(fa, oracle_mod):
+    """T1 with per-cell Ms (extension: it enters the tier prefactor and the
+    correction source as Ms_i), mixed precision, DP45 with dt feedback."""
+    O = oracle_mod
+    nx, ny, nz = 20, 16, 2
+    n = nx * ny * nz
+    phi, faces, deltas = _tiers.geometry(nx, ny, nz, *CELL)
+    tgt, src, ten = _tiers.correction_rows(nx, ny, nz, phi, seed=5)
+    mask = (phi > 0).astype(np.uint8)
+    ms_cell = np.random.default_rng(3).uniform(6e5, 1e6, n)
+    m0 = fa.generate_random_unit_vectors(17, n)
+    m0[mask == 0] = 0.0
+    names = ("xp", "xm", "yp", "ym", "zp", "zm")
+    bg = fa.BoundaryGeometry(volume_fraction=phi, **{f"delta_{k}": deltas[i] for i, k in enumerate(names)},
+                             demag_corr_target_idx=tgt, demag_corr_source_idx=src, demag_corr_tensor=ten,
+                             demag_corr_stencil_size=src.shape[1])
+    adaptive = dict(adaptive_max_error=1e-4, adaptive_dt_min=1e-16, adaptive_dt_max=4e-13, adaptive_headroom=0.8)
+    gpu = fa.NativeFdmBackend(fa.FdmPlan(cells=(nx, ny, nz), cell_size=CELL, initial_magnetization=m0,
+                                         saturation_magnetisation=MS, exchange_stiffness=A_EX, damping=0.2,
+                                         precision="mixed", integrator="dp45", external_field=(2e4, -1e4, 5e3),
+                                         active_mask=mask, demag_kernel="device", ms_cell=ms_cell,
+                                         boundary_correction="full", boundary_geometry=bg, **adaptive))
+    cfg = O.OracleConfig(nx=nx, ny=ny, nz=nz, dx=CELL[0], dy=CELL[1], dz=CELL[2], ms=MS, a=A_EX, alpha=0.2,
+                         integrator="dp45", h_ext=(2e4, -1e4, 5e3), active_mask=mask, ms_cell=ms_cell, tier=2,
+                         phi=phi, deltas=deltas, corr_target=tgt, corr_source=src, corr_tensor=ten,
+                         max_error=1e-4, dt_min=1e-16, dt_max=4e-13, headroom=0.8)
+    cpu = O.OracleProblem(cfg, m0)
+    check_fields(gpu, cpu.observe(), 3e-6)
+    dt = 1e-13
+    for _ in range(8):
+        g = gpu.step(dt)
+        r = cpu.step(dt)
+        assert abs(g.dt - r["dt_used"]) <= 1e-5 * r["dt_used"]
+        dt = r["suggested_next_dt"] or dt
+    assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-6
+    gpu.close()
