@@ -398,16 +398,41 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
         // with a boundary tier the exchange energy stays that of the standard
         // masked stencil (native/backends/fdm/src/reductions_fp64.cu:123-200
         // sums the pair differences without face links or distances)
-        double he[3] = {double(hex[0]), double(hex[1]), double(hex[2])};
-        if (TIER && f.exchange) {
-            Src hp[3];
-            exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, hp);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) he[c] = double(Real(hp[c]));
-        }
         // a lane's column (at most one z chunk) is summed plainly; lanes and
         // workgroups merge with compensation (block_reduce, finalize)
-        acc[0] += w * (md[0] * he[0] + md[1] * he[1] + md[2] * he[2]);
+        if (TIER && f.exchange && !f.region) {
+            // the pair form of the standard masked stencil's energy
+            // (native/backends/fdm/src/reductions_fp64.cu:150-190): over the
+            // cell's +x, +y, +z pairs with an active neighbour,
+            // sum_i w m_i . H_i = -(2A / mu0)(w / Ms) sum |m_j - m_i|^2 / d^2
+            const int64_t nxy = (int64_t)f.nx * f.ny;
+            const bool up[3] = {x < f.nx - 1, y < f.ny - 1, z < f.nz - 1};
+            const int64_t nb[3] = {i + 1, i + f.nx, i + nxy};
+            const double id2[3] = {f.idx2, f.idy2, f.idz2};
+            double e = 0.0;
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const int64_t j = up[t] ? nb[t] : i;
+                const bool use = up[t] && (!f.mask || f.mask[j]);
+                double d2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double dv = double(t == 2 ? vzp[c] : a.src[c * n + j]) - md[c];
+                    d2 += dv * dv;
+                }
+                e += use ? d2 * id2[t] : 0.0;
+            }
+            acc[0] -= (f.ms_cell ? f.two_over_mu0 * f.a_uniform : f.pref) * e;
+        } else {
+            double he[3] = {double(hex[0]), double(hex[1]), double(hex[2])};
+            if (TIER && f.exchange) {
+                Src hp[3];
+                exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, hp);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) he[c] = double(Real(hp[c]));
+            }
+            acc[0] += w * (md[0] * he[0] + md[1] * he[1] + md[2] * he[2]);
+        }
         acc[2] += w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2]));
         if (f.has_ext) acc[4] += w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz);
         const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
