@@ -38,6 +38,11 @@ namespace fm {
 namespace {
 
 constexpr int kPad = 1;         // LDS line padding (complex slots)
+
+// Stand-ins read in place of an absent optional per-cell array, so that the
+// read itself needs no branch: an active-mask byte of 1 and a value of 1.
+__device__ const uint8_t kNoMask = 1;
+__device__ const double kNoArrayF64 = 1.0;
 constexpr int kMaxBlock = 512;  // lanes per workgroup for FFT kernels
 // largest padded z length done in registers (a lane pair per column holds
 // 3 x pz complex values)
@@ -154,8 +159,11 @@ struct XArgs {
     SpecGeom g;
 };
 
-// Ms m at (c, z, y, x) of the state; zero on inactive cells.
-template <typename Real, typename S>
+// Ms m at (c, z, y, x) of the state; zero on inactive cells.  GATHER: the
+// x pass reads the mask, volume fraction and per-cell Ms beside m with
+// unconditional loads (a separate instantiation, so that the kernel for
+// uniform material keeps its register budget).
+template <typename Real, typename S, bool GATHER = false>
 struct MagSource {
     const S *m;
     const uint8_t *mask;
@@ -177,6 +185,20 @@ struct MagSource {
     __device__ Real uniform_at(int c, int z, int y, int x) const {
         return Real(double(m[(int64_t)c * n + ((int64_t)z * ny + y) * nx + x]) * ms);
     }
+    // operator()'s value with every load unconditional (an absent array is
+    // read from a stand-in and its value replaced after the load), so a
+    // lane's loads all issue before the first is used.
+    static constexpr bool kGather = GATHER;
+    __device__ Real gather_at(int c, int z, int y, int x) const {
+        const int64_t i = ((int64_t)z * ny + y) * nx + x;
+        const double mv = double(m[(int64_t)c * n + i]);
+        const double pv = *(phi ? phi + i : &kNoArrayF64);
+        const double cv = *(ms_cell ? ms_cell + i : &kNoArrayF64);
+        const uint8_t kv = *(mask ? mask + i : &kNoMask);
+        const double w = ms_cell ? cv : ms;
+        const bool live = phi || kv != 0;
+        return live ? Real(mv * (phi ? pv * w : w)) : Real(0);
+    }
 };
 
 // Padded real-space Newell kernel at (c, z, y, x) of the (px, py, pz) grid,
@@ -189,6 +211,8 @@ struct NewellSource {
     int z_even;  // the cross-layer kernel: mirrored z slots keep every sign (newell.rs:630-647)
     __device__ bool uniform() const { return false; }
     __device__ double uniform_at(int, int, int, int) const { return 0.0; }
+    static constexpr bool kGather = false;
+    __device__ double gather_at(int, int, int, int) const { return 0.0; }
     __device__ double operator()(int c, int zz, int yy, int x) const {
         if (x == nx || yy == ny || (pz > 1 && zz == nz)) return 0.0;
         const int xi = x < nx ? x : px - x;
@@ -214,6 +238,14 @@ struct FieldSink {
         const int64_t i = ((int64_t)z * ny + y) * nx + x;
         out[(int64_t)c * n + i] = (mask && !mask[i]) ? Real(0) : v;
     }
+    // With a mask, operator() in two halves, so that a lane reads all its
+    // mask bytes before its first store (a byte read may alias the stores,
+    // so the compiler keeps each one behind the store before it).
+    __device__ bool masked() const { return mask != nullptr; }
+    __device__ uint8_t keep(int z, int y, int x) const { return mask[((int64_t)z * ny + y) * nx + x]; }
+    __device__ void put(int c, int z, int y, int x, Real v, uint8_t keep) const {
+        out[(int64_t)c * n + ((int64_t)z * ny + y) * nx + x] = keep ? v : Real(0);
+    }
 };
 
 // Forward real-to-complex along x for xr rows (y0 .. y0+xr-1) of plane
@@ -238,7 +270,7 @@ __device__ __forceinline__ void xfwd_body(const XArgs<Real> &a, const Src &src, 
         if constexpr (N > 0) {
             // fixed length: all 16 loads of a lane in flight before the LDS writes
             cplx<Real> v[16];
-            if (src.uniform() && 2 * a.n_in <= N) {
+            if (!Src::kGather && src.uniform() && 2 * a.n_in <= N) {
                 // branch-free: clamped addresses, every load issued before
                 // any is used; elements t + e N/16 with e >= 8 lie in the
                 // zero padding (n_in <= N/2) and are not loaded
@@ -250,6 +282,21 @@ __device__ __forceinline__ void xfwd_body(const XArgs<Real> &a, const Src &src, 
                     const bool okx = x < a.n_in;
                     const int xs = okx ? x : 0;
                     const Real va = src.uniform_at(c, z, yA, xs), vb = src.uniform_at(c, z, yB, xs);
+                    v[e] = {(okx && oka) ? va : Real(0), (okx && okb) ? vb : Real(0)};
+                }
+#pragma unroll
+                for (int e = 8; e < 16; ++e) v[e] = {Real(0), Real(0)};
+            } else if (Src::kGather && 2 * a.n_in <= N) {
+                // the same with the mask, volume fraction and per-cell Ms
+                // read beside each element, all unconditionally
+                const bool oka = ya < a.rows, okb = ya + 1 < a.rows;
+                const int yA = oka ? ya : 0, yB = okb ? ya + 1 : 0;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int x = t + e * (N / 16);
+                    const bool okx = x < a.n_in;
+                    const int xs = okx ? x : 0;
+                    const Real va = src.gather_at(c, z, yA, xs), vb = src.gather_at(c, z, yB, xs);
                     v[e] = {(okx && oka) ? va : Real(0), (okx && okb) ? vb : Real(0)};
                 }
 #pragma unroll
@@ -448,6 +495,32 @@ __device__ __forceinline__ void xinv_body(const XArgs<Real> &a, const cplx<Real>
     L::template fft<Real>(lds + jl * ls, a.lp, t, a.tw, true, lx);
     // the lanes of line jl store its rows 2 jl (real part) and 2 jl + 1
     const int ya = y0 + 2 * jl;
+    if constexpr (N > 0) {
+        if (2 * a.n_out <= N && sink.masked()) {
+            // at most 8 outputs per row and lane: every mask byte read
+            // (clamped addresses) before the first store
+            const bool oka = ya < a.rows, okb = ya + 1 < a.rows;
+            const int yA = oka ? ya : 0, yB = okb ? ya + 1 : 0;
+            uint8_t ka[8], kb[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int x = t + e * T;
+                const int xs = x < a.n_out ? x : 0;
+                ka[e] = sink.keep(z, yA, xs);
+                kb[e] = sink.keep(z, yB, xs);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int x = t + e * T;
+                if (x < a.n_out) {
+                    const cplx<Real> v = line[L::slot(a.lp, x, lx)];
+                    if (oka) sink.put(c, z, ya, x, v.x * a.scale, ka[e]);
+                    if (okb) sink.put(c, z, ya + 1, x, v.y * a.scale, kb[e]);
+                }
+            }
+            return;
+        }
+    }
     for (int x = t; x < a.n_out; x += T) {
         const cplx<Real> v = line[L::slot(a.lp, x, lx)];
         if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
@@ -1463,19 +1536,26 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
     const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
     const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
     {
-        MagSource<Real, S> ms{src,    c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
-                              dp.nx,  dp.ny,    dp.nz,
-                              (int64_t)c.n, weight_ms ? c.d_phi : nullptr};
+        const MagSource<Real, S> ms{src,    c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
+                                    dp.nx,  dp.ny,    dp.nz,
+                                    (int64_t)c.n, weight_ms ? c.d_phi : nullptr};
         Timed t(c, "demag_x_r2c");
-        with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
-            constexpr int N = decltype(NC)::value;
-            const XFwdPack<Real, MagSource<Real, S>> pk{xa, ms, spec};
-            launch_rec(c, N > 0 ? reinterpret_cast<const void *>(&xfwd_multi<Real, MagSource<Real, S>, N>) : nullptr,
-                       xgrid, xblock, xlds, 3, pk, [&] {
-                           hipLaunchKernelGGL((xfwd_kernel<Real, MagSource<Real, S>, N>), xgrid, xblock, xlds,
-                                              c.stream, xa, ms, spec);
-                       });
-        });
+        auto launch = [&](const auto &src_v) {
+            using Src = std::decay_t<decltype(src_v)>;
+            with_line(dp.lx, dp.tune.fixed != 0, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                const XFwdPack<Real, Src> pk{xa, src_v, spec};
+                launch_rec(c, N > 0 ? reinterpret_cast<const void *>(&xfwd_multi<Real, Src, N>) : nullptr, xgrid,
+                           xblock, xlds, 3, pk, [&] {
+                               hipLaunchKernelGGL((xfwd_kernel<Real, Src, N>), xgrid, xblock, xlds, c.stream, xa,
+                                                  src_v, spec);
+                           });
+            });
+        };
+        if (ms.mask || ms.ms_cell || ms.phi)
+            launch(MagSource<Real, S, true>{ms.m, ms.mask, ms.ms_cell, ms.ms, ms.nx, ms.ny, ms.nz, ms.n, ms.phi});
+        else
+            launch(ms);
     }
     const KernelView<Real> K = kview<Real>(c);
     if (dp.three_d) {
@@ -2435,8 +2515,13 @@ void stack_eval_t(Stack &s, const S *const *src) {
                 MagSource<Real, double> ms{cm, nullptr, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
                 stack_xfwd<Real>(c, xa, ms, spec);
             } else {
-                MagSource<Real, S> ms{src[l], lc.d_mask, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
-                stack_xfwd<Real>(c, xa, ms, spec);
+                if (lc.d_mask) {
+                    MagSource<Real, S, true> ms{src[l], lc.d_mask, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
+                    stack_xfwd<Real>(c, xa, ms, spec);
+                } else {
+                    MagSource<Real, S> ms{src[l], nullptr, nullptr, lc.ms, dp.nx, dp.ny, dp.nz, ncv, nullptr};
+                    stack_xfwd<Real>(c, xa, ms, spec);
+                }
             }
             stack_yz<Real>(s, spec, false);
         }

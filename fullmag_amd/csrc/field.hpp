@@ -498,18 +498,21 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
     block_reduce<MODE>(acc, a.partials + (int64_t)blockIdx.x * (MODE == STAGE_STATS ? kNStat : 1));
 }
 
-// Uniform-material form of stage_kernel: no active mask, no region table,
-// no per-cell Ms, demag and exchange on, at most one earlier slope (the Heun
-// stages and the stats pass; RK4 / RK23 / DP45 / ABM3 stages with more
+// Uniform-material form of stage_kernel: no region table, no per-cell Ms,
+// no boundary tier, demag and exchange on, at most one earlier slope (the
+// Heun stages and the stats pass; RK4 / RK23 / DP45 / ABM3 stages with more
 // slopes use stage_kernel).  The per-cell arithmetic is stage_cell's, in the
 // same order, so results are bit-identical; what changes is that every load
 // of a plane (the z neighbour ahead, the four x/y neighbours, H_d, the base
-// point, the earlier slope) is issued in one branch-free batch before any
-// arithmetic, where stage_kernel's optional-input branches split a plane's
-// loads into several serial waits.  M0_IS_SRC: the base point is src itself
-// (predictor stages and the stats pass), so it is not loaded again.
+// point, the earlier slope, and with MASK their mask bytes) is issued in one
+// branch-free batch before any arithmetic, where stage_kernel's
+// optional-input branches split a plane's loads into several serial waits.
+// M0_IS_SRC: the base point is src itself (predictor stages and the stats
+// pass), so it is not loaded again.  MASK: an active mask is present; an
+// inactive neighbour enters as the centre value (exchange_col) and an
+// inactive cell gets zero field and slope and adds nothing to the sums.
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT>
+          bool HAS_OUT, bool MASK = false>
 __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, Src, Out> &a) {
     static_assert(MODE != STAGE_ERROR && NPREV <= 1, "plain and stats stages with at most one earlier slope");
     static_assert(!M0_IS_SRC || sizeof(Src) == sizeof(State), "a base point read from src is a state");
@@ -543,6 +546,13 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
             cur[c] = src[c * n + i];
             prev[c] = src[c * n + ib];
         }
+        // mask bytes marched along z like src (unused without MASK)
+        const uint8_t *__restrict__ mk = f.mask;
+        uint8_t kcur = 1, kprev = 1;
+        if constexpr (MASK) {
+            kcur = mk[i];
+            kprev = mk[ib];
+        }
         const Src ix = Src(f.idx2), iy = Src(f.idy2), iz = Src(f.idz2), pref = Src(f.pref);
         const Real he[3] = {f.has_ext ? Real(f.hx) : Real(0), f.has_ext ? Real(f.hy) : Real(0),
                             f.has_ext ? Real(f.hz) : Real(0)};
@@ -551,6 +561,14 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
             Src nxt[3], xm[3], xp[3], ym[3], yp[3];
             Real hdv[3], kpv[3];
             State m0v[3];
+            uint8_t knxt = 1, kxm = 1, kxp = 1, kym = 1, kyp = 1;
+            if constexpr (MASK) {
+                knxt = mk[ia];
+                kxm = mk[i + oxm];
+                kxp = mk[i + oxp];
+                kym = mk[i + oym];
+                kyp = mk[i + oyp];
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const Src *sc = src + c * n;
@@ -563,19 +581,27 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
                 if (HAS_OUT && !M0_IS_SRC) m0v[c] = m0p[c * n + i];
                 if (HAS_OUT && NPREV == 1) kpv[c] = kp[c * n + i];
             }
+            const bool act = !MASK || kcur != 0;
             Real m[3], hex[3], h[3], k[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 m[c] = Real(cur[c]);
-                // exchange_col's arithmetic (z neighbours clamped at the grid edge)
+                // exchange_col's arithmetic (z neighbours clamped at the grid
+                // edge, inactive neighbours replaced by the centre)
                 const Src cc = cur[c];
-                const Src zmv = z > 0 ? prev[c] : cc, zpv = z < f.nz - 1 ? nxt[c] : cc;
-                const Src lap = (xp[c] - Src(2) * cc + xm[c]) * ix + (yp[c] - Src(2) * cc + ym[c]) * iy +
+                const Src zmv = (z > 0 && kprev) ? prev[c] : cc, zpv = (z < f.nz - 1 && knxt) ? nxt[c] : cc;
+                const Src xmv = kxm ? xm[c] : cc, xpv = kxp ? xp[c] : cc;
+                const Src ymv = kym ? ym[c] : cc, ypv = kyp ? yp[c] : cc;
+                const Src lap = (xpv - Src(2) * cc + xmv) * ix + (ypv - Src(2) * cc + ymv) * iy +
                                 (zpv - Src(2) * cc + zmv) * iz;
                 hex[c] = Real(pref * lap);
                 h[c] = (hex[c] + hdv[c]) + he[c];
             }
             torque(f, m, h, k);
+            if (MASK && !act) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hex[c] = hdv[c] = h[c] = k[c] = Real(0);
+            }
             if (a.kout) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) a.kout[c * n + i] = k[c];
@@ -594,7 +620,7 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
 #pragma unroll
                 for (int c = 0; c < 3; ++c) a.out[c * n + i] = Out(v[c] * inv);
             }
-            if constexpr (MODE == STAGE_STATS) {
+            if (MODE == STAGE_STATS && act) {
                 double md[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) md[c] = double(cur[c]);
@@ -614,6 +640,8 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
                 prev[c] = cur[c];
                 cur[c] = nxt[c];
             }
+            kprev = kcur;
+            kcur = knxt;
         }
     }
     if constexpr (MODE == STAGE_PLAIN) return;
@@ -621,21 +649,22 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
 }
 
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT>
+          bool HAS_OUT, bool MASK = false>
 __global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, Real, Src, Out> a) {
-    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT>(a);
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK>(a);
 }
 // Multi-instance form: instance blockIdx.z / zdiv of the table.
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT>
+          bool HAS_OUT, bool MASK = false>
 __global__ void __launch_bounds__(kBlock) stage_uniform_multi(const StageArgs<State, Real, Src, Out> *tab, int zdiv) {
     const StageArgs<State, Real, Src, Out> a = tab[blockIdx.z / zdiv];
-    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT>(a);
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK>(a);
 }
 
-// Whether a stage with these arguments may run as stage_uniform_kernel.
+// Whether a stage with these arguments may run as stage_uniform_kernel
+// (with MASK = f.mask != nullptr).
 inline bool stage_uniform_ok(const FieldArgs &f, const void *hd, int nprev) {
-    return !f.mask && !f.region && !f.ms_cell && !f.tier_ex && hd && f.exchange && nprev <= 1;
+    return !f.region && !f.ms_cell && !f.tier_ex && hd && f.exchange && nprev <= 1;
 }
 
 // Column tiling of the stage kernels: tile width TX = 64 (or the smallest

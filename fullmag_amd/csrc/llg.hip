@@ -183,14 +183,19 @@ struct Driver {
         Timed t(c, "llg_stage");
         const dim3 g(grid_stage()), b(kBlock);
         // the uniform kernel (or its multi-instance twin while a batch records)
-        auto uniform = [&](auto np, auto m0src) {
+        auto uniform_m = [&](auto np, auto m0src, auto masked) {
             constexpr int NP = decltype(np)::value;
-            constexpr bool MS = decltype(m0src)::value;
-            launch_rec(c, reinterpret_cast<const void *>(&stage_uniform_multi<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true>),
+            constexpr bool MS = decltype(m0src)::value, MK = decltype(masked)::value;
+            launch_rec(c,
+                       reinterpret_cast<const void *>(&stage_uniform_multi<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK>),
                        g, b, 0, 1, a, [&] {
-                           hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true>),
+                           hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK>),
                                               g, b, 0, c.stream, a);
                        });
+        };
+        auto uniform = [&](auto np, auto m0src) {
+            if (a.f.mask) uniform_m(np, m0src, std::true_type{});
+            else uniform_m(np, m0src, std::false_type{});
         };
         if (out && stage_uniform_ok(a.f, a.hd, a.nprev)) {
             constexpr bool src_is_state = sizeof(Src) == sizeof(State);
@@ -278,8 +283,14 @@ struct Driver {
         {
             Timed t(c, "llg_stats");
             if (stage_uniform_ok(a.f, a.hd, 0)) {
-                if (out)
+                if (out && a.f.mask)
+                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, true, true>),
+                                       dim3(g), dim3(kBlock), 0, c.stream, a);
+                else if (out)
                     hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, true>),
+                                       dim3(g), dim3(kBlock), 0, c.stream, a);
+                else if (a.f.mask)
+                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false, true>),
                                        dim3(g), dim3(kBlock), 0, c.stream, a);
                 else
                     hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false>),

@@ -448,6 +448,43 @@ def test_mixed_precision_random_state_within_fp64_chaos_envelope(fa, oracle_mod)
     assert rel(b.e_total, a.e_total) <= 1e-6
 
 
+@pytest.mark.parametrize("precision,per_cell_ms", [("double", False), ("double", True), ("mixed", False),
+                                                   ("single", False)])
+def test_masked_disc_heun_matches_oracle(fa, oracle_mod, precision, per_cell_ms):
+    """A disc with a hole and scattered inactive cells on 32x32x4 cells of
+    3x3x2 nm: power-of-two x lines, so the fixed-length x transforms read the
+    mask (and per-cell Ms) beside m, and without per-cell Ms the Heun stages
+    and the stats pass run the uniform stage kernel's masked form.  40 Heun
+    steps of 1e-13 s from a random m0 against the oracle: FP64 max|dm| <=
+    1e-11 and energies <= 1e-10 relative; mixed and single 1e-5 and 1e-4."""
+    nx, ny, nz = 32, 32, 4
+    n = nx * ny * nz
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    r2 = (xx - 15.5) ** 2 + (yy - 15.5) ** 2
+    disc = ((r2 <= 15.0 ** 2) & (r2 >= 4.0 ** 2)).astype(np.uint8)
+    mask = np.tile(disc.ravel(), nz)
+    mask[np.random.default_rng(11).choice(n, 40, replace=False)] = 0
+    m0 = fa.generate_random_unit_vectors(2024, n)
+    m0[mask == 0] = 0.0
+    ms_cell = None
+    if per_cell_ms:
+        ms_cell = np.random.default_rng(12).uniform(6e5, 1e6, n)
+    gpu, cpu = make_pair(fa, oracle_mod, cells=(nx, ny, nz), cell=(3e-9, 3e-9, 2e-9), ms=8e5, a=1.3e-11,
+                         alpha=0.1, m0=m0, h_ext=(2e4, -1e4, 5e3), precision=precision, mask=mask,
+                         ms_cell=ms_cell)
+    for _ in range(40):
+        g = gpu.step(1e-13)
+        r = cpu.step(1e-13)
+    dm = float(np.abs(gpu.copy_m() - cpu.m).max())
+    print(f"masked disc {precision} per-cell Ms={per_cell_ms}: max|dm|={dm:.3e}")
+    tol_m, tol_e = (1e-11, 1e-10) if precision == "double" else (1e-5, 1e-4)
+    assert dm <= tol_m
+    for key, gv in (("e_ex", g.e_ex), ("e_demag", g.e_demag), ("e_ext", g.e_ext), ("e_total", g.e_total)):
+        assert rel(gv, r[key]) <= tol_e, key
+    assert rel(g.max_h_eff, r["max_h_eff"]) <= tol_e
+    gpu.close()
+
+
 def test_single_precision_runner_plan_close_to_double(fa, oracle_mod, kats):
     """native_fdm.rs:1272-1346: single vs double m <= 1e-5, E <= 1e-4 rel."""
     k = kats["runner_masked_plan"]
