@@ -15,6 +15,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNStat = 9;  // s_ex, c_ex, s_d, c_d, s_ext, c_ext, max|Heff|^2, max|Hd|^2, max|k|^2
+// Read in place of an absent per-cell Ms, so that the read needs no branch.
+__device__ const double kUnitStandIn = 1.0;
 
 struct FieldArgs {
     int nx, ny, nz;
@@ -154,6 +156,11 @@ __device__ __forceinline__ void exchange_col(const FieldArgs &f, const S *__rest
     }
 }
 
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_tier_vals(const FieldArgs &f, const bool in[6], const S v[6][3],
+                                                   const double wt[6], double phi0, double pre, const S cc[3],
+                                                   C h[3]);
+
 // Sub-cell boundary exchange at cell (x, y, z) = i, in the arithmetic type C
 // of the source.  T0 (tier_ex 1, native/backends/fdm/src/exchange_t0_fp64.cu:
 // 19-133): H = 2A / (mu0 Ms) / max(phi, phi_floor) sum f (m_j - m) / d^2 over
@@ -191,8 +198,18 @@ __device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__res
 #pragma unroll
     for (int s = 0; s < 6; ++s) wt[s] = f.tier_w[(int64_t)(s ^ 1) * n + i];
     const double phi0 = f.phi[i];
-    const C c0[3] = {C(cc[0]), C(cc[1]), C(cc[2])};
     const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * f.ms_cell[i]) : f.tier_pref;
+    exchange_tier_vals<C, S>(f, in, v, wt, phi0, pre, cc, h);
+}
+
+// exchange_tier's arithmetic on loaded values: v[s] the neighbour m of side
+// s (-x, +x, -y, +y, -z, +z; any value where in[s] is false), wt[s] its face
+// link or distance, phi0 the cell's volume fraction, pre its 2 A / (mu0 Ms).
+template <typename C, typename S>
+__device__ __forceinline__ void exchange_tier_vals(const FieldArgs &f, const bool in[6], const S v[6][3],
+                                                   const double wt[6], double phi0, double pre, const S cc[3],
+                                                   C h[3]) {
+    const C c0[3] = {C(cc[0]), C(cc[1]), C(cc[2])};
     C b[3] = {C(0), C(0), C(0)};
     double scale;
     if (f.tier_ex == 1) {
@@ -511,8 +528,13 @@ __global__ void __launch_bounds__(kBlock) stage_kernel(StageArgs<State, Real, Sr
 // pass), so it is not loaded again.  MASK: an active mask is present; an
 // inactive neighbour enters as the centre value (exchange_col) and an
 // inactive cell gets zero field and slope and adds nothing to the sums.
+// TIER: the exchange is the sub-cell boundary stencil of f.tier_ex
+// (exchange_tier on the plane's loaded neighbours, face weights, volume
+// fraction and per-cell Ms, which it allows; the mask then gates only the
+// cell itself), and the stats pass sums the standard masked stencil's
+// exchange energy in stage_cell's pair form from the same loaded values.
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT, bool MASK = false>
+          bool HAS_OUT, bool MASK = false, bool TIER = false>
 __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, Src, Out> &a) {
     static_assert(MODE != STAGE_ERROR && NPREV <= 1, "plain and stats stages with at most one earlier slope");
     static_assert(!M0_IS_SRC || sizeof(Src) == sizeof(State), "a base point read from src is a state");
@@ -581,20 +603,48 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
                 if (HAS_OUT && !M0_IS_SRC) m0v[c] = m0p[c * n + i];
                 if (HAS_OUT && NPREV == 1) kpv[c] = kp[c * n + i];
             }
+            double wt[6], phi0 = 0.0, msc = 1.0;
+            if constexpr (TIER) {
+#pragma unroll
+                for (int s = 0; s < 6; ++s) wt[s] = f.tier_w[(int64_t)(s ^ 1) * n + i];
+                phi0 = f.phi[i];
+                msc = *(f.ms_cell ? f.ms_cell + i : &kUnitStandIn);
+            }
             const bool act = !MASK || kcur != 0;
+            Src het[3];
+            if constexpr (TIER) {
+                const bool in[6] = {x > 0, x < f.nx - 1, y > 0, y < f.ny - 1, f.nz > 1 && z > 0,
+                                    f.nz > 1 && z < f.nz - 1};
+                Src v[6][3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    v[0][c] = xm[c];
+                    v[1][c] = xp[c];
+                    v[2][c] = ym[c];
+                    v[3][c] = yp[c];
+                    v[4][c] = prev[c];
+                    v[5][c] = nxt[c];
+                }
+                const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * msc) : f.tier_pref;
+                exchange_tier_vals<Src, Src>(f, in, v, wt, phi0, pre, cur, het);
+            }
             Real m[3], hex[3], h[3], k[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 m[c] = Real(cur[c]);
-                // exchange_col's arithmetic (z neighbours clamped at the grid
-                // edge, inactive neighbours replaced by the centre)
-                const Src cc = cur[c];
-                const Src zmv = (z > 0 && kprev) ? prev[c] : cc, zpv = (z < f.nz - 1 && knxt) ? nxt[c] : cc;
-                const Src xmv = kxm ? xm[c] : cc, xpv = kxp ? xp[c] : cc;
-                const Src ymv = kym ? ym[c] : cc, ypv = kyp ? yp[c] : cc;
-                const Src lap = (xpv - Src(2) * cc + xmv) * ix + (ypv - Src(2) * cc + ymv) * iy +
-                                (zpv - Src(2) * cc + zmv) * iz;
-                hex[c] = Real(pref * lap);
+                if constexpr (TIER) {
+                    hex[c] = Real(het[c]);
+                } else {
+                    // exchange_col's arithmetic (z neighbours clamped at the
+                    // grid edge, inactive neighbours replaced by the centre)
+                    const Src cc = cur[c];
+                    const Src zmv = (z > 0 && kprev) ? prev[c] : cc, zpv = (z < f.nz - 1 && knxt) ? nxt[c] : cc;
+                    const Src xmv = kxm ? xm[c] : cc, xpv = kxp ? xp[c] : cc;
+                    const Src ymv = kym ? ym[c] : cc, ypv = kyp ? yp[c] : cc;
+                    const Src lap = (xpv - Src(2) * cc + xmv) * ix + (ypv - Src(2) * cc + ymv) * iy +
+                                    (zpv - Src(2) * cc + zmv) * iz;
+                    hex[c] = Real(pref * lap);
+                }
                 h[c] = (hex[c] + hdv[c]) + he[c];
             }
             torque(f, m, h, k);
@@ -624,8 +674,28 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
                 double md[3];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) md[c] = double(cur[c]);
-                const double w = 1.0;
-                acc[0] += w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2]));
+                const double w = (TIER && f.ms_cell) ? msc : 1.0;
+                if constexpr (TIER) {
+                    // stage_cell's pair form over the cell's +x, +y, +z pairs
+                    const bool up[3] = {x < f.nx - 1, y < f.ny - 1, z < f.nz - 1};
+                    const uint8_t kn[3] = {kxp, kyp, knxt};
+                    const double id2[3] = {f.idx2, f.idy2, f.idz2};
+                    double e = 0.0;
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) {
+                        const bool use = up[t] && kn[t] != 0;
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            const double dv = double(t == 0 ? xp[c] : t == 1 ? yp[c] : nxt[c]) - md[c];
+                            d2 += dv * dv;
+                        }
+                        e += use ? d2 * id2[t] : 0.0;
+                    }
+                    acc[0] -= (f.ms_cell ? f.two_over_mu0 * f.a_uniform : f.pref) * e;
+                } else {
+                    acc[0] += w * (md[0] * double(hex[0]) + md[1] * double(hex[1]) + md[2] * double(hex[2]));
+                }
                 acc[2] += w * (md[0] * double(hdv[0]) + md[1] * double(hdv[1]) + md[2] * double(hdv[2]));
                 if (f.has_ext) acc[4] += w * (md[0] * f.hx + md[1] * f.hy + md[2] * f.hz);
                 const double h2 = double(h[0]) * h[0] + double(h[1]) * h[1] + double(h[2]) * h[2];
@@ -649,22 +719,23 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
 }
 
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT, bool MASK = false>
+          bool HAS_OUT, bool MASK = false, bool TIER = false>
 __global__ void __launch_bounds__(kBlock) stage_uniform_kernel(StageArgs<State, Real, Src, Out> a) {
-    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK>(a);
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK, TIER>(a);
 }
 // Multi-instance form: instance blockIdx.z / zdiv of the table.
 template <typename State, typename Real, typename Src, typename Out, int MODE, int NPREV, bool M0_IS_SRC,
-          bool HAS_OUT, bool MASK = false>
+          bool HAS_OUT, bool MASK = false, bool TIER = false>
 __global__ void __launch_bounds__(kBlock) stage_uniform_multi(const StageArgs<State, Real, Src, Out> *tab, int zdiv) {
     const StageArgs<State, Real, Src, Out> a = tab[blockIdx.z / zdiv];
-    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK>(a);
+    stage_uniform_body<State, Real, Src, Out, MODE, NPREV, M0_IS_SRC, HAS_OUT, MASK, TIER>(a);
 }
 
 // Whether a stage with these arguments may run as stage_uniform_kernel
-// (with MASK = f.mask != nullptr).
+// (with MASK = f.mask != nullptr and TIER = f.tier_ex != 0; per-cell Ms only
+// with a tier).
 inline bool stage_uniform_ok(const FieldArgs &f, const void *hd, int nprev) {
-    return !f.region && !f.ms_cell && !f.tier_ex && hd && f.exchange && nprev <= 1;
+    return !f.region && (f.tier_ex || !f.ms_cell) && hd && f.exchange && nprev <= 1;
 }
 
 // Column tiling of the stage kernels: tile width TX = 64 (or the smallest

@@ -183,19 +183,25 @@ struct Driver {
         Timed t(c, "llg_stage");
         const dim3 g(grid_stage()), b(kBlock);
         // the uniform kernel (or its multi-instance twin while a batch records)
-        auto uniform_m = [&](auto np, auto m0src, auto masked) {
+        auto uniform_m = [&](auto np, auto m0src, auto masked, auto tier) {
             constexpr int NP = decltype(np)::value;
-            constexpr bool MS = decltype(m0src)::value, MK = decltype(masked)::value;
+            constexpr bool MS = decltype(m0src)::value, MK = decltype(masked)::value, TR = decltype(tier)::value;
             launch_rec(c,
-                       reinterpret_cast<const void *>(&stage_uniform_multi<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK>),
+                       reinterpret_cast<const void *>(
+                           &stage_uniform_multi<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK, TR>),
                        g, b, 0, 1, a, [&] {
-                           hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK>),
+                           hipLaunchKernelGGL((stage_uniform_kernel<State, Real, Src, Out, STAGE_PLAIN, NP, MS, true, MK, TR>),
                                               g, b, 0, c.stream, a);
                        });
         };
         auto uniform = [&](auto np, auto m0src) {
-            if (a.f.mask) uniform_m(np, m0src, std::true_type{});
-            else uniform_m(np, m0src, std::false_type{});
+            if (a.f.tier_ex) {
+                if (a.f.mask) uniform_m(np, m0src, std::true_type{}, std::true_type{});
+                else uniform_m(np, m0src, std::false_type{}, std::true_type{});
+            } else {
+                if (a.f.mask) uniform_m(np, m0src, std::true_type{}, std::false_type{});
+                else uniform_m(np, m0src, std::false_type{}, std::false_type{});
+            }
         };
         if (out && stage_uniform_ok(a.f, a.hd, a.nprev)) {
             constexpr bool src_is_state = sizeof(Src) == sizeof(State);
@@ -283,18 +289,22 @@ struct Driver {
         {
             Timed t(c, "llg_stats");
             if (stage_uniform_ok(a.f, a.hd, 0)) {
-                if (out && a.f.mask)
-                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, true, true>),
+                auto launch = [&](auto has_out, auto masked, auto tier) {
+                    constexpr bool HO = decltype(has_out)::value, MK = decltype(masked)::value,
+                                   TR = decltype(tier)::value;
+                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, HO, MK, TR>),
                                        dim3(g), dim3(kBlock), 0, c.stream, a);
-                else if (out)
-                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, true>),
-                                       dim3(g), dim3(kBlock), 0, c.stream, a);
-                else if (a.f.mask)
-                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false, true>),
-                                       dim3(g), dim3(kBlock), 0, c.stream, a);
-                else
-                    hipLaunchKernelGGL((stage_uniform_kernel<State, Real, State, Real, STAGE_STATS, 0, true, false>),
-                                       dim3(g), dim3(kBlock), 0, c.stream, a);
+                };
+                auto with_tier = [&](auto has_out, auto masked) {
+                    if (a.f.tier_ex) launch(has_out, masked, std::true_type{});
+                    else launch(has_out, masked, std::false_type{});
+                };
+                auto with_mask = [&](auto has_out) {
+                    if (a.f.mask) with_tier(has_out, std::true_type{});
+                    else with_tier(has_out, std::false_type{});
+                };
+                if (out) with_mask(std::true_type{});
+                else with_mask(std::false_type{});
             } else if (a.f.tier_ex) {
                 hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_STATS, true>), dim3(g), dim3(kBlock),
                                    0, c.stream, a);

@@ -39,6 +39,49 @@ def test_predictor_reuse_is_bit_exact(fa, prec, cells):
     b.close()
 
 
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+@pytest.mark.parametrize("kind", ["masked", "tier_volume", "tier_full"])
+def test_predictor_reuse_is_bit_exact_masked_and_tiers(fa, oracle_mod, prec, kind):
+    """The same on a masked cylinder, without and with the sub-cell boundary
+    tiers (the stats pass and the plain stage are then the uniform stage
+    kernel's masked and tier forms)."""
+    from tests import _tiers
+
+    cells = (24, 20, 3)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    phi, faces, deltas = _tiers.geometry(nx, ny, nz, 3e-9, 4e-9, 2e-9)
+    mask = (phi > 0).astype(np.uint8)
+    m0 = fa.generate_random_unit_vectors(17, n)
+    m0[mask == 0] = 0.0
+    names = ("xp", "xm", "yp", "ym", "zp", "zm")
+    kw = {}
+    if kind != "masked":
+        kw = dict(boundary_correction="volume" if kind == "tier_volume" else "full",
+                  boundary_geometry=fa.BoundaryGeometry(
+                      volume_fraction=phi, **{f"face_link_{k}": faces[i] for i, k in enumerate(names)},
+                      **{f"delta_{k}": deltas[i] for i, k in enumerate(names)}))
+
+    def make():
+        return fa.NativeFdmBackend(fa.FdmPlan(cells=cells, cell_size=(3e-9, 4e-9, 2e-9), initial_magnetization=m0,
+                                              saturation_magnetisation=8e5, exchange_stiffness=1.3e-11, damping=0.2,
+                                              external_field=H_EXT, precision=prec, active_mask=mask,
+                                              demag_kernel="device", **kw))
+
+    a, b = make(), make()
+    for _ in range(5):
+        sa = a.step(5e-14)
+        b.set_external_field(H_EXT)
+        sb = b.step(5e-14)
+        np.testing.assert_array_equal(a.copy_m(), b.copy_m())
+        assert sa.e_total == sb.e_total
+    a.step(5e-14)
+    b.step_many(5e-14, 1)
+    np.testing.assert_array_equal(a.copy_m(), b.copy_m())
+    a.close()
+    b.close()
+
+
 def test_predictor_dropped_on_dt_field_and_state_changes(fa, oracle_mod):
     cells = (24, 20, 3)
     nx, ny, nz = cells

@@ -100,20 +100,25 @@ def test_dp45_block_256x256x64_mixed_vs_fp64(fa):
     vortex m0 (-y, x, 0.1)/norm + 1% noise, adaptive DP45 with max error 1e-6
     and dt fed back from suggested_next_dt (as the CPU runner does,
     cpu_reference.rs:504-506), demag padded 512x512x128 (FP32 FFT in mixed).
-    Both precisions take the same number of accepted steps; |m| = 1, the
-    damped energy never rises, and mixed stays within 1e-5 of FP64."""
+    The FP64 run is offered the step sizes the mixed run accepted, so both
+    reach the same simulated times (each still checks its own error
+    estimate); |m| = 1, the damped energy never rises, and mixed stays within
+    1e-5 of FP64."""
     nx, ny, nz = 256, 256, 64
     base = dict(cells=(nx, ny, nz), cell_size=(2e-9, 2e-9, 2e-9), initial_magnetization=vortex_m0(nx, ny, nz),
                 saturation_magnetisation=1.4e6, exchange_stiffness=2.1e-11, damping=0.01, integrator="dp45",
                 demag_kernel="device", adaptive_max_error=1e-6)
     steps = 25
     out = {}
+    accepted = []
     for prec in ("mixed", "double"):
         b = fa.NativeFdmBackend(fa.FdmPlan(precision=prec, **base))
         dt, e = 1e-13, []
-        for _ in range(steps):
-            s = b.step(dt)
+        for k in range(steps):
+            s = b.step(dt if prec == "mixed" else accepted[k])
             e.append(s.e_total)
+            if prec == "mixed":
+                accepted.append(s.dt)
             dt = s.dt_suggested or dt
         m = b.copy_m()
         np.testing.assert_allclose(np.linalg.norm(m, axis=1), 1.0, atol=1e-12)
