@@ -289,8 +289,14 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
                     !ctx_alloc(*c, (void **)&c->d_corr_off, corr_off.size() * 4, "demag correction offsets") ||
                     !ctx_alloc(*c, (void **)&c->d_corr_row, corr_row.size() * 4, "demag correction rows") ||
                     !ctx_alloc(*c, (void **)&c->d_corr_src, rows * st * 4, "demag correction sources") ||
-                    !ctx_alloc(*c, (void **)&c->d_corr_n, rows * st * 6 * 8, "demag correction tensors"))
+                    !ctx_alloc(*c, &c->d_corr_n, rows * st * 6 * (c->mode == MODE_MIXED ? 4 : 8),
+                               "demag correction tensors"))
                     return c;
+                // under mixed precision the rows are stored in FP32, the
+                // precision of the demag field they correct
+                std::vector<float> ten32;
+                if (c->mode == MODE_MIXED)
+                    ten32.assign(p.demag_corr_tensor, p.demag_corr_tensor + rows * st * 6);
                 if (!ctx_upload(*c, c->d_corr_cell, corr_cell.data(), corr_cell.size() * 4, "hipMemcpy(corr)",
                                 c->last_error) ||
                     !ctx_upload(*c, c->d_corr_off, corr_off.data(), corr_off.size() * 4, "hipMemcpy(corr)",
@@ -299,8 +305,11 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
                                 c->last_error) ||
                     !ctx_upload(*c, c->d_corr_src, p.demag_corr_source_idx, rows * st * 4, "hipMemcpy(corr)",
                                 c->last_error) ||
-                    !ctx_upload(*c, c->d_corr_n, p.demag_corr_tensor, rows * st * 6 * 8, "hipMemcpy(corr)",
-                                c->last_error))
+                    !(ten32.empty()
+                          ? ctx_upload(*c, c->d_corr_n, p.demag_corr_tensor, rows * st * 6 * 8, "hipMemcpy(corr)",
+                                       c->last_error)
+                          : ctx_upload(*c, c->d_corr_n, ten32.data(), rows * st * 6 * 4, "hipMemcpy(corr)",
+                                       c->last_error)))
                     return c;
             }
         }

@@ -139,7 +139,7 @@ struct Context {
     int32_t *d_corr_cell = nullptr;
     uint32_t *d_corr_off = nullptr, *d_corr_row = nullptr;
     int32_t *d_corr_src = nullptr;  // rows x stencil source cells (< 0: unused)
-    double *d_corr_n = nullptr;     // rows x stencil x 6: Nxx, Nxy, Nxz, Nyy, Nyz, Nzz
+    void *d_corr_n = nullptr;       // rows x stencil x 6: Nxx, Nxy, Nxz, Nyy, Nyz, Nzz; FP64, FP32 when mixed
 
     double max_err = 1e-5, dt_min = 1e-18, dt_max = 1e-10, headroom = 0.8;
 

@@ -1466,7 +1466,7 @@ struct BoundaryCorrArgs {
     const int32_t *cell;
     const uint32_t *off, *row;
     const int32_t *src;
-    const double *tensor;
+    const Real *tensor;  // the field's precision (FP32 rows under mixed precision)
     uint32_t groups, stencil;
 };
 
@@ -1475,28 +1475,57 @@ struct BoundaryCorrArgs {
 // target cell gains -sum_s dN(t, s) . (phi_s Ms_s m_s) per row, dN symmetric
 // with the components Nxx, Nxy, Nxz, Nyy, Nyz, Nzz; sources < 0 are skipped.
 // One wavefront per target cell: its lanes share each row's stencil entries
-// (coalesced reads of the row's sources and tensors), a butterfly reduction
-// gives the row's sum in FP64, and the cell's rows are added in plan order,
-// so a cell named by several rows gets all of them without atomics.
+// (entry lane + 64 u, u < 4, per pass), a butterfly reduction gives the row's
+// sum in FP64, and the cell's rows are added in plan order, so a cell named
+// by several rows gets all of them without atomics.  A pass issues all of a
+// lane's loads together: its source indices, then the gathered sources and
+// the tensors, with every address clamped to a valid entry; a skipped entry
+// subtracts exactly zero.
 template <typename Real, typename S>
 __global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Real, S> a) {
+    constexpr int U = 4;
     const uint32_t g = blockIdx.x * 4u + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
     if (g >= a.groups) return;  // a whole wavefront
     const int64_t ti = a.cell[g], n = a.n;
     double h[3] = {double(a.out[ti]), double(a.out[n + ti]), double(a.out[2 * n + ti])};
+    const double *msp = a.ms_cell;
     for (uint32_t e = a.off[g]; e < a.off[g + 1]; ++e) {
         const uint64_t r0 = (uint64_t)a.row[e] * a.stencil;
         double cx = 0.0, cy = 0.0, cz = 0.0;
-        for (uint32_t s = lane; s < a.stencil; s += 64u) {
-            const int32_t si = a.src[r0 + s];
-            if (si < 0) continue;
-            const double *nn = a.tensor + (r0 + s) * 6;
-            const double w = a.phi[si] * (a.ms_cell ? a.ms_cell[si] : a.ms);
-            const double mx = w * double(a.m[si]), my = w * double(a.m[n + si]), mz = w * double(a.m[2 * n + si]);
-            cx -= nn[0] * mx + nn[1] * my + nn[2] * mz;
-            cy -= nn[1] * mx + nn[3] * my + nn[4] * mz;
-            cz -= nn[2] * mx + nn[4] * my + nn[5] * mz;
+        for (uint32_t s0 = 0; s0 < a.stencil; s0 += U * 64u) {
+            int32_t si[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t s = s0 + lane + 64u * u;
+                ok[u] = s < a.stencil;
+                si[u] = a.src[r0 + (ok[u] ? s : 0u)];
+            }
+            double w[U], mv[U][3];
+            Real nn[U][6];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                ok[u] = ok[u] && si[u] >= 0;
+                const int64_t j = ok[u] ? si[u] : 0;
+                const double pv = a.phi[j];
+                const double cv = *(msp ? msp + j : &kNoArrayF64);
+                w[u] = pv * (msp ? cv : a.ms);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) mv[u][q] = double(a.m[q * n + j]);
+                const Real *t = a.tensor + (r0 + (ok[u] ? s0 + lane + 64u * u : 0u)) * 6;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) nn[u][q] = t[q];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double mx = w[u] * mv[u][0], my = w[u] * mv[u][1], mz = w[u] * mv[u][2];
+                const double n0 = nn[u][0], n1 = nn[u][1], n2 = nn[u][2];
+                const double n3 = nn[u][3], n4 = nn[u][4], n5 = nn[u][5];
+                cx -= ok[u] ? n0 * mx + n1 * my + n2 * mz : 0.0;
+                cy -= ok[u] ? n1 * mx + n3 * my + n4 * mz : 0.0;
+                cz -= ok[u] ? n2 * mx + n4 * my + n5 * mz : 0.0;
+            }
         }
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -1738,7 +1767,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
         ba.off = c.d_corr_off;
         ba.row = c.d_corr_row;
         ba.src = c.d_corr_src;
-        ba.tensor = c.d_corr_n;
+        ba.tensor = static_cast<const Real *>(c.d_corr_n);
         ba.groups = c.corr_groups;
         ba.stencil = c.corr_stencil;
         Timed t(c, "demag_boundary");
