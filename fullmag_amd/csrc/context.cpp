@@ -41,6 +41,9 @@ void ctx_free_all(Context &c) {
     c.pending.clear();
     for (hipEvent_t e : c.event_pool) hipEventDestroy(e);
     c.event_pool.clear();
+    if (c.ev_stats) hipEventDestroy(c.ev_stats);
+    c.ev_stats = nullptr;
+    c.spec_pre = false;
     if (c.stream) hipStreamDestroy(c.stream);
     c.stream = nullptr;
     c.device_bytes = 0;

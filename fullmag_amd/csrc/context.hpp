@@ -163,6 +163,11 @@ struct Context {
     // the stats pass of the previous step (its torque is the next step's
     // first slope, evaluated on the same inputs)
     bool pre_valid = false;
+    // the spectrum holds the convolved predictor mt[0] left with pre_valid
+    // (enqueued after a step's stats while the host returns); any other
+    // demag evaluation clears it
+    bool spec_pre = false;
+    hipEvent_t ev_stats = nullptr;  // recorded after a step's stats copy
     double pre_dt = 0.0;
     bool obs_frozen = false;  // observables kept from before an upload
     bool obs_valid = false;   // h_ex_obs / h_eff_obs hold the fields at m
@@ -172,7 +177,7 @@ struct Context {
     double *d_partials = nullptr;
     int max_partial_blocks = 0;
     double *d_result = nullptr;
-    double *h_result = nullptr;  // pinned
+    double *h_result = nullptr;  // pinned, coherent (the finalize kernels store into it)
 
     // adjoint work buffers (allocated by the first adjoint call)
     bool adj_ready = false;
@@ -276,7 +281,11 @@ bool demag_setup(Context &c, const fullmag_fdm_plan_desc &plan, bool newell_on_d
 // weight_ms is false.  Inactive cells get zero.  With boundary tiers (and
 // weight_ms) the source is phi Ms src in every cell and the sparse
 // correction rows are added afterwards, also at inactive targets.
-void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true);
+// phase: kDemagFull; kDemagSpectrum stops before the x C2R, leaving the
+// convolved spectrum of src; kDemagFinish runs only the x C2R (and the
+// boundary correction rows) from such a spectrum.
+constexpr int kDemagFull = 0, kDemagSpectrum = 1, kDemagFinish = 2;
+void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms = true, int phase = kDemagFull);
 // Multilayer stack demag (demag.hip).  stack_demag_setup plans the common
 // grid's transforms and builds the pair kernels (self kernel for a layer
 // with itself, the shifted kernel of newell.rs:528-651 otherwise);

@@ -1544,26 +1544,15 @@ __global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Rea
     }
 }
 
+// x R2C of Ms src, the y and z transforms with the tensor product and the
+// inverse y and z transforms: everything up to the x C2R, in the spectrum.
 template <typename Real, typename S>
-void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
+void demag_transform_t(Context &c, const S *src, const XArgs<Real> &xa, dim3 xgrid, dim3 xblock, size_t xlds,
+                       bool weight_ms) {
     DemagPlan &dp = c.dp;
     cplx<Real> *spec = static_cast<cplx<Real> *>(dp.spec);
     const SpecGeom g{dp.hx, dp.three_d ? dp.nz : 1, dp.rows};
     const size_t ce = sizeof(cplx<Real>);
-    XArgs<Real> xa{};
-    xa.tw = static_cast<const cplx<Real> *>(dp.tw_x);
-    xa.lp = dp.lx;
-    xa.rows = dp.ny;
-    xa.n_in = dp.nx;
-    xa.n_out = dp.nx;
-    xa.scale = Real(dp.inv_npad);
-    xa.g = g;
-    xa.xr = dp.xr;
-    xa.xr_log2 = ilog2(dp.xr);
-    xa.ls = dp.x_ls;
-    const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
-    const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
-    const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
     {
         const MagSource<Real, S> ms{src,    c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
                                     dp.nx,  dp.ny,    dp.nz,
@@ -1742,6 +1731,33 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms) {
             launch_rec(c, reinterpret_cast<const void *>(&yconv_multi<Real, kMaxBlock>), grid, block, lds, 1, ya,
                        [&] { hipLaunchKernelGGL((yconv_kernel<Real, kMaxBlock>), grid, block, lds, c.stream, ya); });
     }
+}
+
+template <typename Real, typename S>
+void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms, int phase) {
+    DemagPlan &dp = c.dp;
+    cplx<Real> *spec = static_cast<cplx<Real> *>(dp.spec);
+    const SpecGeom g{dp.hx, dp.three_d ? dp.nz : 1, dp.rows};
+    const size_t ce = sizeof(cplx<Real>);
+    XArgs<Real> xa{};
+    xa.tw = static_cast<const cplx<Real> *>(dp.tw_x);
+    xa.lp = dp.lx;
+    xa.rows = dp.ny;
+    xa.n_in = dp.nx;
+    xa.n_out = dp.nx;
+    xa.scale = Real(dp.inv_npad);
+    xa.g = g;
+    xa.xr = dp.xr;
+    xa.xr_log2 = ilog2(dp.xr);
+    xa.ls = dp.x_ls;
+    const dim3 xgrid((unsigned)cdiv(dp.ny, dp.xr), (unsigned)dp.nz, 3);
+    const dim3 xblock((unsigned)((dp.xr / 2) * dp.lx.threads));
+    const size_t xlds = (size_t)(dp.xr / 2) * dp.x_ls * ce;
+    if (phase != kDemagFinish) {
+        c.spec_pre = false;  // the spectrum is overwritten
+        demag_transform_t<Real, S>(c, src, xa, xgrid, xblock, xlds, weight_ms);
+    }
+    if (phase == kDemagSpectrum) return;
     {
         FieldSink<Real> sink{out, c.d_mask, dp.nx, dp.ny, (int64_t)c.n};
         Timed t(c, "demag_x_c2r");
@@ -2144,13 +2160,13 @@ bool demag_replan(Context &c, const DemagTuning &t, std::string &err) {
     return true;
 }
 
-void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms) {
+void demag_eval(Context &c, const void *src, bool src_f64, void *out, bool weight_ms, int phase) {
     if (c.mode == MODE_DOUBLE) {
-        demag_eval_t<double, double>(c, static_cast<const double *>(src), static_cast<double *>(out), weight_ms);
+        demag_eval_t<double, double>(c, static_cast<const double *>(src), static_cast<double *>(out), weight_ms, phase);
     } else if (src_f64) {
-        demag_eval_t<float, double>(c, static_cast<const double *>(src), static_cast<float *>(out), weight_ms);
+        demag_eval_t<float, double>(c, static_cast<const double *>(src), static_cast<float *>(out), weight_ms, phase);
     } else {
-        demag_eval_t<float, float>(c, static_cast<const float *>(src), static_cast<float *>(out), weight_ms);
+        demag_eval_t<float, float>(c, static_cast<const float *>(src), static_cast<float *>(out), weight_ms, phase);
     }
 }
 

@@ -22,7 +22,8 @@
 namespace fm {
 namespace {
 
-// Reduces per-block partials into out[0..nq).
+// Reduces per-block partials into out[0..nq); `out` is the pinned,
+// coherent host result buffer, read once the stream has passed this kernel.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) finalize_kernel(const double *partials, int nblocks, double *out) {
     constexpr int nq = MODE == STAGE_STATS ? kNStat : 1;
@@ -38,6 +39,7 @@ __global__ void __launch_bounds__(kBlock) finalize_kernel(const double *partials
         }
     }
     block_reduce<MODE>(acc, out);
+    if (threadIdx.x == 0) __threadfence_system();
 }
 
 // out = normalise(m0 + dt sum_j a_j k_j), no field evaluation (DP45 FSAL
@@ -256,8 +258,7 @@ struct Driver {
                                    c.stream, a);
         }
         hipLaunchKernelGGL((finalize_kernel<STAGE_ERROR>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
-                           c.d_result);
-        hipMemcpyAsync(c.h_result, c.d_result, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+                           c.h_result);
         hipStreamSynchronize(c.stream);
         timing_resolve(c);
         return std::sqrt(c.h_result[0]);
@@ -314,17 +315,22 @@ struct Driver {
             }
         }
         hipLaunchKernelGGL((finalize_kernel<STAGE_STATS>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
-                           c.d_result);
-        hipMemcpyAsync(c.h_result, c.d_result, kNStat * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+                           c.h_result);
     }
 
     // lib.rs:1217-1249.  With use_pre, a predictor left by the previous
     // step's stats pass (same m, same H, same dt) replaces the first stage.
+    // The predictor's demag transforms may already be in the spectrum
+    // (enqueued by the previous step after its stats; spec_pre): then only
+    // the x C2R remains.
     void heun(double dt, bool use_pre = false) {
         ensure_hd();
-        if (!(use_pre && c.pre_valid && c.pre_dt == dt)) stage<State, Real>(m(), k(0), mt(0), {}, 1.0, dt);
+        const bool reuse = use_pre && c.pre_valid && c.pre_dt == dt;
+        if (!reuse) stage<State, Real>(m(), k(0), mt(0), {}, 1.0, dt);
         c.pre_valid = false;
-        demag_at_stage(mt(0));
+        if (reuse && c.spec_pre) demag_eval(c, mt(0), kRealF64, hd(), true, kDemagFinish);
+        else demag_at_stage(mt(0));
+        c.spec_pre = false;
         stage<Real, State>(mt(0), nullptr, m(), {{0, 0.5}}, 0.5, dt);
         demag_at_state(m());
         c.hd_at_m = true;
@@ -473,7 +479,21 @@ struct Driver {
         if (!stats_pending) heun_aware_stats(dt);
         c.obs_valid = false;
         c.obs_frozen = false;
-        if (!hip_ok(hipStreamSynchronize(c.stream), "integrator step", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
+        // A Heun step leaves the next step's predictor (for the same dt).
+        // Its demag transforms, which write only the spectrum, are enqueued
+        // behind the stats and the host waits for the stats alone, so the
+        // device keeps working while the caller takes the stats and issues
+        // its next call.  Anything but that next step clears spec_pre (any
+        // other demag evaluation) or pre_valid, and the work is discarded.
+        const bool ahead = c.integrator == FULLMAG_FDM_INTEGRATOR_HEUN && c.pre_valid && c.demag && !c.stack &&
+                           c.ev_stats && hipEventRecord(c.ev_stats, c.stream) == hipSuccess;
+        if (ahead) {
+            demag_eval(c, mt(0), kRealF64, nullptr, true, kDemagSpectrum);
+            c.spec_pre = true;
+        }
+        if (!hip_ok(ahead ? hipEventSynchronize(c.ev_stats) : hipStreamSynchronize(c.stream), "integrator step",
+                    c.last_error))
+            return FULLMAG_FDM_ERR_CUDA;
         if (!hip_ok(hipGetLastError(), "integrator step launch", c.last_error)) return FULLMAG_FDM_ERR_CUDA;
         timing_resolve(c);
         c.step_count++;
@@ -594,7 +614,10 @@ bool alloc_integrator_buffers(Context &c) {
     if (!ctx_alloc(c, (void **)&c.d_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double), "partials"))
         return false;
     if (!ctx_alloc(c, (void **)&c.d_result, 16 * sizeof(double), "result")) return false;
-    if (!hip_ok(hipHostMalloc((void **)&c.h_result, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc",
+    if (!hip_ok(hipEventCreateWithFlags(&c.ev_stats, hipEventDisableTiming), "hipEventCreate", c.last_error))
+        return false;
+    // coherent: the finalize kernels store a step's results here directly
+    if (!hip_ok(hipHostMalloc((void **)&c.h_result, 16 * sizeof(double), hipHostMallocCoherent), "hipHostMalloc",
                 c.last_error))
         return false;
     return true;

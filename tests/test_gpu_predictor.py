@@ -107,3 +107,46 @@ def test_predictor_dropped_on_dt_field_and_state_changes(fa, oracle_mod):
         cpu.step(dt)
         assert float(np.abs(gpu.copy_m() - cpu.m).max()) <= 1e-12, i
     gpu.close()
+
+
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+@pytest.mark.parametrize("cells", [(48, 40, 4), (40, 24, 1)])
+def test_next_predictor_transforms_survive_interleaved_calls(fa, prec, cells):
+    """A Heun step enqueues the next predictor's demag transforms behind its
+    stats and returns once the stats are in.  Observable copies, previews,
+    snapshots, a refresh, step_many and a dt change between steps must see the
+    fields of the step just taken, and every later step must be bit-identical
+    to a backend whose stored predictor is dropped before each step."""
+    a = backend(fa, cells, prec, 23)
+    b = backend(fa, cells, prec, 23)
+
+    def snap(x, name):
+        s = x.begin_field_snapshot(name)
+        out = s.wait()
+        s.close()
+        return out
+
+    ops = [lambda x: x.copy_h_demag(), lambda x: x.copy_h_eff(), lambda x: x.copy_h_ex(),
+           lambda x: x.copy_live_preview_field("H_demag", (8, 8, 1)), lambda x: snap(x, "H_demag"),
+           lambda x: snap(x, "m"), None, lambda x: x.refresh_observables(), None, lambda x: x.copy_m()]
+    dts = [5e-14, 5e-14, 5e-14, 4e-14, 4e-14, 5e-14, 5e-14, 5e-14, 5e-14, 5e-14]
+    for i, (op, dt) in enumerate(zip(ops, dts)):
+        if op is not None:
+            ra, rb = op(a), op(b)
+            if ra is not None:
+                np.testing.assert_array_equal(ra, rb, err_msg=str(i))
+        b.set_external_field(H_EXT)  # same field: drops b's stored predictor
+        sa, sb = a.step(dt), b.step(dt)
+        np.testing.assert_array_equal(a.copy_m(), b.copy_m(), err_msg=str(i))
+        assert sa.e_total == sb.e_total and sa.max_h_demag == sb.max_h_demag, i
+        if i == 5:
+            sa, sb = a.step_many(5e-14, 3), b.step_many(5e-14, 3)
+            np.testing.assert_array_equal(a.copy_m(), b.copy_m())
+    # the fields after the last step equal a fresh evaluation at that m
+    hd = a.copy_h_demag()
+    c = backend(fa, cells, prec, 23)
+    c.upload_magnetization(a.copy_m())
+    c.refresh_observables()
+    np.testing.assert_array_equal(hd, c.copy_h_demag())
+    for x in (a, b, c):
+        x.close()
