@@ -319,12 +319,15 @@ def dp45_block(fa, torch, args):
     """256x256x64 cells of 2 nm, Ms 1.4e6, A 2.1e-11, alpha 0.01, no Zeeman,
     m0 = normalise((-y, x, 0.1) + 0.01 N(0,1)) (coordinates in cells from the
     grid centre, seed 3), DP45 with max error 1e-6 and dt feedback from
-    suggested_next_dt, demag padded 512x512x128; --steps accepted steps."""
+    suggested_next_dt, demag padded 512x512x128; --steps accepted steps, or
+    with --sim-ns every accepted step up to that simulated time."""
     nx, ny, nz = 256, 256, 64
     n = nx * ny * nz
     base = dict(cells=(nx, ny, nz), cell_size=(2e-9, 2e-9, 2e-9), initial_magnetization=vortex_m0(nx, ny, nz),
                 saturation_magnetisation=1.4e6, exchange_stiffness=2.1e-11, damping=0.01, integrator="dp45",
                 demag_kernel="device", adaptive_max_error=1e-6)
+    if args.sim_ns > 0:
+        return dp45_block_horizon(fa, torch, base, n, args.sim_ns * 1e-9)
     res, final = {}, {}
     for prec in ("mixed", "double"):
         b = fa.NativeFdmBackend(fa.FdmPlan(precision=prec, **base))
@@ -358,6 +361,54 @@ def dp45_block(fa, torch, args):
                       "(-y, x, 0.1) + 1% noise, DP45 max error 1e-6, padded 512x512x128",
           "cells": n, "mixed": m, "double": res["double"], "mixed_vs_fp64_same_accepted_steps": acc,
           "speedup_vs_fp64": m["cell_steps_per_s"] / res["double"]["cell_steps_per_s"]})
+
+
+def dp45_block_horizon(fa, torch, base, n, t_end):
+    """The 3-D block DP45 run over its configured simulated time: each
+    precision steps with dt feedback until t >= t_end.  No Zeeman term and
+    alpha > 0, so the total energy may not rise; reported: accepted steps,
+    wall time, the largest energy rise between accepted steps (relative),
+    max ||m| - 1| at the end, and mixed against FP64 at the end (each at its
+    own final time; the difference of those times is reported beside)."""
+    res, final = {}, {}
+    for prec in ("mixed", "double"):
+        b = fa.NativeFdmBackend(fa.FdmPlan(precision=prec, **base))
+        dt, steps, rise, e_prev = 1e-13, 0, 0.0, None
+        e_first = None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s = None
+        while s is None or s.time < t_end:
+            s = b.step(min(dt, t_end - s.time) if s is not None else dt)
+            steps += 1
+            dt = s.dt_suggested or dt
+            if e_prev is not None:
+                rise = max(rise, (s.e_total - e_prev) / abs(e_prev))
+            else:
+                e_first = s.e_total
+            e_prev = s.e_total
+            if steps % 2000 == 0:
+                print(f"{prec}: {steps} accepted steps, t = {s.time:.4e} s, E = {s.e_total:.6e} J", file=sys.stderr,
+                      flush=True)
+        el = time.perf_counter() - t0
+        m = b.copy_m()
+        res[prec] = {"accepted_steps": steps, "wall_s": el, "ms_per_step": 1e3 * el / steps,
+                     "cell_steps_per_s": n * steps / el, "sim_time_s": s.time, "mean_dt": s.time / steps,
+                     "e_first": e_first, "e_final": s.e_total, "max_rel_energy_rise": rise,
+                     "max_norm_dev": float(np.abs(np.linalg.norm(m, axis=1) - 1.0).max())}
+        final[prec] = (m, s)
+        b.close()
+    (m_mx, s_mx), (m_64, s_64) = final["mixed"], final["double"]
+    mx = res["mixed"]
+    line("DP45 accepted cell-steps/s, 256x256x64 block over its simulated time (mixed precision)",
+         mx["cell_steps_per_s"], "cell-steps/s", mx["accepted_steps"], 0, mx["ms_per_step"], "f32 fields / f64 state",
+         {"workload": "3-D block 256x256x64 cells of 2 nm, Ms 1.4e6, A 2.1e-11, alpha 0.01, no Zeeman, vortex m0 "
+                      "(-y, x, 0.1) + 1% noise, DP45 max error 1e-6 with dt feedback, padded 512x512x128",
+          "cells": n, "t_end_s": t_end, "mixed": mx, "double": res["double"],
+          "mixed_vs_fp64_at_end": {"max_abs_dm": float(np.abs(m_mx - m_64).max()),
+                                   "rel_energy_err": abs(s_mx.e_total - s_64.e_total) / abs(s_64.e_total),
+                                   "final_time_diff_s": s_mx.time - s_64.time},
+          "speedup_vs_fp64": res["double"]["wall_s"] / mx["wall_s"]})
 
 
 def adjoint_film(fa, torch, args):
@@ -552,6 +603,8 @@ def main():
     ap.add_argument("--smooth-steps", type=int, default=1000)
     ap.add_argument("--horizon-steps", type=int, default=5000,
                     help="the workload's configured step count: mixed vs FP64 after this many steps")
+    ap.add_argument("--sim-ns", type=float, default=0.0,
+                    help="dp45-block: run each precision until this simulated time (ns) instead of --steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
