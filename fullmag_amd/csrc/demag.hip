@@ -211,7 +211,7 @@ struct NewellSource {
     int z_even;  // the cross-layer kernel: mirrored z slots keep every sign (newell.rs:630-647)
     __device__ bool uniform() const { return false; }
     __device__ double uniform_at(int, int, int, int) const { return 0.0; }
-    static constexpr bool kGather = false;
+    [[maybe_unused]] static constexpr bool kGather = false;  // read by the fixed-length x pass only
     __device__ double gather_at(int, int, int, int) const { return 0.0; }
     __device__ double operator()(int c, int zz, int yy, int x) const {
         if (x == nx || yy == ny || (pz > 1 && zz == nz)) return 0.0;
