@@ -89,38 +89,52 @@ __global__ void __launch_bounds__(kBlock) observe_kernel(FieldArgs f, const Stat
     }
 }
 
-// Box-average preview (native/backends/fdm/src/context.cu:222-286 semantics).
+// Box-average preview.  Bin (bx, by, bz) averages the cells
+// [bx nx / pnx, (bx + 1) nx / pnx) x [by ny / pny, ...) (at least one cell
+// per axis) over the z planes [z0 + bz zs, z0 + (bz + 1) zs) clamped to the
+// grid, the bin edges and clamps of native/backends/fdm/src/context.cu:
+// 222-286, summed in FP64.  One wavefront per bin: the lanes walk the bin's
+// cells x-fastest with stride 64, so each row of the box is read by
+// consecutive lanes, and a wave reduction combines the lane sums.
 template <typename In, typename OutT>
-__global__ void preview_kernel(const In *fx, const In *fy, const In *fz, uint32_t nx, uint32_t ny, uint32_t nz,
-                               uint32_t pnx, uint32_t pny, uint32_t pnz, uint32_t z0, uint32_t zs, OutT *out) {
-    const uint64_t count = (uint64_t)pnx * pny * pnz;
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= count) return;
+__global__ void __launch_bounds__(256) preview_kernel(const In *fx, const In *fy, const In *fz, uint32_t nx,
+                                                      uint32_t ny, uint32_t nz, uint32_t pnx, uint32_t pny,
+                                                      uint32_t pnz, uint32_t z0, uint32_t zs, OutT *out) {
+    const uint64_t bins = (uint64_t)pnx * pny * pnz;
+    const uint64_t p = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // one bin per wavefront
+    const int lane = (int)(threadIdx.x & 63);
+    if (p >= bins) return;
     const uint32_t bx = (uint32_t)(p % pnx), by = (uint32_t)((p / pnx) % pny), bz = (uint32_t)(p / ((uint64_t)pnx * pny));
-    uint32_t x0 = (uint32_t)((uint64_t)bx * nx / pnx), x1 = (uint32_t)((uint64_t)(bx + 1) * nx / pnx);
-    if (x1 <= x0) x1 = x0 + 1;
-    if (x1 > nx) x1 = nx;
-    uint32_t y0 = (uint32_t)((uint64_t)by * ny / pny), y1 = (uint32_t)((uint64_t)(by + 1) * ny / pny);
-    if (y1 <= y0) y1 = y0 + 1;
-    if (y1 > ny) y1 = ny;
-    uint32_t za = z0 + bz * zs;
-    if (za >= nz) za = nz - 1;
+    auto edges = [](uint32_t b, uint32_t n, uint32_t pn, uint32_t &lo, uint32_t &hi) {
+        lo = (uint32_t)((uint64_t)b * n / pn);
+        hi = (uint32_t)((uint64_t)(b + 1) * n / pn);
+        hi = hi > lo ? hi : lo + 1;
+        hi = hi < n ? hi : n;
+    };
+    uint32_t xa, xb, ya, yb;
+    edges(bx, nx, pnx, xa, xb);
+    edges(by, ny, pny, ya, yb);
+    const uint32_t za = z0 + bz * zs < nz ? z0 + bz * zs : nz - 1;
     uint32_t zb = z0 + (bz + 1) * zs;
-    if (zb <= za) zb = za + 1;
-    if (zb > nz) zb = nz;
-    double sx = 0, sy = 0, sz = 0, cnt = 0;
-    for (uint32_t z = za; z < zb; ++z)
-        for (uint32_t y = y0; y < y1; ++y)
-            for (uint32_t x = x0; x < x1; ++x) {
-                const uint64_t i = ((uint64_t)z * ny + y) * nx + x;
-                sx += double(fx[i]);
-                sy += double(fy[i]);
-                sz += double(fz[i]);
-                cnt += 1.0;
-            }
-    out[p * 3 + 0] = OutT(sx / cnt);
-    out[p * 3 + 1] = OutT(sy / cnt);
-    out[p * 3 + 2] = OutT(sz / cnt);
+    zb = zb > za ? zb : za + 1;
+    zb = zb < nz ? zb : nz;
+    const uint32_t wx = xb - xa, wy = yb - ya;
+    const uint64_t cells = (uint64_t)wx * wy * (zb - za);
+    double s[3] = {0.0, 0.0, 0.0};
+    for (uint64_t k = (uint64_t)lane; k < cells; k += 64) {
+        const uint64_t row = k / wx;
+        const uint64_t i = ((uint64_t)(za + row / wy) * ny + (ya + row % wy)) * nx + xa + (k - row * wx);
+        s[0] += double(fx[i]);
+        s[1] += double(fy[i]);
+        s[2] += double(fz[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += __shfl_down(s[c], off);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[p * 3 + c] = OutT(s[c] / double(cells));
 }
 
 // H_demag of every layer of a stack at the layers' present m.
@@ -827,7 +841,7 @@ bool materialize_observables(Context &c) {
 bool preview_field(Context &c, const void *field, bool field_f64, uint32_t pnx, uint32_t pny, uint32_t pnz,
                    uint32_t z0, uint32_t zs, void *dev_out, bool out_f64) {
     const uint64_t count = (uint64_t)pnx * pny * pnz;
-    const dim3 grid((unsigned)cdiv(count, 256)), block(256);
+    const dim3 grid((unsigned)cdiv(count * 64, 256)), block(256);  // a wavefront per bin
     const uint64_t n = c.n;
     if (field_f64) {
         const double *f = static_cast<const double *>(field);

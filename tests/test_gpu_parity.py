@@ -644,3 +644,50 @@ def test_config2_full_size_properties(fa):
     np.testing.assert_allclose(np.linalg.norm(ma, axis=1), 1.0, atol=1e-14)
     assert float(np.abs(ma - mb).max()) <= 1e-5
     assert rel(b.e_total, a.e_total) <= 1e-4
+
+
+def _box_means(field, cells, pcells, z0, zs):
+    """Box averages with the bin edges of native/backends/fdm/src/context.cu:
+    222-286: [b n / pn, (b + 1) n / pn), at least one cell, clamped; z planes
+    [z0 + b zs, z0 + (b + 1) zs) clamped to the grid."""
+    nx, ny, nz = cells
+    pnx, pny, pnz = pcells
+    f = field.reshape(nz, ny, nx, 3)
+
+    def edges(b, n, pn):
+        lo, hi = b * n // pn, (b + 1) * n // pn
+        hi = max(hi, lo + 1)
+        return lo, min(hi, n)
+
+    out = np.empty((pnz, pny, pnx, 3))
+    for bz in range(pnz):
+        za = min(z0 + bz * zs, nz - 1)
+        zb = min(max(z0 + (bz + 1) * zs, za + 1), nz)
+        for by in range(pny):
+            ya, yb = edges(by, ny, pny)
+            for bx in range(pnx):
+                xa, xb = edges(bx, nx, pnx)
+                out[bz, by, bx] = f[za:zb, ya:yb, xa:xb].reshape(-1, 3).mean(axis=0)
+    return out.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("pcells,z0,zs", [((3, 2, 1), 0, 4), ((5, 3, 2), 1, 2), ((7, 11, 4), 0, 1), ((48, 40, 4), 0, 1),
+                                          ((1, 1, 1), 2, 9)])
+def test_preview_box_means_large_and_uneven_bins(fa, pcells, z0, zs):
+    """Preview bins of up to 1920 cells (a wavefront's lanes loop over them),
+    uneven bin edges, z strides past the grid, and full resolution, against
+    numpy box means; FP64 and FP32 outputs."""
+    cells = (48, 40, 4)
+    n = 48 * 40 * 4
+    m0 = fa.generate_random_unit_vectors(31, n)
+    b = fa.NativeFdmBackend(fa.FdmPlan(cells=cells, cell_size=(4e-9, 4e-9, 2e-9), initial_magnetization=m0,
+                                       saturation_magnetisation=8e5, exchange_stiffness=1.3e-11, damping=0.1,
+                                       precision="double", demag_kernel="device"))
+    hd = b.copy_h_demag()
+    for name, field in (("m", m0), ("H_demag", hd)):
+        ref = _box_means(field, cells, pcells, z0, zs)
+        got = b.copy_live_preview_field(name, pcells, z0, zs)
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13 * np.abs(field).max())
+        got32 = b.copy_live_preview_field(name, pcells, z0, zs, f32=True)
+        np.testing.assert_allclose(got32, ref, rtol=0, atol=1e-6 * np.abs(field).max())
+    b.close()
