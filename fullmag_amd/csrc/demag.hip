@@ -521,6 +521,26 @@ __device__ __forceinline__ void xinv_body(const XArgs<Real> &a, const cplx<Real>
             return;
         }
     }
+    if constexpr (std::is_same<Sink, FieldSink<float>>::value) {
+        if (!sink.masked() && (a.n_out & 3) == 0 && (sink.n & 3) == 0) {
+            // four consecutive outputs of a row per lane and one 16-byte
+            // store (rows start on 16-byte boundaries: nx = n_out and n are
+            // multiples of four)
+            for (int x = 4 * t; x < a.n_out; x += 4 * T) {
+                cplx<Real> v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = line[L::slot(a.lp, x + e, lx)];
+                const int64_t i = ((int64_t)z * sink.ny + ya) * sink.nx + x;
+                if (ya < a.rows)
+                    *reinterpret_cast<float4 *>(sink.out + (int64_t)c * sink.n + i) =
+                        float4{v[0].x * a.scale, v[1].x * a.scale, v[2].x * a.scale, v[3].x * a.scale};
+                if (ya + 1 < a.rows)
+                    *reinterpret_cast<float4 *>(sink.out + (int64_t)c * sink.n + i + sink.nx) =
+                        float4{v[0].y * a.scale, v[1].y * a.scale, v[2].y * a.scale, v[3].y * a.scale};
+            }
+            return;
+        }
+    }
     for (int x = t; x < a.n_out; x += T) {
         const cplx<Real> v = line[L::slot(a.lp, x, lx)];
         if (ya < a.rows) sink(c, z, ya, x, v.x * a.scale);
