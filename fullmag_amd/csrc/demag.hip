@@ -657,7 +657,16 @@ __device__ __forceinline__ void line_pipe_body(const LineArgs<Real> &a) {
         }
         if (l < a.lines) {
             cplx<Real> *g = a.data + l * a.line_pitch;
-            for (int e = t; e < a.n_out; e += T) g[e] = line[p2slot(e)];
+            if (sizeof(Real) == 4 && ((a.n_out | a.line_pitch) & 1) == 0) {
+                // two consecutive outputs per lane in one 16-byte store (an
+                // even line pitch keeps every line 16-byte aligned)
+                for (int e = 2 * t; e < a.n_out; e += 2 * T) {
+                    const cplx<Real> u = line[p2slot(e)], v = line[p2slot(e + 1)];
+                    *reinterpret_cast<float4 *>(g + e) = float4{float(u.x), float(u.y), float(v.x), float(v.y)};
+                }
+            } else {
+                for (int e = t; e < a.n_out; e += T) g[e] = line[p2slot(e)];
+            }
         }
         if (PIPE) __syncthreads();
     }
