@@ -495,6 +495,46 @@ __device__ __forceinline__ void xinv_body(const XArgs<Real> &a, const cplx<Real>
     L::template fft<Real>(lds + jl * ls, a.lp, t, a.tw, true, lx);
     // the lanes of line jl store its rows 2 jl (real part) and 2 jl + 1
     const int ya = y0 + 2 * jl;
+    if constexpr (N > 0 && std::is_same<Sink, FieldSink<float>>::value) {
+        if (2 * a.n_out <= N && sink.masked() && (a.n_out & 3) == 0 && (sink.n & 3) == 0) {
+            // four consecutive outputs of a row per lane: their mask bytes in
+            // one 4-byte load, every mask word read before the first store,
+            // and the values in one 16-byte store (rows start on 16-byte
+            // boundaries: nx = n_out and n are multiples of four)
+            constexpr int TT = N / 16, Q = (N / 2 + 4 * TT - 1) / (4 * TT);
+            const bool oka = ya < a.rows, okb = ya + 1 < a.rows;
+            const int yA = oka ? ya : 0, yB = okb ? ya + 1 : 0;
+            uint32_t ka[Q], kb[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int x = 4 * (t + q * TT);
+                const int xs = x < a.n_out ? x : 0;
+                ka[q] = *reinterpret_cast<const uint32_t *>(sink.mask + ((int64_t)z * sink.ny + yA) * sink.nx + xs);
+                kb[q] = *reinterpret_cast<const uint32_t *>(sink.mask + ((int64_t)z * sink.ny + yB) * sink.nx + xs);
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int x = 4 * (t + q * TT);
+                if (x < a.n_out) {
+                    cplx<Real> v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[e] = line[L::slot(a.lp, x + e, lx)];
+                    auto row = [&](uint32_t k, int part) {
+                        float o[4];
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            o[e] = ((k >> (8 * e)) & 0xffu) ? (part ? v[e].y : v[e].x) * a.scale : 0.0f;
+                        return float4{o[0], o[1], o[2], o[3]};
+                    };
+                    const int64_t i = ((int64_t)z * sink.ny + ya) * sink.nx + x;
+                    if (oka) *reinterpret_cast<float4 *>(sink.out + (int64_t)c * sink.n + i) = row(ka[q], 0);
+                    if (okb)
+                        *reinterpret_cast<float4 *>(sink.out + (int64_t)c * sink.n + i + sink.nx) = row(kb[q], 1);
+                }
+            }
+            return;
+        }
+    }
     if constexpr (N > 0) {
         if (2 * a.n_out <= N && sink.masked()) {
             // at most 8 outputs per row and lane: every mask byte read
