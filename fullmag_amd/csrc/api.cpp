@@ -277,11 +277,23 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
                     const double *dl[6] = {p.delta_xp, p.delta_xm, p.delta_yp, p.delta_ym, p.delta_zp, p.delta_zm};
                     std::copy(dl, dl + 6, w);
                 }
-                if (!ctx_alloc(*c, (void **)&c->d_tier_w, 6 * n * 8, "boundary face links / distances")) return c;
-                for (int k = 0; k < 6; ++k)
-                    if (!ctx_upload(*c, c->d_tier_w + k * n, w[k], n * 8, "hipMemcpy(boundary arrays)",
+                if (c->mode == MODE_MIXED) {
+                    // FP32 like the fields they shape (half the bytes per pass)
+                    std::vector<float> w32(6 * n);
+                    for (int k = 0; k < 6; ++k)
+                        for (uint64_t i = 0; i < n; ++i) w32[k * n + i] = (float)w[k][i];
+                    if (!ctx_alloc(*c, (void **)&c->d_tier_w32, 6 * n * 4, "boundary face links / distances") ||
+                        !ctx_upload(*c, c->d_tier_w32, w32.data(), 6 * n * 4, "hipMemcpy(boundary arrays)",
                                     c->last_error))
                         return c;
+                } else {
+                    if (!ctx_alloc(*c, (void **)&c->d_tier_w, 6 * n * 8, "boundary face links / distances"))
+                        return c;
+                    for (int k = 0; k < 6; ++k)
+                        if (!ctx_upload(*c, c->d_tier_w + k * n, w[k], n * 8, "hipMemcpy(boundary arrays)",
+                                        c->last_error))
+                            return c;
+                }
             }
             if (c->corr_groups > 0) {
                 const uint64_t rows = p.demag_corr_target_count, st = c->corr_stencil;

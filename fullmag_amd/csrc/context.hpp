@@ -133,6 +133,7 @@ struct Context {
     double phi_floor = 0.05, delta_min = 0.0;
     double *d_phi = nullptr;    // volume fraction; non-null: demag packs phi Ms m
     double *d_tier_w = nullptr; // 6 n face links (T0) or distances (T1): xp, xm, yp, ym, zp, zm
+    float *d_tier_w32 = nullptr; // the same in FP32, in their place under mixed precision
     // sparse demag correction, rows grouped by target cell: group g adds rows
     // d_corr_row[d_corr_off[g] .. d_corr_off[g + 1]) (plan order) to cell d_corr_cell[g]
     uint32_t corr_groups = 0, corr_stencil = 0;

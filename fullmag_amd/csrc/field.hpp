@@ -43,6 +43,7 @@ struct FieldArgs {
     int tier_ex;
     const double *phi;     // volume fraction
     const double *tier_w;  // 6 n: xp, xm, yp, ym, zp, zm face links or distances
+    const float *tier_w32; // the same in FP32, in their place under mixed precision
     double phi_floor, delta_min;
     double tier_pref;      // 2 A / (mu0 Ms) with the plan's scalar A
     double a_tier;         // the plan's scalar A (per-cell Ms)
@@ -161,6 +162,15 @@ __device__ __forceinline__ void exchange_tier_vals(const FieldArgs &f, const boo
                                                    const double wt[6], double phi0, double pre, const S cc[3],
                                                    C h[3]);
 
+// Face link or distance of side s (-x, +x, -y, +y, -z, +z) at cell i: side s
+// reads array s ^ 1 of xp, xm, yp, ym, zp, zm.  W = float (mixed precision,
+// whose fields are FP32) reads the FP32 copy, W = double the FP64 arrays.
+template <typename W>
+__device__ __forceinline__ double tier_weight(const FieldArgs &f, int s, int64_t i) {
+    if constexpr (sizeof(W) == 4) return double(f.tier_w32[(int64_t)(s ^ 1) * f.n + i]);
+    else return f.tier_w[(int64_t)(s ^ 1) * f.n + i];
+}
+
 // Sub-cell boundary exchange at cell (x, y, z) = i, in the arithmetic type C
 // of the source.  T0 (tier_ex 1, native/backends/fdm/src/exchange_t0_fp64.cu:
 // 19-133): H = 2A / (mu0 Ms) / max(phi, phi_floor) sum f (m_j - m) / d^2 over
@@ -171,7 +181,7 @@ __device__ __forceinline__ void exchange_tier_vals(const FieldArgs &f, const boo
 // phi <= 0 or m = 0 get zero.  The active mask and the region table do not
 // enter.  cc, vzm, vzp: m at i and at i -+ nx ny (read only where that
 // neighbour is inside the grid).
-template <typename C, typename S>
+template <typename C, typename S, typename W = double>
 __device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__restrict__ m, int64_t i, int x, int y,
                                               int z, const S cc[3], const S vzm[3], const S vzp[3], C h[3]) {
     const int64_t n = f.n;
@@ -196,7 +206,7 @@ __device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__res
     }
     double wt[6];
 #pragma unroll
-    for (int s = 0; s < 6; ++s) wt[s] = f.tier_w[(int64_t)(s ^ 1) * n + i];
+    for (int s = 0; s < 6; ++s) wt[s] = tier_weight<W>(f, s, i);
     const double phi0 = f.phi[i];
     const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * f.ms_cell[i]) : f.tier_pref;
     exchange_tier_vals<C, S>(f, in, v, wt, phi0, pre, cc, h);
@@ -242,7 +252,7 @@ __device__ __forceinline__ void exchange_tier_vals(const FieldArgs &f, const boo
 }
 
 // exchange_tier at cell i with every neighbour loaded from memory.
-template <typename C, typename S>
+template <typename C, typename S, typename W = double>
 __device__ __forceinline__ void exchange_tier_at(const FieldArgs &f, const S *__restrict__ m, int64_t i, C h[3]) {
     const int64_t nxy = (int64_t)f.nx * f.ny;
     const uint32_t ui = (uint32_t)i;
@@ -258,7 +268,7 @@ __device__ __forceinline__ void exchange_tier_at(const FieldArgs &f, const S *__
         vzm[q] = m[q * f.n + zm];
         vzp[q] = m[q * f.n + zp];
     }
-    exchange_tier<C, S>(f, m, i, (int)x, (int)y, (int)z, cc, vzm, vzp, h);
+    exchange_tier<C, S, W>(f, m, i, (int)x, (int)y, (int)z, cc, vzm, vzp, h);
 }
 
 // Workgroup b runs on XCD b mod 8 (round-robin dispatch over the 8 XCDs of
@@ -368,7 +378,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
         if (f.exchange) {
             Src he[3];
             if (TIER) {
-                exchange_tier<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
+                exchange_tier<Src, Src, Real>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
             } else {
                 exchange_col<Src, Src>(f, a.src, i, x, y, z, cc, vzm, vzp, he);
             }
@@ -606,7 +616,7 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
             double wt[6], phi0 = 0.0, msc = 1.0;
             if constexpr (TIER) {
 #pragma unroll
-                for (int s = 0; s < 6; ++s) wt[s] = f.tier_w[(int64_t)(s ^ 1) * n + i];
+                for (int s = 0; s < 6; ++s) wt[s] = tier_weight<Real>(f, s, i);
                 phi0 = f.phi[i];
                 msc = *(f.ms_cell ? f.ms_cell + i : &kUnitStandIn);
             }
@@ -792,6 +802,7 @@ FieldArgs field_args(const Context &c) {
     f.tier_ex = c.tier_ex;
     f.phi = c.d_phi;
     f.tier_w = c.d_tier_w;
+    f.tier_w32 = c.d_tier_w32;
     f.phi_floor = c.phi_floor;
     f.delta_min = c.delta_min;
     f.tier_pref = 2.0 * c.a / (kMu0 * c.ms);

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(kBlock) observe_kernel(FieldArgs f, const Stat
         if (act) {
             if (f.exchange) {
                 State he[3];
-                if (TIER) exchange_tier_at<State, State>(f, m, i, he);
+                if (TIER) exchange_tier_at<State, State, Real>(f, m, i, he);
                 else exchange_at<State, State>(f, m, i, he);
                 for (int c = 0; c < 3; ++c) hex[c] = Real(he[c]);
             }
