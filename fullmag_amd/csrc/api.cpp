@@ -267,9 +267,16 @@ Context *create_impl(const fullmag_fdm_plan_desc *plan, const fullmag_fdm_ext_op
                 return c;
         }
         if (c->tier) {
-            if (!ctx_alloc(*c, (void **)&c->d_phi, n * 8, "volume fraction") ||
-                !ctx_upload(*c, c->d_phi, p.volume_fraction, n * 8, "hipMemcpy(volume fraction)", c->last_error))
+            if (c->mode == MODE_MIXED) {
+                std::vector<float> phi32(p.volume_fraction, p.volume_fraction + n);
+                if (!ctx_alloc(*c, (void **)&c->d_phi32, n * 4, "volume fraction") ||
+                    !ctx_upload(*c, c->d_phi32, phi32.data(), n * 4, "hipMemcpy(volume fraction)", c->last_error))
+                    return c;
+            } else if (!ctx_alloc(*c, (void **)&c->d_phi, n * 8, "volume fraction") ||
+                       !ctx_upload(*c, c->d_phi, p.volume_fraction, n * 8, "hipMemcpy(volume fraction)",
+                                   c->last_error)) {
                 return c;
+            }
             if (c->tier_ex) {
                 const double *w[6] = {p.face_link_xp, p.face_link_xm, p.face_link_yp,
                                       p.face_link_ym, p.face_link_zp, p.face_link_zm};

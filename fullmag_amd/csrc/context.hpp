@@ -132,6 +132,7 @@ struct Context {
     int tier_ex = 0;            // exchange stencil: 0 standard, 1 T0 face links, 2 T1 distances
     double phi_floor = 0.05, delta_min = 0.0;
     double *d_phi = nullptr;    // volume fraction; non-null: demag packs phi Ms m
+    float *d_phi32 = nullptr;   // the same in FP32, in its place under mixed precision
     double *d_tier_w = nullptr; // 6 n face links (T0) or distances (T1): xp, xm, yp, ym, zp, zm
     float *d_tier_w32 = nullptr; // the same in FP32, in their place under mixed precision
     // sparse demag correction, rows grouped by target cell: group g adds rows

@@ -43,6 +43,12 @@ constexpr int kPad = 1;         // LDS line padding (complex slots)
 // read itself needs no branch: an active-mask byte of 1 and a value of 1.
 __device__ const uint8_t kNoMask = 1;
 __device__ const double kNoArrayF64 = 1.0;
+__device__ const float kNoArrayF32 = 1.0f;
+template <typename T>
+__device__ __forceinline__ const T *no_array() {
+    if constexpr (sizeof(T) == 4) return &kNoArrayF32;
+    else return &kNoArrayF64;
+}
 constexpr int kMaxBlock = 512;  // lanes per workgroup for FFT kernels
 // largest padded z length done in registers (a lane pair per column holds
 // 3 x pz complex values)
@@ -171,10 +177,12 @@ struct MagSource {
     double ms;
     int nx, ny, nz;
     int64_t n;
-    const double *phi;  // boundary tiers: every cell packs phi Ms m, mask or not
+    // boundary tiers: every cell packs phi Ms m, mask or not (the volume
+    // fraction in the field's precision)
+    const Real *phi;
     __device__ Real operator()(int c, int z, int y, int x) const {
         const int64_t i = ((int64_t)z * ny + y) * nx + x;
-        if (phi) return Real(double(m[(int64_t)c * n + i]) * (phi[i] * (ms_cell ? ms_cell[i] : ms)));
+        if (phi) return Real(double(m[(int64_t)c * n + i]) * (double(phi[i]) * (ms_cell ? ms_cell[i] : ms)));
         if (mask && !mask[i]) return Real(0);
         const double w = ms_cell ? ms_cell[i] : ms;
         return Real(double(m[(int64_t)c * n + i]) * w);
@@ -192,7 +200,7 @@ struct MagSource {
     __device__ Real gather_at(int c, int z, int y, int x) const {
         const int64_t i = ((int64_t)z * ny + y) * nx + x;
         const double mv = double(m[(int64_t)c * n + i]);
-        const double pv = *(phi ? phi + i : &kNoArrayF64);
+        const double pv = double(*(phi ? phi + i : no_array<Real>()));
         const double cv = *(ms_cell ? ms_cell + i : &kNoArrayF64);
         const uint8_t kv = *(mask ? mask + i : &kNoMask);
         const double w = ms_cell ? cv : ms;
@@ -1525,11 +1533,20 @@ void launch_lines(Context &c, const LinePlan &lp, LineArgs<Real> la, size_t ce) 
     });
 }
 
+// The volume fraction of a tier backend in the field's precision (FP32
+// under mixed precision).
+template <typename Real>
+const Real *phi_array(const Context &c) {
+    if constexpr (sizeof(Real) == 4) return c.d_phi32;
+    else return c.d_phi;
+}
+
 template <typename Real, typename S>
 struct BoundaryCorrArgs {
     Real *out;
     const S *m;
-    const double *phi, *ms_cell;
+    const Real *phi;  // volume fraction in the field's precision
+    const double *ms_cell;
     double ms;
     int64_t n;
     const int32_t *cell;
@@ -1577,7 +1594,7 @@ __global__ void __launch_bounds__(256) boundary_corr_kernel(BoundaryCorrArgs<Rea
             for (int u = 0; u < U; ++u) {
                 ok[u] = ok[u] && si[u] >= 0;
                 const int64_t j = ok[u] ? si[u] : 0;
-                const double pv = a.phi[j];
+                const double pv = double(a.phi[j]);
                 const double cv = *(msp ? msp + j : &kNoArrayF64);
                 w[u] = pv * (msp ? cv : a.ms);
 #pragma unroll
@@ -1625,7 +1642,7 @@ void demag_transform_t(Context &c, const S *src, const XArgs<Real> &xa, dim3 xgr
     {
         const MagSource<Real, S> ms{src,    c.d_mask, weight_ms ? c.d_ms_cell : nullptr, weight_ms ? c.ms : 1.0,
                                     dp.nx,  dp.ny,    dp.nz,
-                                    (int64_t)c.n, weight_ms ? c.d_phi : nullptr};
+                                    (int64_t)c.n, weight_ms ? phi_array<Real>(c) : nullptr};
         Timed t(c, "demag_x_r2c");
         auto launch = [&](const auto &src_v) {
             using Src = std::decay_t<decltype(src_v)>;
@@ -1844,7 +1861,7 @@ void demag_eval_t(Context &c, const S *src, Real *out, bool weight_ms, int phase
         BoundaryCorrArgs<Real, S> ba{};
         ba.out = out;
         ba.m = src;
-        ba.phi = c.d_phi;
+        ba.phi = phi_array<Real>(c);
         ba.ms_cell = c.d_ms_cell;
         ba.ms = c.ms;
         ba.n = (int64_t)c.n;

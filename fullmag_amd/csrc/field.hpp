@@ -42,6 +42,7 @@ struct FieldArgs {
     // intersection distances) replaces the stencil above
     int tier_ex;
     const double *phi;     // volume fraction
+    const float *phi32;    // the same in FP32, in its place under mixed precision
     const double *tier_w;  // 6 n: xp, xm, yp, ym, zp, zm face links or distances
     const float *tier_w32; // the same in FP32, in their place under mixed precision
     double phi_floor, delta_min;
@@ -170,6 +171,12 @@ __device__ __forceinline__ double tier_weight(const FieldArgs &f, int s, int64_t
     if constexpr (sizeof(W) == 4) return double(f.tier_w32[(int64_t)(s ^ 1) * f.n + i]);
     else return f.tier_w[(int64_t)(s ^ 1) * f.n + i];
 }
+// Volume fraction at cell i, read like tier_weight.
+template <typename W>
+__device__ __forceinline__ double tier_phi(const FieldArgs &f, int64_t i) {
+    if constexpr (sizeof(W) == 4) return double(f.phi32[i]);
+    else return f.phi[i];
+}
 
 // Sub-cell boundary exchange at cell (x, y, z) = i, in the arithmetic type C
 // of the source.  T0 (tier_ex 1, native/backends/fdm/src/exchange_t0_fp64.cu:
@@ -207,7 +214,7 @@ __device__ __forceinline__ void exchange_tier(const FieldArgs &f, const S *__res
     double wt[6];
 #pragma unroll
     for (int s = 0; s < 6; ++s) wt[s] = tier_weight<W>(f, s, i);
-    const double phi0 = f.phi[i];
+    const double phi0 = tier_phi<W>(f, i);
     const double pre = f.ms_cell ? 2.0 * f.a_tier / (kMu0 * f.ms_cell[i]) : f.tier_pref;
     exchange_tier_vals<C, S>(f, in, v, wt, phi0, pre, cc, h);
 }
@@ -617,7 +624,7 @@ __device__ __forceinline__ void stage_uniform_body(const StageArgs<State, Real, 
             if constexpr (TIER) {
 #pragma unroll
                 for (int s = 0; s < 6; ++s) wt[s] = tier_weight<Real>(f, s, i);
-                phi0 = f.phi[i];
+                phi0 = tier_phi<Real>(f, i);
                 msc = *(f.ms_cell ? f.ms_cell + i : &kUnitStandIn);
             }
             const bool act = !MASK || kcur != 0;
@@ -801,6 +808,7 @@ FieldArgs field_args(const Context &c) {
     f.alpha = c.alpha;
     f.tier_ex = c.tier_ex;
     f.phi = c.d_phi;
+    f.phi32 = c.d_phi32;
     f.tier_w = c.d_tier_w;
     f.tier_w32 = c.d_tier_w32;
     f.phi_floor = c.phi_floor;
