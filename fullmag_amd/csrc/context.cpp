@@ -34,6 +34,8 @@ void ctx_free_all(Context &c) {
     c.allocations.clear();
     if (c.h_result) hipHostFree(c.h_result);
     c.h_result = nullptr;
+    if (c.h_partials) hipHostFree(c.h_partials);
+    c.h_partials = nullptr;
     for (auto &pe : c.pending) {
         hipEventDestroy(pe.a);
         hipEventDestroy(pe.b);

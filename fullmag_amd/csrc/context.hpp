@@ -179,7 +179,9 @@ struct Context {
     double *d_partials = nullptr;
     int max_partial_blocks = 0;
     double *d_result = nullptr;
-    double *h_result = nullptr;  // pinned, coherent (the finalize kernels store into it)
+    double *h_result = nullptr;  // pinned: the merged stats of the last stats pass
+    double *h_partials = nullptr;  // pinned, coherent: workgroup partials of the stats / error passes
+    unsigned stats_blocks = 0;     // workgroups of the last stats pass
 
     // adjoint work buffers (allocated by the first adjoint call)
     bool adj_ready = false;

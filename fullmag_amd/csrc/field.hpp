@@ -433,7 +433,7 @@ __device__ __forceinline__ void stage_cell(const StageArgs<State, Real, Src, Out
         // masked stencil (native/backends/fdm/src/reductions_fp64.cu:123-200
         // sums the pair differences without face links or distances)
         // a lane's column (at most one z chunk) is summed plainly; lanes and
-        // workgroups merge with compensation (block_reduce, finalize)
+        // workgroups merge with compensation (block_reduce, then the host)
         if (TIER && f.exchange && !f.region) {
             // the pair form of the standard masked stencil's energy
             // (native/backends/fdm/src/reductions_fp64.cu:150-190): over the

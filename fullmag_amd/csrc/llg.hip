@@ -22,26 +22,6 @@
 namespace fm {
 namespace {
 
-// Reduces per-block partials into out[0..nq); `out` is the pinned,
-// coherent host result buffer, read once the stream has passed this kernel.
-template <int MODE>
-__global__ void __launch_bounds__(kBlock) finalize_kernel(const double *partials, int nblocks, double *out) {
-    constexpr int nq = MODE == STAGE_STATS ? kNStat : 1;
-    double acc[kNStat];
-    for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
-        const double *p = partials + (int64_t)b * nq;
-        if (MODE == STAGE_STATS) {
-            for (int e = 0; e < 3; ++e) neu_merge(acc[2 * e], acc[2 * e + 1], p[2 * e], p[2 * e + 1]);
-            for (int q = 6; q < 9; ++q) acc[q] = fmax(acc[q], p[q]);
-        } else {
-            acc[0] = fmax(acc[0], p[0]);
-        }
-    }
-    block_reduce<MODE>(acc, out);
-    if (threadIdx.x == 0) __threadfence_system();
-}
-
 // out = normalise(m0 + dt sum_j a_j k_j), no field evaluation (DP45 FSAL
 // first stage, ABM3 predictor).
 template <typename State, typename Real, typename Out>
@@ -261,6 +241,7 @@ struct Driver {
         }
         a.anew = wnew;
         a.dt = dt;
+        a.partials = c.h_partials;
         const unsigned g = grid_reduce();
         {
             Timed t(c, "llg_error");
@@ -271,11 +252,11 @@ struct Driver {
                 hipLaunchKernelGGL((stage_kernel<State, Real, State, Real, STAGE_ERROR>), dim3(g), dim3(kBlock), 0,
                                    c.stream, a);
         }
-        hipLaunchKernelGGL((finalize_kernel<STAGE_ERROR>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
-                           c.h_result);
         hipStreamSynchronize(c.stream);
         timing_resolve(c);
-        return std::sqrt(c.h_result[0]);
+        double e = 0.0;
+        for (unsigned b = 0; b < g; ++b) e = std::fmax(e, c.h_partials[b]);
+        return std::sqrt(e);
     }
 
     // A stack layer's field couples to the other layers: it is evaluated for
@@ -300,6 +281,7 @@ struct Driver {
         a.out = out;
         a.anew = 1.0;
         a.dt = dt;
+        a.partials = c.h_partials;
         const unsigned g = grid_reduce();
         {
             Timed t(c, "llg_stats");
@@ -328,8 +310,29 @@ struct Driver {
                                    c.stream, a);
             }
         }
-        hipLaunchKernelGGL((finalize_kernel<STAGE_STATS>), dim3(1), dim3(kBlock), 0, c.stream, c.d_partials, (int)g,
-                           c.h_result);
+        c.stats_blocks = g;
+    }
+
+    // The stats pass's workgroups store their partials (compensated energy
+    // sums and maxima) in pinned host memory; once the stream has passed the
+    // pass, the host merges them in workgroup order into h_result (the
+    // merge of neu_merge), which spares the device a one-workgroup
+    // reduction launch per step.
+    void merge_stats() {
+        double acc[kNStat];
+        for (int q = 0; q < kNStat; ++q) acc[q] = 0.0;
+        const double *p = c.h_partials;
+        for (unsigned b = 0; b < c.stats_blocks; ++b, p += kNStat) {
+            for (int e = 0; e < 3; ++e) {
+                double &sum = acc[2 * e], &comp = acc[2 * e + 1];
+                const double v = p[2 * e], t = sum + v;
+                comp += std::fabs(sum) >= std::fabs(v) ? (sum - t) + v : (v - t) + sum;
+                sum = t;
+                comp += p[2 * e + 1];
+            }
+            for (int q = 6; q < kNStat; ++q) acc[q] = std::fmax(acc[q], p[q]);
+        }
+        for (int q = 0; q < kNStat; ++q) c.h_result[q] = acc[q];
     }
 
     // lib.rs:1217-1249.  With use_pre, a predictor left by the previous
@@ -555,6 +558,7 @@ struct Driver {
     double fill_pending_dt_ = 0.0;
 
     void fill(fullmag_fdm_step_stats *st, double dt, double suggested) {
+        merge_stats();
         const double *r = c.h_result;
         const double v = c.dx * c.dy * c.dz;
         const double w = c.has_ms_cell ? 1.0 : c.ms;
@@ -628,10 +632,14 @@ bool alloc_integrator_buffers(Context &c) {
     if (!ctx_alloc(c, (void **)&c.d_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double), "partials"))
         return false;
     if (!ctx_alloc(c, (void **)&c.d_result, 16 * sizeof(double), "result")) return false;
+    // coherent: the stats and error passes store their workgroup partials here
+    if (!hip_ok(hipHostMalloc((void **)&c.h_partials, (size_t)c.max_partial_blocks * kNStat * sizeof(double),
+                              hipHostMallocCoherent),
+                "hipHostMalloc(partials)", c.last_error))
+        return false;
     if (!hip_ok(hipEventCreateWithFlags(&c.ev_stats, hipEventDisableTiming), "hipEventCreate", c.last_error))
         return false;
-    // coherent: the finalize kernels store a step's results here directly
-    if (!hip_ok(hipHostMalloc((void **)&c.h_result, 16 * sizeof(double), hipHostMallocCoherent), "hipHostMalloc",
+    if (!hip_ok(hipHostMalloc((void **)&c.h_result, 16 * sizeof(double), hipHostMallocDefault), "hipHostMalloc",
                 c.last_error))
         return false;
     return true;
